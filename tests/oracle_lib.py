@@ -1,0 +1,114 @@
+"""ctypes bindings to the CPU oracle (oracle/liboracle.so) and, where it was built, the
+reference's own core compiled from its sources (oracle/_ref/libsmolar_ref.so).
+
+Test infrastructure only: used by tests/, __graft_entry__.smoke() and bench.py's
+CPU-baseline leg as the checker, never by the product.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libsmolar_ref.so")
+
+PD = C.POINTER(C.c_double)
+PPD = C.POINTER(PD)
+
+
+def ptr(a):
+    """Pointer to the data of a C-contiguous float64 array (None passes NULL)."""
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(PD)
+
+
+_SMOLAR_ARGS = [C.c_int, C.c_int, C.c_int, PD, PD, PD, C.c_double,
+                PD, PD, PD, PD, PD, PD, PPD, PPD, C.c_int, C.c_int]
+
+
+def _bind(lib):
+    if hasattr(lib, "orc_smolar"):
+        lib.orc_smolar.argtypes = _SMOLAR_ARGS
+        lib.orc_smolar.restype = C.c_int
+        for name in ("orc_frame_courant",):
+            getattr(lib, name).restype = None
+        lib.orc_per_step.argtypes = [C.c_int] * 3 + [PD] * 3 + [C.c_int, C.c_int, PD, PD, PD]
+        lib.orc_per_step.restype = C.c_int
+        lib.orc_per_run.argtypes = [C.c_int] * 3 + [PD] * 3 + [C.c_int, C.c_int, PD, PD, C.c_int]
+        lib.orc_per_run.restype = C.c_int
+        lib.orc_per_courant.argtypes = [C.c_int] * 3 + [PD] * 3 + [C.c_double] + [PD] * 9
+        lib.orc_per_courant.restype = None
+        lib.orc_per_donor.argtypes = [C.c_int] * 3 + [PD] * 5
+        lib.orc_per_antidiff.argtypes = [C.c_int] * 3 + [PD] * 7
+        lib.orc_per_limiter_beta.argtypes = [C.c_int] * 3 + [PD] * 7
+        lib.orc_per_limiter_apply.argtypes = [C.c_int] * 3 + [PD] * 5
+    if hasattr(lib, "smolar"):
+        lib.smolar.argtypes = _SMOLAR_ARGS
+        lib.smolar.restype = C.c_int
+    return lib
+
+
+_oracle = None
+_ref = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        _oracle = _bind(C.CDLL(ORACLE_SO))
+    return _oracle
+
+
+def have_ref():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        _ref = _bind(C.CDLL(REF_SO))
+    return _ref
+
+
+def call_smolar(fn, nx, ny, ns, dx, dy, ds, dt, u0, u1, v0, v1, w0, w1, c0, c1, order,
+                flow_type=1):
+    """Calls a smolar()-signature function on numpy buffers; returns (new_c0, new_c1)
+    as the arrays the output pointers point to (the reference permutes them)."""
+    p0 = ptr(c0)
+    p1 = ptr(c1)
+    pp0 = C.pointer(p0)
+    pp1 = C.pointer(p1)
+    rc = fn(nx, ny, ns, ptr(dx), ptr(dy), ptr(ds), dt, ptr(u0), ptr(u1), ptr(v0), ptr(v1),
+            ptr(w0), ptr(w1), pp0, pp1, flow_type, order)
+    a0 = C.addressof(pp0.contents.contents)
+    a1 = C.addressof(pp1.contents.contents)
+    by_addr = {c0.ctypes.data: c0, c1.ctypes.data: c1}
+    return rc, by_addr[a0], by_addr[a1]
+
+
+def per_step(nx, ny, nz, U, V, W, iord, nonosc, cin, q=None):
+    out = np.empty_like(cin)
+    rc = oracle().orc_per_step(nx, ny, nz, ptr(U), ptr(V), ptr(W), iord, int(nonosc),
+                               ptr(q), ptr(cin), ptr(out))
+    assert rc == 0
+    return out
+
+
+def per_run(nx, ny, nz, U, V, W, iord, nonosc, c, nsteps, q=None):
+    c = np.array(c, dtype=np.float64, copy=True, order="C")
+    rc = oracle().orc_per_run(nx, ny, nz, ptr(U), ptr(V), ptr(W), iord, int(nonosc),
+                              ptr(q), ptr(c), nsteps)
+    assert rc == 0
+    return c
+
+
+def per_courant(nx, ny, nz, dx, dy, dz, dt, u0, u1, v0, v1, w0, w1):
+    U = np.empty(nx * ny * nz)
+    V = np.empty_like(U)
+    W = np.empty_like(U)
+    oracle().orc_per_courant(nx, ny, nz, ptr(dx), ptr(dy), ptr(dz), dt, ptr(u0), ptr(u1),
+                             ptr(v0), ptr(v1), ptr(w0), ptr(w1), ptr(U), ptr(V), ptr(W))
+    return U, V, W
