@@ -1,0 +1,85 @@
+/* Device-resident MPDATA interface (extensions beside the drop-in smolar.h).
+ *
+ * No counterpart in the reference except where noted.  A grid handle keeps the face
+ * Courant numbers, the field and all scratch in HBM; steps are enqueued on the
+ * library's stream for the current device and run without host round trips.
+ *
+ * Boundary modes:
+ *   MPD_FRAME     the reference's semantics (core/src/smolar.c): the outer one-cell
+ *                 frame of the grid is held at its initial values and only the interior
+ *                 is advanced; arrays are nx*ny*nz including the frame.
+ *   MPD_PERIODIC  every axis wraps; every cell is advanced.  The donor and
+ *                 antidiffusive formulas are the reference's interior formulas.
+ * Options: iord >= 1 passes per step (1 = donor cell); nonoscillatory limiter
+ * (Smolarkiewicz & Grabowski 1990) on the corrective passes, PERIODIC only; a
+ * per-step emission increment q added after the last pass (c <- A(c) + q), PERIODIC
+ * only.
+ *
+ * Arrays are FP64, cell index k*nx*ny + j*nx + i.  PERIODIC face arrays use the same
+ * layout (U[i,j,k] is the x-face between cells i and i+1 mod nx).  FRAME face arrays
+ * use the reference's layouts (uu.c:122, vv.c:133, ww.c:299).
+ *
+ * Every function returns MPD_OK (0) or an MPD_ERR_* code; mpd_error_string() gives the
+ * calling thread's last message.  Nothing aborts the process.
+ * Link: -lsmolar_hip.
+ */
+#pragma once
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPD_OK 0
+#define MPD_ERR_ARG 1         /* invalid argument: sizes, NULL pointers, options */
+#define MPD_ERR_DEVICE 2      /* HIP runtime failure or no usable gfx950 device */
+#define MPD_ERR_NOMEM 3       /* device or host allocation failed */
+#define MPD_ERR_UNSUPPORTED 4 /* option combination not provided */
+#define MPD_ERR_NUMERIC 5     /* a health check found a non-finite or negative value */
+
+#define MPD_FRAME 0
+#define MPD_PERIODIC 1
+
+const char* mpd_error_string(void);
+/* Number of visible HIP devices (0 without a GPU); MPD_OK even when 0. */
+int mpd_device_count(int* n);
+
+/* ---------------------------------------------------------------- single grid --- */
+typedef struct mpd_grid_s* mpd_grid;
+
+int mpd_create(mpd_grid* g, int nx, int ny, int nz, int boundary, int iord, int nonosc);
+int mpd_destroy(mpd_grid g);
+/* Face Courant numbers from cell-centred winds at two time levels, spacings and dt,
+ * computed on the device (uu.c:132-134, vv.c:141-145, ww.c:305-311).  Host arrays. */
+int mpd_set_winds(mpd_grid g, const double* dx, const double* dy, const double* dz,
+                  double dt, const double* u0, const double* u1, const double* v0,
+                  const double* v1, const double* w0, const double* w1);
+/* Face Courant numbers given directly (host arrays in the boundary mode's layout). */
+int mpd_set_courant(mpd_grid g, const double* U, const double* V, const double* W);
+int mpd_get_courant(mpd_grid g, double* U, double* V, double* W);
+int mpd_set_field(mpd_grid g, const double* c);
+int mpd_get_field(mpd_grid g, double* c);
+/* FRAME mode: the previous pass's field of the last step (the reference's *c0 on
+ * return from smolar()). */
+int mpd_get_previous_field(mpd_grid g, double* c);
+/* Per-step emission increment (host array, nx*ny*nz), or NULL to clear. */
+int mpd_set_emission(mpd_grid g, const double* q);
+/* Enqueue nsteps steps; returns without waiting. */
+int mpd_step(mpd_grid g, int nsteps);
+int mpd_synchronize(mpd_grid g);
+/* Device address of the current field (valid until the next mpd_step). */
+int mpd_field_device_ptr(mpd_grid g, double** dptr);
+/* Runs nsteps steps bracketed by HIP events on the launch stream; *ms = elapsed time. */
+int mpd_step_timed(mpd_grid g, int nsteps, double* ms);
+/* Mass (sum of c), minimum and maximum of the field, reduced on the device: the
+ * failure sentinel for long runs (MPD_ERR_NUMERIC if any value is non-finite). */
+int mpd_field_stats(mpd_grid g, double* mass, double* cmin, double* cmax);
+/* Kernel-path selection for PERIODIC grids: 0 = automatic (fused), 1 = staged
+ * (one kernel per stage, intermediates in HBM), 2 = fused. */
+int mpd_set_path(mpd_grid g, int path);
+/* Tile shape of the fused kernel (0,0 = autotuned default). */
+int mpd_set_tile(mpd_grid g, int tile_x, int tile_y);
+
+#ifdef __cplusplus
+}
+#endif

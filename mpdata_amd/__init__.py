@@ -1,0 +1,248 @@
+"""Python host binding of libsmolar_hip.so (MPDATA on AMD Instinct MI355X).
+
+Mirrors the reference's C interface (smolar.h): ``smolar()`` with the same argument
+meaning and pointer contract, ``msmolar()``/``msmolar_gather()`` over a device-resident
+descriptor, plus the ``Grid`` class over the mpd_* extension interface (periodic
+boundaries, nonoscillatory limiter, emission, batched ensembles, adjoint).
+
+The shared library is the only compute path: importing this package on a machine
+without it raises, and every call goes through the C ABI.  There is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsmolar_hip.so")
+
+FRAME = 0
+PERIODIC = 1
+SOLENOIDAL = 0
+DIVERGENT = 1
+MSMOLAR_SERIAL, MSMOLAR_PTHREADS, MSMOLAR_OPENMP = 0, 1, 2
+
+PD = C.POINTER(C.c_double)
+PPD = C.POINTER(PD)
+PI = C.POINTER(C.c_int)
+
+ERRORS = {1: "MPD_ERR_ARG", 2: "MPD_ERR_DEVICE", 3: "MPD_ERR_NOMEM",
+          4: "MPD_ERR_UNSUPPORTED", 5: "MPD_ERR_NUMERIC"}
+
+
+class MpdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+# every exported function of include/*.h: name -> (restype, argtypes)
+_SMOLAR = [C.c_int, C.c_int, C.c_int, PD, PD, PD, C.c_double, PD, PD, PD, PD, PD, PD,
+           PPD, PPD, C.c_int, C.c_int]
+_MSMOLAR = [C.POINTER(C.c_void_p)] + _SMOLAR + [C.c_int] * 4
+SIGNATURES = {
+    # smolar.h
+    "smolar": (C.c_int, _SMOLAR),
+    "msmolar": (C.c_int, _MSMOLAR),
+    "msmolar_gather": (C.c_int, _MSMOLAR),
+    "msmolar_free": (C.c_int, [C.POINTER(C.c_void_p)]),
+    # timing.h
+    "smolar_get_timer_resolution": (None, [C.c_void_p]),
+    "smolar_get_time": (None, [C.c_void_p]),
+    "smolar_get_time_diff": (C.c_double, [C.c_void_p, C.c_void_p]),
+    "smolar_print_time_diff": (None, [C.c_void_p, C.c_void_p]),
+    "smolar_device_timer_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+    "smolar_device_timer_start": (C.c_int, [C.c_void_p]),
+    "smolar_device_timer_stop": (C.c_int, [C.c_void_p]),
+    "smolar_device_timer_elapsed_ms": (C.c_int, [C.c_void_p, PD]),
+    "smolar_device_timer_destroy": (C.c_int, [C.c_void_p]),
+    # mpdata.h
+    "mpd_error_string": (C.c_char_p, []),
+    "mpd_device_count": (C.c_int, [PI]),
+    "mpd_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, C.c_int,
+                             C.c_int, C.c_int]),
+    "mpd_destroy": (C.c_int, [C.c_void_p]),
+    "mpd_set_winds": (C.c_int, [C.c_void_p, PD, PD, PD, C.c_double, PD, PD, PD, PD, PD, PD]),
+    "mpd_set_courant": (C.c_int, [C.c_void_p, PD, PD, PD]),
+    "mpd_get_courant": (C.c_int, [C.c_void_p, PD, PD, PD]),
+    "mpd_set_field": (C.c_int, [C.c_void_p, PD]),
+    "mpd_get_field": (C.c_int, [C.c_void_p, PD]),
+    "mpd_get_previous_field": (C.c_int, [C.c_void_p, PD]),
+    "mpd_set_emission": (C.c_int, [C.c_void_p, PD]),
+    "mpd_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpd_synchronize": (C.c_int, [C.c_void_p]),
+    "mpd_field_device_ptr": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "mpd_step_timed": (C.c_int, [C.c_void_p, C.c_int, PD]),
+    "mpd_field_stats": (C.c_int, [C.c_void_p, PD, PD, PD]),
+    "mpd_set_path": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpd_set_tile": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded libsmolar_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} missing: build it with "
+                              "`python -c 'import __graft_entry__ as g; g.build()'`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise MpdError(rc, lib().mpd_error_string().decode())
+
+
+def _p(a):
+    if a is None:
+        return None
+    if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]):
+        raise TypeError("arrays must be C-contiguous float64 numpy arrays")
+    return a.ctypes.data_as(PD)
+
+
+def device_count():
+    n = C.c_int(0)
+    _check(lib().mpd_device_count(C.byref(n)))
+    return n.value
+
+
+def smolar(nx, ny, ns, dx, dy, ds, dt, u0, u1, v0, v1, w0, w1, c0, c1, flow_type=DIVERGENT,
+           order=3):
+    """One frame-mode step, reference smolar() semantics (smolar.h:45-48).  Returns
+    (c0, c1): the arrays the reference's *c0/*c1 point to on return -- c1 holds the
+    new field, c0 the previous pass's field (swapped for even order)."""
+    p0, p1 = PD(_p(c0).contents), PD(_p(c1).contents)
+    pp0, pp1 = C.pointer(p0), C.pointer(p1)
+    _check(lib().smolar(nx, ny, ns, _p(dx), _p(dy), _p(ds), float(dt), _p(u0), _p(u1),
+                        _p(v0), _p(v1), _p(w0), _p(w1), pp0, pp1, flow_type, order))
+    by_addr = {c0.ctypes.data: c0, c1.ctypes.data: c1}
+    return (by_addr[C.addressof(pp0.contents.contents)],
+            by_addr[C.addressof(pp1.contents.contents)])
+
+
+class MSmolar:
+    """msmolar()/msmolar_gather() (smolar.h:95-136) over a device-resident descriptor."""
+
+    def __init__(self):
+        self.desc = C.c_void_p(None)
+
+    def step(self, nx, ny, ns, dx, dy, ds, dt, u0, u1, v0, v1, w0, w1, c0, c1,
+             flow_type=DIVERGENT, order=3, sx=1, sy=1, ss=1, mode=MSMOLAR_SERIAL):
+        pp0, pp1 = C.pointer(PD(_p(c0).contents)), C.pointer(PD(_p(c1).contents))
+        _check(lib().msmolar(C.byref(self.desc), nx, ny, ns, _p(dx), _p(dy), _p(ds),
+                             float(dt), _p(u0), _p(u1), _p(v0), _p(v1), _p(w0), _p(w1),
+                             pp0, pp1, flow_type, order, sx, sy, ss, mode))
+        by_addr = {c0.ctypes.data: c0, c1.ctypes.data: c1}
+        return (by_addr[C.addressof(pp0.contents.contents)],
+                by_addr[C.addressof(pp1.contents.contents)])
+
+    def gather(self, c0, c1, nx=0, ny=0, ns=0):
+        pp0, pp1 = C.pointer(PD(_p(c0).contents)), C.pointer(PD(_p(c1).contents))
+        _check(lib().msmolar_gather(C.byref(self.desc), nx, ny, ns, None, None, None, 0.0,
+                                    None, None, None, None, None, None, pp0, pp1, 0, 0, 0,
+                                    0, 0, 0))
+
+    def close(self):
+        if self.desc:
+            lib().msmolar_free(C.byref(self.desc))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Grid:
+    """A device-resident advection instance (mpd_grid)."""
+
+    def __init__(self, nx, ny, nz, boundary=PERIODIC, iord=2, nonosc=True):
+        self.shape = (nx, ny, nz)
+        self.n = nx * ny * nz
+        self.boundary = boundary
+        self.h = C.c_void_p(None)
+        _check(lib().mpd_create(C.byref(self.h), nx, ny, nz, boundary, iord, int(nonosc)))
+
+    def face_sizes(self):
+        nx, ny, nz = self.shape
+        if self.boundary == FRAME:
+            return ((nx - 1) * (ny - 2) * (nz - 2), (nx - 2) * (ny - 1) * (nz - 2),
+                    (nx - 2) * (ny - 2) * (nz - 1))
+        return (self.n,) * 3
+
+    def set_winds(self, dx, dy, dz, dt, u0, u1, v0, v1, w0, w1):
+        _check(lib().mpd_set_winds(self.h, _p(dx), _p(dy), _p(dz), float(dt), _p(u0),
+                                   _p(u1), _p(v0), _p(v1), _p(w0), _p(w1)))
+
+    def set_courant(self, U, V, W):
+        _check(lib().mpd_set_courant(self.h, _p(U), _p(V), _p(W)))
+
+    def get_courant(self):
+        out = [np.empty(s) for s in self.face_sizes()]
+        _check(lib().mpd_get_courant(self.h, *(_p(a) for a in out)))
+        return out
+
+    def set_field(self, c):
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        assert c.size == self.n
+        _check(lib().mpd_set_field(self.h, _p(c)))
+
+    def field(self):
+        out = np.empty(self.n)
+        _check(lib().mpd_get_field(self.h, _p(out)))
+        return out
+
+    def previous_field(self):
+        out = np.empty(self.n)
+        _check(lib().mpd_get_previous_field(self.h, _p(out)))
+        return out
+
+    def set_emission(self, q):
+        if q is None:
+            _check(lib().mpd_set_emission(self.h, None))
+        else:
+            q = np.ascontiguousarray(q, dtype=np.float64)
+            _check(lib().mpd_set_emission(self.h, _p(q)))
+
+    def step(self, nsteps=1):
+        _check(lib().mpd_step(self.h, nsteps))
+
+    def step_timed(self, nsteps):
+        ms = C.c_double(0)
+        _check(lib().mpd_step_timed(self.h, nsteps, C.byref(ms)))
+        return ms.value
+
+    def synchronize(self):
+        _check(lib().mpd_synchronize(self.h))
+
+    def stats(self):
+        m, lo, hi = C.c_double(), C.c_double(), C.c_double()
+        _check(lib().mpd_field_stats(self.h, C.byref(m), C.byref(lo), C.byref(hi)))
+        return m.value, lo.value, hi.value
+
+    def set_path(self, path):
+        _check(lib().mpd_set_path(self.h, path))
+
+    def set_tile(self, tx, ty):
+        _check(lib().mpd_set_tile(self.h, tx, ty))
+
+    def close(self):
+        if self.h:
+            lib().mpd_destroy(self.h)
+            self.h = C.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

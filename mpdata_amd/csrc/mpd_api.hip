@@ -1,0 +1,427 @@
+// C ABI of the library: the drop-in smolar()/msmolar()/msmolar_gather() of the
+// reference (smolar.h:45-48, 95-99, 132-136) and the device-resident mpd_* interface.
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/smolar.h"
+#include "mpd_grid.hpp"
+
+namespace mpd {
+
+int periodic_staged_step(mpd_grid_s* g);
+int periodic_fused_step(mpd_grid_s* g);
+bool periodic_fused_supported(const mpd_grid_s* g);
+
+int grid_init(mpd_grid_s* g, int nx, int ny, int nz, int boundary, int iord, int nonosc)
+{
+    if (boundary != MPD_FRAME && boundary != MPD_PERIODIC)
+        return fail(MPD_ERR_ARG, "boundary must be MPD_FRAME or MPD_PERIODIC");
+    if (iord < 1 || iord > 64) return fail(MPD_ERR_ARG, "iord %d outside 1..64", iord);
+    if (boundary == MPD_FRAME && (nx < 3 || ny < 3 || nz < 3))
+        return fail(MPD_ERR_ARG, "frame grid %dx%dx%d: every dimension must be >= 3", nx, ny,
+                    nz);
+    if (nx < 1 || ny < 1 || nz < 1)
+        return fail(MPD_ERR_ARG, "grid %dx%dx%d: dimensions must be >= 1", nx, ny, nz);
+    if ((size_t)nx * ny * nz > ((size_t)1 << 31) - 1)
+        return fail(MPD_ERR_ARG, "grid of %zu cells exceeds 2^31-1", (size_t)nx * ny * nz);
+    if (nonosc && boundary != MPD_PERIODIC)
+        return fail(MPD_ERR_UNSUPPORTED, "the nonoscillatory limiter needs MPD_PERIODIC");
+    int rc = ensure_device();
+    if (rc) return rc;
+    g->nx = nx;
+    g->ny = ny;
+    g->nz = nz;
+    g->boundary = boundary;
+    g->iord = iord;
+    g->nonosc = nonosc ? 1 : 0;
+    g->ncell = (size_t)nx * ny * nz;
+    if (boundary == MPD_FRAME) {
+        g->nface[0] = (size_t)(nx - 1) * (ny - 2) * (nz - 2);
+        g->nface[1] = (size_t)(nx - 2) * (ny - 1) * (nz - 2);
+        g->nface[2] = (size_t)(nx - 2) * (ny - 2) * (nz - 1);
+    } else {
+        g->nface[0] = g->nface[1] = g->nface[2] = g->ncell;
+    }
+    for (int d = 0; d < 3; d++)
+        if ((rc = g->face[d].alloc(g->nface[d])) != MPD_OK) return rc;
+    for (int b = 0; b < 2; b++)
+        if ((rc = g->c[b].alloc(g->ncell)) != MPD_OK) return rc;
+    g->cur = 0;
+    return MPD_OK;
+}
+
+static int step_once(mpd_grid_s* g)
+{
+    if (g->boundary == MPD_FRAME) return frame_step(g);
+    return periodic_step(g);
+}
+
+int periodic_step(mpd_grid_s* g)
+{
+    bool fused = (g->path == 2) || (g->path == 0 && periodic_fused_supported(g));
+    if (fused) {
+        if (!periodic_fused_supported(g))
+            return fail(MPD_ERR_UNSUPPORTED, "fused path does not support this grid");
+        return periodic_fused_step(g);
+    }
+    return periodic_staged_step(g);
+}
+
+static int h2d(double* d, const double* h, size_t n)
+{
+    MPD_HIP(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, stream()));
+    return MPD_OK;
+}
+
+static int d2h(double* h, const double* d, size_t n)
+{
+    MPD_HIP(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    return MPD_OK;
+}
+
+static int sync() { MPD_HIP(hipStreamSynchronize(stream())); return MPD_OK; }
+
+// Uploads spacings and the six wind arrays (identical host pointers are uploaded
+// once) and computes the face Courant numbers of g.
+static int winds_to_courant(mpd_grid_s* g, const double* dx, const double* dy,
+                            const double* dz, double dt, const double* const w[6],
+                            DevBuf sp[3], DevBuf wd[6])
+{
+    int rc;
+    const int dims[3] = {g->nx, g->ny, g->nz};
+    const double* hs[3] = {dx, dy, dz};
+    for (int d = 0; d < 3; d++) {
+        if ((rc = sp[d].alloc(dims[d])) != MPD_OK) return rc;
+        if ((rc = h2d(sp[d].p, hs[d], dims[d])) != MPD_OK) return rc;
+    }
+    const double* dev[6];
+    for (int a = 0; a < 6; a++) {
+        int same = -1;
+        for (int b = 0; b < a; b++)
+            if (w[b] == w[a]) same = b;
+        if (same >= 0) {
+            dev[a] = dev[same];
+            continue;
+        }
+        if ((rc = wd[a].alloc(g->ncell)) != MPD_OK) return rc;
+        if ((rc = h2d(wd[a].p, w[a], g->ncell)) != MPD_OK) return rc;
+        dev[a] = wd[a].p;
+    }
+    rc = courant_from_winds(g, sp[0].p, sp[1].p, sp[2].p, dt, dev[0], dev[1], dev[2], dev[3],
+                            dev[4], dev[5]);
+    if (rc) return rc;
+    g->has_winds = true;
+    return MPD_OK;
+}
+
+}  // namespace mpd
+
+using namespace mpd;
+
+// ------------------------------------------------------------------ drop-in ABI ---
+
+namespace {
+// Per-thread device state of smolar(): reused while the grid shape and order repeat,
+// so repeated calls do not allocate (the reference mallocs per call, uu.c:115).
+struct SmolarCache {
+    mpd_grid_s g;
+    DevBuf sp[3], wd[6];
+};
+thread_local SmolarCache* t_cache = nullptr;
+}  // namespace
+
+extern "C" int smolar(int nx, int ny, int ns, double* dx, double* dy, double* ds,
+                      const double dt, double* u0, double* u1, double* v0, double* v1,
+                      double* w0, double* w1, double** c0, double** c1, int flow_type,
+                      int order)
+{
+    (void)flow_type;  // never read by the reference (uu.c:111)
+    clear_error();
+    if (!dx || !dy || !ds || !u0 || !u1 || !v0 || !v1 || !w0 || !w1 || !c0 || !c1 || !*c0 ||
+        !*c1)
+        return fail(MPD_ERR_ARG, "smolar: NULL array argument");
+    if (order < 1) return fail(MPD_ERR_ARG, "smolar: order %d < 1", order);
+    if (!t_cache) t_cache = new SmolarCache;
+    mpd_grid_s& g = t_cache->g;
+    int rc;
+    if (g.nx != nx || g.ny != ny || g.nz != ns || g.iord != order || !g.c[0].p) {
+        t_cache->g.~mpd_grid_s();
+        new (&t_cache->g) mpd_grid_s();
+        if ((rc = grid_init(&g, nx, ny, ns, MPD_FRAME, order, 0)) != MPD_OK) return rc;
+    }
+    const double* w[6] = {u0, u1, v0, v1, w0, w1};
+    if ((rc = winds_to_courant(&g, dx, dy, ds, dt, w, t_cache->sp, t_cache->wd))) return rc;
+    double* h0 = *c0;
+    double* h1 = *c1;
+    if ((rc = h2d(g.c[0].p, h0, g.ncell))) return rc;
+    g.cur = 0;
+    if ((rc = frame_step(&g))) return rc;
+    // device c[0] mirrors the caller's first buffer, c[1] the second (smolar.c:44-58)
+    if ((rc = d2h(h1, g.c[1].p, g.ncell))) return rc;
+    if (order > 1 && (rc = d2h(h0, g.c[0].p, g.ncell))) return rc;
+    if ((rc = sync())) return rc;
+    if (order % 2 == 0) {
+        *c1 = h0;
+        *c0 = h1;
+    }
+    return MPD_OK;
+}
+
+extern "C" int msmolar(void** desc, int nx, int ny, int ns, double* dx, double* dy,
+                       double* ds, const double dt, double* u0, double* u1, double* v0,
+                       double* v1, double* w0, double* w1, double** c0, double** c1,
+                       int flow_type, int order, int sx, int sy, int ss, int mode)
+{
+    (void)flow_type;
+    (void)sx;
+    (void)sy;
+    (void)ss;
+    (void)mode;
+    clear_error();
+    if (!desc || !c0 || !c1) return fail(MPD_ERR_ARG, "msmolar: NULL argument");
+    int rc;
+    if (!*desc) {  // msmolar.c:25-32: the first call captures every array
+        if (!dx || !dy || !ds || !u0 || !u1 || !v0 || !v1 || !w0 || !w1 || !*c0)
+            return fail(MPD_ERR_ARG, "msmolar: NULL array argument on the first call");
+        mpd_grid_s* g = new mpd_grid_s;
+        if ((rc = grid_init(g, nx, ny, ns, MPD_FRAME, order, 0)) != MPD_OK) {
+            delete g;
+            return rc;
+        }
+        DevBuf sp[3], wd[6];
+        const double* w[6] = {u0, u1, v0, v1, w0, w1};
+        if ((rc = winds_to_courant(g, dx, dy, ds, dt, w, sp, wd)) ||
+            (rc = h2d(g->c[0].p, *c0, g->ncell)) || (rc = sync())) {
+            delete g;
+            return rc;
+        }
+        g->has_field = true;
+        *desc = g;
+    }
+    mpd_grid_s* g = (mpd_grid_s*)*desc;
+    if (g->nx != nx || g->ny != ny || g->nz != ns || g->iord != order)
+        return fail(MPD_ERR_ARG, "msmolar: grid or order differs from the first call");
+    if ((rc = frame_step(g))) return rc;
+    if (order % 2 == 0) {  // msmolar.c:183-187
+        double* t = *c0;
+        *c0 = *c1;
+        *c1 = t;
+    }
+    return MPD_OK;
+}
+
+extern "C" int msmolar_gather(void** desc, int nx, int ny, int ns, double* dx, double* dy,
+                              double* ds, const double dt, double* u0, double* u1,
+                              double* v0, double* v1, double* w0, double* w1, double** c0,
+                              double** c1, int flow_type, int order, int sx, int sy, int ss,
+                              int mode)
+{
+    (void)dx; (void)dy; (void)ds; (void)dt; (void)u0; (void)u1; (void)v0; (void)v1;
+    (void)w0; (void)w1; (void)flow_type; (void)order; (void)sx; (void)sy; (void)ss;
+    (void)mode; (void)nx; (void)ny; (void)ns;
+    clear_error();
+    if (!desc || !*desc || !c0 || !c1 || !*c0 || !*c1)
+        return fail(MPD_ERR_ARG, "msmolar_gather: NULL argument or no state");
+    mpd_grid_s* g = (mpd_grid_s*)*desc;
+    int rc;
+    if ((rc = d2h(*c1, g->c[g->cur].p, g->ncell)) ||
+        (rc = d2h(*c0, g->c[1 - g->cur].p, g->ncell)) || (rc = sync()))
+        return rc;
+    return MPD_OK;
+}
+
+extern "C" int msmolar_free(void** desc)
+{
+    if (desc && *desc) {
+        delete (mpd_grid_s*)*desc;
+        *desc = nullptr;
+    }
+    return MPD_OK;
+}
+
+// -------------------------------------------------------------- mpd_* interface ---
+
+#define GRID_OR_FAIL(g)                                                  \
+    do {                                                                 \
+        clear_error();                                                   \
+        if (!(g)) return fail(MPD_ERR_ARG, "%s: NULL grid", __func__);   \
+    } while (0)
+
+extern "C" int mpd_create(mpd_grid* out, int nx, int ny, int nz, int boundary, int iord,
+                          int nonosc)
+{
+    clear_error();
+    if (!out) return fail(MPD_ERR_ARG, "mpd_create: NULL handle pointer");
+    *out = nullptr;
+    mpd_grid_s* g = new mpd_grid_s;
+    int rc = grid_init(g, nx, ny, nz, boundary, iord, nonosc);
+    if (rc) {
+        delete g;
+        return rc;
+    }
+    *out = g;
+    return MPD_OK;
+}
+
+extern "C" int mpd_destroy(mpd_grid g)
+{
+    if (g) {
+        (void)hipStreamSynchronize(stream());
+        delete g;
+    }
+    return MPD_OK;
+}
+
+extern "C" int mpd_set_winds(mpd_grid g, const double* dx, const double* dy, const double* dz,
+                             double dt, const double* u0, const double* u1, const double* v0,
+                             const double* v1, const double* w0, const double* w1)
+{
+    GRID_OR_FAIL(g);
+    if (!dx || !dy || !dz || !u0 || !u1 || !v0 || !v1 || !w0 || !w1)
+        return fail(MPD_ERR_ARG, "mpd_set_winds: NULL array");
+    DevBuf sp[3], wd[6];
+    const double* w[6] = {u0, u1, v0, v1, w0, w1};
+    int rc = winds_to_courant(g, dx, dy, dz, dt, w, sp, wd);
+    if (rc) return rc;
+    return sync();  // the temporaries are freed on return
+}
+
+extern "C" int mpd_set_courant(mpd_grid g, const double* U, const double* V, const double* W)
+{
+    GRID_OR_FAIL(g);
+    if (!U || !V || !W) return fail(MPD_ERR_ARG, "mpd_set_courant: NULL array");
+    const double* h[3] = {U, V, W};
+    int rc;
+    for (int d = 0; d < 3; d++)
+        if ((rc = h2d(g->face[d].p, h[d], g->nface[d]))) return rc;
+    g->has_winds = true;
+    return sync();
+}
+
+extern "C" int mpd_get_courant(mpd_grid g, double* U, double* V, double* W)
+{
+    GRID_OR_FAIL(g);
+    if (!U || !V || !W) return fail(MPD_ERR_ARG, "mpd_get_courant: NULL array");
+    double* h[3] = {U, V, W};
+    int rc;
+    for (int d = 0; d < 3; d++)
+        if ((rc = d2h(h[d], g->face[d].p, g->nface[d]))) return rc;
+    return sync();
+}
+
+extern "C" int mpd_set_field(mpd_grid g, const double* c)
+{
+    GRID_OR_FAIL(g);
+    if (!c) return fail(MPD_ERR_ARG, "mpd_set_field: NULL array");
+    int rc = h2d(g->c[g->cur].p, c, g->ncell);
+    if (rc) return rc;
+    g->has_field = true;
+    return sync();
+}
+
+extern "C" int mpd_get_field(mpd_grid g, double* c)
+{
+    GRID_OR_FAIL(g);
+    if (!c) return fail(MPD_ERR_ARG, "mpd_get_field: NULL array");
+    int rc = d2h(c, g->c[g->cur].p, g->ncell);
+    if (rc) return rc;
+    return sync();
+}
+
+extern "C" int mpd_get_previous_field(mpd_grid g, double* c)
+{
+    GRID_OR_FAIL(g);
+    if (!c) return fail(MPD_ERR_ARG, "mpd_get_previous_field: NULL array");
+    int rc = d2h(c, g->c[1 - g->cur].p, g->ncell);
+    if (rc) return rc;
+    return sync();
+}
+
+extern "C" int mpd_set_emission(mpd_grid g, const double* q)
+{
+    GRID_OR_FAIL(g);
+    if (!q) {
+        g->has_q = false;
+        return MPD_OK;
+    }
+    if (g->boundary != MPD_PERIODIC)
+        return fail(MPD_ERR_UNSUPPORTED, "emission needs MPD_PERIODIC");
+    int rc = g->q.alloc(g->ncell);
+    if (rc) return rc;
+    if ((rc = h2d(g->q.p, q, g->ncell))) return rc;
+    g->has_q = true;
+    return sync();
+}
+
+extern "C" int mpd_step(mpd_grid g, int nsteps)
+{
+    GRID_OR_FAIL(g);
+    if (nsteps < 0) return fail(MPD_ERR_ARG, "mpd_step: nsteps %d < 0", nsteps);
+    if (!g->has_winds || !g->has_field)
+        return fail(MPD_ERR_ARG, "mpd_step: set the winds (or Courant numbers) and the field first");
+    for (int s = 0; s < nsteps; s++) {
+        int rc = step_once(g);
+        if (rc) return rc;
+    }
+    return MPD_OK;
+}
+
+extern "C" int mpd_synchronize(mpd_grid g)
+{
+    GRID_OR_FAIL(g);
+    return sync();
+}
+
+extern "C" int mpd_field_device_ptr(mpd_grid g, double** dptr)
+{
+    GRID_OR_FAIL(g);
+    if (!dptr) return fail(MPD_ERR_ARG, "mpd_field_device_ptr: NULL");
+    *dptr = g->c[g->cur].p;
+    return MPD_OK;
+}
+
+extern "C" int mpd_step_timed(mpd_grid g, int nsteps, double* ms)
+{
+    GRID_OR_FAIL(g);
+    if (!ms) return fail(MPD_ERR_ARG, "mpd_step_timed: NULL");
+    hipEvent_t a, b;
+    MPD_HIP(hipEventCreate(&a));
+    MPD_HIP(hipEventCreate(&b));
+    int rc = MPD_OK;
+    hipError_t e = hipEventRecord(a, stream());
+    if (e == hipSuccess) rc = mpd_step(g, nsteps);
+    if (e == hipSuccess && rc == MPD_OK) e = hipEventRecord(b, stream());
+    if (e == hipSuccess && rc == MPD_OK) e = hipEventSynchronize(b);
+    float f = 0.f;
+    if (e == hipSuccess && rc == MPD_OK) e = hipEventElapsedTime(&f, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "mpd_step_timed event");
+    *ms = f;
+    return MPD_OK;
+}
+
+extern "C" int mpd_field_stats(mpd_grid g, double* mass, double* cmin, double* cmax)
+{
+    GRID_OR_FAIL(g);
+    return field_stats(g, mass, cmin, cmax);
+}
+
+extern "C" int mpd_set_path(mpd_grid g, int path)
+{
+    GRID_OR_FAIL(g);
+    if (path < 0 || path > 2) return fail(MPD_ERR_ARG, "mpd_set_path: %d not in 0..2", path);
+    g->path = path;
+    return MPD_OK;
+}
+
+extern "C" int mpd_set_tile(mpd_grid g, int tile_x, int tile_y)
+{
+    GRID_OR_FAIL(g);
+    if (tile_x < 0 || tile_y < 0) return fail(MPD_ERR_ARG, "mpd_set_tile: negative tile");
+    g->tile_x = tile_x;
+    g->tile_y = tile_y;
+    return MPD_OK;
+}

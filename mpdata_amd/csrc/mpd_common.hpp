@@ -1,0 +1,45 @@
+// Host-side helpers shared by the library's translation units: error reporting,
+// the per-device launch stream and a checked-allocation helper.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdio>
+#include <string>
+
+#include "../../include/mpdata.h"
+
+namespace mpd {
+
+// Records the calling thread's last error message and returns the code.
+int fail(int code, const char* fmt, ...);
+void clear_error();
+
+// HIP status -> MPD code with a message naming the call site.
+int hip_fail(hipError_t e, const char* what);
+
+#define MPD_HIP(call)                                                   \
+    do {                                                                \
+        hipError_t e_ = (call);                                         \
+        if (e_ != hipSuccess) return ::mpd::hip_fail(e_, #call);        \
+    } while (0)
+
+// Ensures a usable device is current; returns MPD_OK or MPD_ERR_DEVICE.
+int ensure_device();
+// The stream every kernel of this library is launched on (one per device, default
+// priority, created on first use).
+hipStream_t stream();
+
+// Device buffer owned by RAII.
+struct DevBuf {
+    double* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    int alloc(size_t count);  // no-op when already count elements
+    void release();
+};
+
+}  // namespace mpd

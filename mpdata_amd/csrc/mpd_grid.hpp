@@ -1,0 +1,41 @@
+// The device-resident grid object behind mpd_grid, smolar() and msmolar().
+#pragma once
+#include "mpd_common.hpp"
+
+struct mpd_grid_s {
+    int nx = 0, ny = 0, nz = 0;
+    int boundary = MPD_PERIODIC;
+    int iord = 2;
+    int nonosc = 0;
+    int path = 0;  // 0 auto, 1 staged, 2 fused
+    int tile_x = 0, tile_y = 0;
+    size_t ncell = 0;
+    size_t nface[3] = {0, 0, 0};  // elements per face family (layout of the mode)
+
+    mpd::DevBuf face[3];  // Courant numbers U, V, W
+    mpd::DevBuf c[2];     // field ping-pong; c[cur] is the newest
+    int cur = 0;
+    mpd::DevBuf q;        // emission increment
+    bool has_q = false;
+    bool has_winds = false, has_field = false;
+
+    // scratch of the staged paths
+    mpd::DevBuf fa[3], fb[3];  // antidiffusive velocities, ping-pong
+    mpd::DevBuf cp[2];         // intermediate pass fields
+    mpd::DevBuf bup, bdn;      // limiter ratios
+    mpd::DevBuf red;           // reduction partials
+};
+
+namespace mpd {
+
+int grid_init(mpd_grid_s* g, int nx, int ny, int nz, int boundary, int iord, int nonosc);
+// Face Courant numbers from winds already on the device.
+int courant_from_winds(mpd_grid_s* g, const double* dx, const double* dy, const double* dz,
+                       double dt, const double* u0, const double* u1, const double* v0,
+                       const double* v1, const double* w0, const double* w1);
+// One step of the grid's boundary mode; advances g->cur.
+int frame_step(mpd_grid_s* g);
+int periodic_step(mpd_grid_s* g);
+int field_stats(mpd_grid_s* g, double* mass, double* cmin, double* cmax);
+
+}  // namespace mpd

@@ -1,0 +1,202 @@
+"""GPU parity: the HIP path through the C ABI against the oracle and the reference's
+golden fixtures.
+
+Frame mode (the drop-in smolar()/msmolar()) must equal the reference bit for bit.
+Periodic mode (both kernel paths) must agree with the oracle to a normwise relative
+error <= 1e-12 per step and after N steps (max|gpu - oracle| / max|oracle|); the staged
+path is compiled unfused and is expected to be bit-identical.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import mpdata_amd as M
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+TOL = 1e-12
+
+
+def relerr(got, want):
+    return float(np.abs(got - want).max() / max(np.abs(want).max(), 1e-300))
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if M.device_count() < 1:
+        pytest.skip("no GPU")
+
+
+# ------------------------------------------------------------------ frame mode ---
+
+@pytest.mark.parametrize("fixture,orders", [("frame_3d.npz", (1, 2, 3, 4)),
+                                            ("frame_2d.npz", (1, 2, 3))])
+def test_smolar_matches_reference_fixtures(fixture, orders):
+    d = dict(np.load(os.path.join(GOLD, fixture)))
+    nx, ny, ns = (int(x) for x in d["dims"])
+    print(f"case {fixture}", flush=True)
+    for order in orders:
+        c0 = d["c"].copy()
+        c1 = np.zeros_like(c0)
+        for s in range(1, 13):
+            a0, a1 = M.smolar(nx, ny, ns, d["dx"], d["dy"], d["ds"], float(d["dt"]), d["u0"],
+                              d["u1"], d["v0"], d["v1"], d["w0"], d["w1"], c0, c1, order=order)
+            c0, c1 = a1, a0
+            if s in (1, 12):
+                np.testing.assert_array_equal(c0, d[f"out_o{order}_s{s}"],
+                                              err_msg=f"order {order} step {s}")
+
+
+def test_smolar_pointer_contract_and_previous_pass():
+    d = dict(np.load(os.path.join(GOLD, "frame_3d.npz")))
+    nx, ny, ns = (int(x) for x in d["dims"])
+    args = (d["dx"], d["dy"], d["ds"], float(d["dt"]), d["u0"], d["u1"], d["v0"], d["v1"],
+            d["w0"], d["w1"])
+    for order in (1, 2, 3):
+        g0, g1 = d["c"].copy(), np.full(d["c"].size, 5.0)
+        r0, r1 = M.smolar(nx, ny, ns, *args, g0, g1, order=order)
+        o0, o1 = d["c"].copy(), np.full(d["c"].size, 5.0)
+        _, q0, q1 = O.call_smolar(O.oracle().orc_smolar, nx, ny, ns, *args, o0, o1, order)
+        assert (r1 is g1) == (q1 is o1)
+        np.testing.assert_array_equal(r1, q1)
+        np.testing.assert_array_equal(r0, q0)  # previous pass's field, as the reference
+
+
+def test_ring2d_trajectory_matches_reference():
+    d = dict(np.load(os.path.join(GOLD, "ring2d_32.npz")))
+    n, _, m = (int(x) for x in d["dims"])
+    one = np.ones(n)
+    c0, c1 = d["c"].copy(), np.zeros_like(d["c"])
+    for s in range(1, 51):
+        a0, a1 = M.smolar(n, n, m, one, one, np.ones(m), float(d["dt"]), d["u"], d["u"],
+                          d["v"], d["v"], d["w"], d["w"], c0, c1, order=3)
+        c0, c1 = a1, a0
+        if s in (1, 10, 50):
+            np.testing.assert_array_equal(c0, d[f"out_s{s}"])
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_msmolar_matches_reference_fixture(order):
+    d = dict(np.load(os.path.join(GOLD, "frame_3d.npz")))
+    nx, ny, ns = (int(x) for x in d["dims"])
+    ms = M.MSmolar()
+    c0, c1 = d["c"].copy(), np.zeros_like(d["c"])
+    for _ in range(12):
+        c0, c1 = ms.step(nx, ny, ns, d["dx"], d["dy"], d["ds"], float(d["dt"]), d["u0"],
+                         d["u1"], d["v0"], d["v1"], d["w0"], d["w1"], c0, c1, order=order)
+    ms.gather(c0, c1)
+    ms.close()
+    np.testing.assert_array_equal(c1, d[f"out_o{order}_s12"])
+
+
+def test_frame_grid_handle_matches_oracle():
+    rng = np.random.default_rng(11)
+    nx, ny, ns = 20, 17, 12
+    n = nx * ny * ns
+    sp = [rng.uniform(0.9, 1.1, k) for k in (nx, ny, ns)]
+    winds = [rng.uniform(-0.3, 0.3, n) for _ in range(6)]
+    c = rng.uniform(0, 1, n)
+    g = M.Grid(nx, ny, ns, M.FRAME, iord=3, nonosc=False)
+    g.set_winds(*sp, 0.8, *winds)
+    g.set_field(c)
+    g.step(5)
+    got = g.field()
+    c0, c1 = c.copy(), np.zeros(n)
+    for _ in range(5):
+        _, a0, a1 = O.call_smolar(O.oracle().orc_smolar, nx, ny, ns, *sp, 0.8, *winds, c0, c1, 3)
+        c0, c1 = a1, a0
+    np.testing.assert_array_equal(got, c0)
+
+
+# --------------------------------------------------------------- periodic mode ---
+
+def per_case(nx, ny, nz, seed, cmax=0.2):
+    rng = np.random.default_rng(seed)
+    n = nx * ny * nz
+    U, V, W = (rng.uniform(-cmax, cmax, n) for _ in range(3))
+    if nz == 1:
+        W[:] = 0.0
+    c = rng.uniform(0, 1, n) ** 3
+    return U, V, W, c
+
+
+PER_CASES = [(16, 16, 16), (24, 20, 1), (13, 11, 7), (64, 8, 5), (1, 9, 10), (5, 1, 1)]
+
+
+@pytest.mark.parametrize("path", [1, 0])
+@pytest.mark.parametrize("dims", PER_CASES)
+def test_periodic_matches_oracle(dims, path):
+    nx, ny, nz = dims
+    U, V, W, c = per_case(nx, ny, nz, nx * 31 + ny * 7 + nz)
+    for iord in (1, 2, 3):
+        for nonosc in (False, True):
+            print(f"case dims={dims} path={path} iord={iord} nonosc={nonosc}", flush=True)
+            g = M.Grid(nx, ny, nz, M.PERIODIC, iord=iord, nonosc=nonosc)
+            g.set_path(path)
+            g.set_courant(U, V, W)
+            g.set_field(c)
+            want = c
+            for s in range(4):
+                g.step(1)
+                want = O.per_step(nx, ny, nz, U, V, W, iord, nonosc, want)
+                got = g.field()
+                e = relerr(got, want)
+                assert e <= TOL, f"iord {iord} nonosc {nonosc} step {s}: {e:.3e}"
+                if path == 1:
+                    np.testing.assert_array_equal(got, want)
+            g.close()
+
+
+def test_periodic_matches_padded_reference_fixture():
+    d = dict(np.load(os.path.join(GOLD, "periodic.npz")))
+    for tag in ("a", "b"):
+        nx, ny, nz = (int(x) for x in d[f"{tag}_dims"])
+        for order in (1, 2, 3):
+            g = M.Grid(nx, ny, nz, M.PERIODIC, iord=order, nonosc=False)
+            g.set_winds(d[f"{tag}_dx"], d[f"{tag}_dy"], d[f"{tag}_dz"], float(d[f"{tag}_dt"]),
+                        d[f"{tag}_u"], d[f"{tag}_u"], d[f"{tag}_v"], d[f"{tag}_v"],
+                        d[f"{tag}_w"], d[f"{tag}_w"])
+            g.set_field(d[f"{tag}_c"])
+            g.step(3)
+            want = d[f"{tag}_out_o{order}"]
+            assert relerr(g.field(), want) <= TOL
+            g.close()
+
+
+@pytest.mark.parametrize("path", [1, 0])
+def test_periodic_emission_and_long_run(path):
+    nx, ny, nz = 48, 40, 32
+    U, V, W, c = per_case(nx, ny, nz, 5, cmax=0.25)
+    q = np.random.default_rng(6).uniform(0, 1e-3, nx * ny * nz)
+    g = M.Grid(nx, ny, nz, M.PERIODIC, iord=2, nonosc=True)
+    g.set_path(path)
+    g.set_courant(U, V, W)
+    g.set_field(c)
+    g.set_emission(q)
+    g.step(20)
+    want = O.per_run(nx, ny, nz, U, V, W, 2, True, c, 20, q)
+    got = g.field()
+    assert relerr(got, want) <= TOL
+    mass, lo, hi = g.stats()
+    assert abs(mass - want.sum()) <= 1e-12 * want.sum()
+    assert lo >= 0.0
+    g.set_emission(None)
+    g.close()
+
+
+def test_mass_and_positivity_at_128():
+    """Size-independent invariants on the 128^3 rotation case the bench measures."""
+    import bench
+
+    case = bench.rotation_case(128, cmax=0.4)
+    g = M.Grid(128, 128, 128, M.PERIODIC, iord=2, nonosc=True)
+    g.set_courant(case["U"], case["V"], case["W"])
+    g.set_field(case["c"])
+    m0, _, hi0 = g.stats()
+    g.step(16)
+    m1, lo, hi = g.stats()
+    assert abs(m1 - m0) <= 1e-12 * m0
+    assert lo >= 0.0 and hi <= hi0 * (1 + 1e-14)
+    g.close()
