@@ -40,6 +40,9 @@ extern "C" {
 #define MPD_FRAME 0
 #define MPD_PERIODIC 1
 
+#define MPD_ARITH_EXACT 0 /* unfused FP64, IEEE division: bit-identical to the reference order */
+#define MPD_ARITH_FAST 1  /* contracted multiply-adds, reciprocal division: <= 1e-12 per step */
+
 const char* mpd_error_string(void);
 /* Number of visible HIP devices (0 without a GPU); MPD_OK even when 0. */
 int mpd_device_count(int* n);
@@ -77,6 +80,9 @@ int mpd_field_stats(mpd_grid g, double* mass, double* cmin, double* cmax);
 /* Kernel-path selection for PERIODIC grids: 0 = automatic (fused), 1 = staged
  * (one kernel per stage, intermediates in HBM), 2 = fused. */
 int mpd_set_path(mpd_grid g, int path);
+/* Arithmetic of the PERIODIC fused kernels: MPD_ARITH_FAST (default) or
+ * MPD_ARITH_EXACT.  FRAME grids and the staged path always compute exactly. */
+int mpd_set_arith(mpd_grid g, int arith);
 /* Tile shape of the fused kernel (0,0 = autotuned default). */
 int mpd_set_tile(mpd_grid g, int tile_x, int tile_y);
 

@@ -18,6 +18,8 @@ LIB_PATH = os.path.join(HERE, "libsmolar_hip.so")
 
 FRAME = 0
 PERIODIC = 1
+ARITH_EXACT = 0
+ARITH_FAST = 1
 SOLENOIDAL = 0
 DIVERGENT = 1
 MSMOLAR_SERIAL, MSMOLAR_PTHREADS, MSMOLAR_OPENMP = 0, 1, 2
@@ -76,6 +78,7 @@ SIGNATURES = {
     "mpd_field_stats": (C.c_int, [C.c_void_p, PD, PD, PD]),
     "mpd_set_path": (C.c_int, [C.c_void_p, C.c_int]),
     "mpd_set_tile": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "mpd_set_arith": (C.c_int, [C.c_void_p, C.c_int]),
 }
 
 _lib = None
@@ -235,6 +238,9 @@ class Grid:
 
     def set_tile(self, tx, ty):
         _check(lib().mpd_set_tile(self.h, tx, ty))
+
+    def set_arith(self, arith):
+        _check(lib().mpd_set_arith(self.h, arith))
 
     def close(self):
         if self.h:

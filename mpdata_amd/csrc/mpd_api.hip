@@ -10,8 +10,8 @@
 namespace mpd {
 
 int periodic_staged_step(mpd_grid_s* g);
-int periodic_fused_step(mpd_grid_s* g);
-bool periodic_fused_supported(const mpd_grid_s* g);
+int periodic_fused_step_exact(mpd_grid_s* g);
+int periodic_fused_step_fast(mpd_grid_s* g);
 
 int grid_init(mpd_grid_s* g, int nx, int ny, int nz, int boundary, int iord, int nonosc)
 {
@@ -57,14 +57,13 @@ static int step_once(mpd_grid_s* g)
     return periodic_step(g);
 }
 
+// The fused kernels serve every corrective pass; a donor-only step (iord 1) is a single
+// staged kernel already.
 int periodic_step(mpd_grid_s* g)
 {
-    bool fused = (g->path == 2) || (g->path == 0 && periodic_fused_supported(g));
-    if (fused) {
-        if (!periodic_fused_supported(g))
-            return fail(MPD_ERR_UNSUPPORTED, "fused path does not support this grid");
-        return periodic_fused_step(g);
-    }
+    if (g->path != 1 && g->iord >= 2)
+        return g->arith == MPD_ARITH_FAST ? periodic_fused_step_fast(g)
+                                          : periodic_fused_step_exact(g);
     return periodic_staged_step(g);
 }
 
@@ -414,6 +413,15 @@ extern "C" int mpd_set_path(mpd_grid g, int path)
     GRID_OR_FAIL(g);
     if (path < 0 || path > 2) return fail(MPD_ERR_ARG, "mpd_set_path: %d not in 0..2", path);
     g->path = path;
+    return MPD_OK;
+}
+
+extern "C" int mpd_set_arith(mpd_grid g, int arith)
+{
+    GRID_OR_FAIL(g);
+    if (arith != MPD_ARITH_EXACT && arith != MPD_ARITH_FAST)
+        return fail(MPD_ERR_ARG, "mpd_set_arith: %d is neither exact nor fast", arith);
+    g->arith = arith;
     return MPD_OK;
 }
 

@@ -8,6 +8,7 @@
 #include "mpd_grid.hpp"
 
 namespace mpd {
+using namespace exact;
 namespace {
 
 constexpr int TPB = 256;

@@ -8,6 +8,7 @@ struct mpd_grid_s {
     int iord = 2;
     int nonosc = 0;
     int path = 0;  // 0 auto, 1 staged, 2 fused
+    int arith = MPD_ARITH_FAST;
     int tile_x = 0, tile_y = 0;
     size_t ncell = 0;
     size_t nface[3] = {0, 0, 0};  // elements per face family (layout of the mode)

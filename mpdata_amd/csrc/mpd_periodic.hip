@@ -13,6 +13,7 @@
 #include "mpd_grid.hpp"
 
 namespace mpd {
+using namespace exact;
 namespace {
 
 constexpr int TPB = 256;
@@ -147,10 +148,7 @@ __global__ void k_per_beta(PerIdx x, const double* __restrict__ cn, const double
     double fxl = 0.5 * F2(cp[q.xl], c, U[q.xl]), fxr = 0.5 * F2(c, cp[q.xr], U[t]);
     double fyl = 0.5 * F2(cp[q.yd], c, V[q.yd]), fyr = 0.5 * F2(c, cp[q.yu], V[t]);
     double fzl = 0.5 * F2(cp[q.zb], c, W[q.zb]), fzr = 0.5 * F2(c, cp[q.zf], W[t]);
-    double ain = ((pos(fxl) - neg(fxr)) + (pos(fyl) - neg(fyr))) + (pos(fzl) - neg(fzr));
-    double aout = ((pos(fxr) - neg(fxl)) + (pos(fyr) - neg(fyl))) + (pos(fzr) - neg(fzl));
-    bup[t] = (hi - c) / (ain + MINV);
-    bdn[t] = (c - lo) / (aout + MINV);
+    beta(hi, lo, c, fxl, fxr, fyl, fyr, fzl, fzr, bup[t], bdn[t]);
 }
 
 // Donor pass with limited velocities computed on the fly from the unlimited faces and

@@ -125,16 +125,22 @@ def per_case(nx, ny, nz, seed, cmax=0.2):
 PER_CASES = [(16, 16, 16), (24, 20, 1), (13, 11, 7), (64, 8, 5), (1, 9, 10), (5, 1, 1)]
 
 
-@pytest.mark.parametrize("path", [1, 0])
+# (path, arith): staged exact, fused exact, fused fast
+PATHS = [(1, M.ARITH_EXACT), (2, M.ARITH_EXACT), (2, M.ARITH_FAST)]
+
+
+@pytest.mark.parametrize("path,arith", PATHS)
 @pytest.mark.parametrize("dims", PER_CASES)
-def test_periodic_matches_oracle(dims, path):
+def test_periodic_matches_oracle(dims, path, arith):
     nx, ny, nz = dims
     U, V, W, c = per_case(nx, ny, nz, nx * 31 + ny * 7 + nz)
     for iord in (1, 2, 3):
         for nonosc in (False, True):
-            print(f"case dims={dims} path={path} iord={iord} nonosc={nonosc}", flush=True)
+            print(f"case dims={dims} path={path} arith={arith} iord={iord} nonosc={nonosc}",
+                  flush=True)
             g = M.Grid(nx, ny, nz, M.PERIODIC, iord=iord, nonosc=nonosc)
-            g.set_path(path)
+            g.set_path(path if iord > 1 else 1)
+            g.set_arith(arith)
             g.set_courant(U, V, W)
             g.set_field(c)
             want = c
@@ -144,9 +150,39 @@ def test_periodic_matches_oracle(dims, path):
                 got = g.field()
                 e = relerr(got, want)
                 assert e <= TOL, f"iord {iord} nonosc {nonosc} step {s}: {e:.3e}"
-                if path == 1:
+                if arith == M.ARITH_EXACT or iord == 1:
+                    # unfused IEEE arithmetic in the oracle's order: bit-identical
                     np.testing.assert_array_equal(got, want)
             g.close()
+
+
+TILES = [(32, 8), (64, 4), (16, 16), (32, 4), (64, 8), (16, 8)]
+
+
+@pytest.mark.parametrize("tile", TILES)
+def test_fused_tiles_match_oracle(tile):
+    for dims in [(70, 19, 9), (33, 17, 3), (128, 8, 40)]:
+        nx, ny, nz = dims
+        U, V, W, c = per_case(nx, ny, nz, 17 + nx)
+        for iord in (2, 3):
+            for nonosc in (False, True):
+                want = O.per_run(nx, ny, nz, U, V, W, iord, nonosc, c, 3)
+                for arith in (M.ARITH_EXACT, M.ARITH_FAST):
+                    print(f"case tile={tile} dims={dims} iord={iord} nonosc={nonosc} "
+                          f"arith={arith}", flush=True)
+                    g = M.Grid(nx, ny, nz, M.PERIODIC, iord=iord, nonosc=nonosc)
+                    g.set_path(2)
+                    g.set_arith(arith)
+                    g.set_tile(*tile)
+                    g.set_courant(U, V, W)
+                    g.set_field(c)
+                    g.step(3)
+                    got = g.field()
+                    g.close()
+                    if arith == M.ARITH_EXACT:
+                        np.testing.assert_array_equal(got, want)
+                    else:
+                        assert relerr(got, want) <= TOL
 
 
 def test_periodic_matches_padded_reference_fixture():
@@ -165,7 +201,7 @@ def test_periodic_matches_padded_reference_fixture():
             g.close()
 
 
-@pytest.mark.parametrize("path", [1, 0])
+@pytest.mark.parametrize("path", [1, 2])
 def test_periodic_emission_and_long_run(path):
     nx, ny, nz = 48, 40, 32
     U, V, W, c = per_case(nx, ny, nz, 5, cmax=0.25)
