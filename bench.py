@@ -3,16 +3,21 @@
 Workload (default): ring3d plain MPDATA -- 128^3 triply periodic FP64 grid, solid-body
 rotation about the axis (1,1,1)/sqrt(3) through the centre scaled to a maximum face
 Courant number of 0.4, a Gaussian blob (sigma 8 cells) 30 cells off the axis, iord=2
-with the nonoscillatory limiter.  A step is one MPDATA time step over the whole grid;
-the fields are resident in HBM before timing starts.
+with the nonoscillatory limiter, 128 steps.  A step is one MPDATA time step over the
+whole grid; the fields are resident in HBM before timing starts.
 
-Prints ONE JSON line: cell-updates/s over the timed steps (hipEvents on the launch
-stream, host clock around synchronizations as a cross-check), the roofline of the
-dominant kernel (algorithmic bytes: 40 B per cell-update = read c^n and the three face
-Courant numbers, write c^{n+1}), the parity of the measured path against the CPU oracle,
-and the oracle timed on a bounded sample of the same workload on the host cores.
+Prints ONE JSON line:
+  value        cell-updates/s over the K timed steps (HIP events on the launch stream)
+  roofline     algorithmic bytes of a step (40 B per cell-update: read c^n and the three
+               face Courant numbers, write c^{n+1}) over the summed average durations of
+               the step's kernels, against MI355X's 8 TB/s HBM peak; per-kernel figures
+               in roofline.kernels
+  parity       max|gpu - oracle| / max|oracle| after 1 step and after the full run
+  cpu_baseline the oracle (FP64 C restatement of the reference step, 1 thread) timed on
+               the host cores over the same run, which is also the parity reference
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps S] [--no-cpu]
+  python bench.py --calibrate-copy N     (plain device copy of N doubles, for counters)
 """
 import argparse
 import json
@@ -27,9 +32,9 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published peak
 BYTES_PER_CELL = 40    # c^n + U + V + W read, c^{n+1} written (FP64)
 
 
-def face_courant_from_cell_winds(u, v, w, n, dt=1.0):
+def face_courant_from_cell_winds(u, v, w, dt=1.0):
     """Face Courant numbers of a periodic grid with unit spacing from steady cell-centred
-    winds (u0 == u1): 0.25*dt*(2u_i + 2u_{i+1}), the reference formula (uu.c:132-134)."""
+    winds (u0 == u1): 0.25*dt*(2u_i + 2u_{i+1}) (the formula of uu.c:132-134)."""
     U = 0.25 * dt * ((u + np.roll(u, -1, axis=2)) * 2.0)
     V = 0.25 * dt * ((v + np.roll(v, -1, axis=1)) * 2.0)
     W = 0.25 * dt * ((w + np.roll(w, -1, axis=0)) * 2.0)
@@ -43,11 +48,10 @@ def rotation_case(n=128, cmax=0.4, sigma=8.0, offset=30.0):
     ax = np.arange(n, dtype=np.float64) - n / 2
     Z, Y, X = np.meshgrid(ax, ax, ax, indexing="ij")
     k = np.ones(3) / np.sqrt(3.0)
-    # omega * (k x r)
-    u = k[1] * Z - k[2] * Y
+    u = k[1] * Z - k[2] * Y  # omega * (k x r)
     v = k[2] * X - k[0] * Z
     w = k[0] * Y - k[1] * X
-    U, V, W = face_courant_from_cell_winds(u, v, w, n)
+    U, V, W = face_courant_from_cell_winds(u, v, w)
     s = cmax / max(np.abs(U).max(), np.abs(V).max(), np.abs(W).max())
     U, V, W = U * s, V * s, W * s
     c0 = offset * np.array([1.0, -1.0, 0.0]) / np.sqrt(2.0)
@@ -67,6 +71,10 @@ def host_cpu_model():
     return "unknown"
 
 
+def relerr(got, want):
+    return float(np.abs(got - want).max() / np.abs(want).max())
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,37 +83,78 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--iord", type=int, default=2)
     ap.add_argument("--path", type=int, default=0, help="0 auto, 1 staged, 2 fused")
-    ap.add_argument("--cpu-steps", type=int, default=16)
+    ap.add_argument("--arith", type=int, default=1, help="0 exact, 1 fast")
+    ap.add_argument("--cpu-steps", type=int, default=128,
+                    help="oracle steps: the CPU baseline sample and the parity reference")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="N")
+    ap.add_argument("--ensemble", type=int, default=192,
+                    help="instances of the batched-ensemble comparison (0: skip)")
+    ap.add_argument("--large", type=int, default=256,
+                    help="edge of the large single-grid roofline run (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU: every box has one GPU and "
-                          "the path shards by slab decomposition only in design "
-                          "(see DESIGN.md); run with --gpus 1"}))
+        print(json.dumps({"error": "this build measures one GPU (every box has one); the "
+                          "step shards by z-slabs with halo exchange only by design -- run "
+                          "with --gpus 1"}))
         sys.exit(2)
 
     sys.path.insert(0, ROOT)
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import mpdata_amd as M
 
+    if args.calibrate_copy:
+        ms = M.calibration_copy(args.calibrate_copy, 10)
+        nbytes = 16 * args.calibrate_copy
+        print(json.dumps({"calibration_copy_doubles": args.calibrate_copy, "ms": ms,
+                          "GB/s": nbytes / ms / 1e6}))
+        return
+
     n = args.n
     case = rotation_case(n)
     cells = n ** 3
-    g = M.Grid(n, n, n, M.PERIODIC, iord=args.iord, nonosc=True)
-    g.set_path(args.path)
-    g.set_courant(case["U"], case["V"], case["W"])
-    g.set_field(case["c"])
+
+    def make_grid():
+        g = M.Grid(n, n, n, M.PERIODIC, iord=args.iord, nonosc=True)
+        g.set_path(args.path)
+        g.set_arith(args.arith)
+        g.set_courant(case["U"], case["V"], case["W"])
+        g.set_field(case["c"])
+        return g
+
+    g = make_grid()
     g.step(args.warmup)
     g.synchronize()
     t0 = time.perf_counter()
     ev_ms = g.step_timed(args.steps)
     wall = time.perf_counter() - t0
+    kms = g.kernel_times(8)
+    g.close()
     ms_per_step = ev_ms / args.steps
     value = cells * args.steps / (ev_ms * 1e-3)
-    mass, cmin, cmax_ = g.stats()
-    g.close()
 
-    achieved = cells * BYTES_PER_CELL / (ms_per_step * 1e-3) / 1e9
+    # kernels of one step, in launch order, with their own input/output bytes per cell
+    if args.path == 1:
+        names = ["staged"] * len(kms)
+        own = [None] * len(kms)
+    else:
+        names, own = [], []
+        for p in range(1, args.iord):
+            names += [f"k_f1 (pass {p})", f"k_f2 (pass {p})"]
+            last = p == args.iord - 1
+            own += [64 if p == 1 else 56, 48 if last else 72]
+    kern_ms = sum(kms)
+    achieved = cells * BYTES_PER_CELL / (kern_ms * 1e-3) / 1e9
+    kernels = [{"name": nm, "avg_ms": t, "io_bytes_per_cell": b,
+                "io_GB/s": (cells * b / (t * 1e-3) / 1e9) if b else None}
+               for nm, t, b in zip(names, kms, own)]
+    dominant = max(kernels, key=lambda k: k["avg_ms"])
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_rotation{n}_iord{args.iord}.json")
+    if os.path.exists(tfile) and args.path == 0 and args.arith == 1:
+        traffic = json.load(open(tfile)).get("hbm_bytes_per_step")
+
     line = {
         "metric": "cell-updates/s at <=1e-12 rel err (FP64 MPDATA step, 1 GPU)",
         "value": value,
@@ -118,47 +167,101 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seedless analytic rotation wind and Gaussian blob)",
-        "config": {"workload": f"ring3d periodic {n}^3 rotation, iord={args.iord} "
+        "data": "synthetic (analytic rotation wind and Gaussian blob, no RNG)",
+        "config": {"workload": f"ring3d: periodic {n}^3 solid-body rotation about (1,1,1), "
+                               f"max Courant 0.4, Gaussian blob sigma 8, iord={args.iord} "
                                "nonoscillatory", "grid": [n, n, n], "iord": args.iord,
-                   "nonoscillatory": True, "boundary": "periodic", "max_courant": 0.4,
-                   "path": ["auto", "staged", "fused"][args.path]},
+                   "nonoscillatory": True, "boundary": "periodic",
+                   "path": ["auto(fused)", "staged", "fused"][args.path],
+                   "arith": ["exact", "fast"][args.arith]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_step": cells * BYTES_PER_CELL,
+                     "step_kernel_ms": kern_ms, "dominant_kernel": dominant["name"],
+                     "kernels": kernels},
         "wall_s_timed": wall,
-        "field_after": {"mass": mass, "min": cmin, "max": cmax_},
     }
 
-    # parity of the measured path with the oracle on the same inputs
     import oracle_lib as O
 
-    kpar = 2
-    gp = M.Grid(n, n, n, M.PERIODIC, iord=args.iord, nonosc=True)
-    gp.set_path(args.path)
-    gp.set_courant(case["U"], case["V"], case["W"])
-    gp.set_field(case["c"])
-    gp.step(kpar)
-    got = gp.field()
-    gp.close()
-    t0 = time.perf_counter()
-    want = O.per_run(n, n, n, case["U"], case["V"], case["W"], args.iord, True, case["c"],
-                     kpar)
-    t_par = time.perf_counter() - t0
-    line["parity"] = {"steps": kpar,
-                      "rel_err": float(np.abs(got - want).max() / np.abs(want).max())}
-
+    g = make_grid()
+    g.step(1)
+    got1 = g.field()
+    want1 = O.per_step(n, n, n, case["U"], case["V"], case["W"], args.iord, True, case["c"])
+    par = {"steps_1": {"rel_err": relerr(got1, want1)}}
     if not args.no_cpu:
         ks = max(1, args.cpu_steps)
+        g.step(ks - 1)
+        gotk = g.field()
+        mass, cmin, cmax_ = g.stats()
         t0 = time.perf_counter()
-        O.per_run(n, n, n, case["U"], case["V"], case["W"], args.iord, True, case["c"], ks)
+        wantk = O.per_run(n, n, n, case["U"], case["V"], case["W"], args.iord, True,
+                          case["c"], ks)
         t_cpu = time.perf_counter() - t0
+        par[f"steps_{ks}"] = {"rel_err": relerr(gotk, wantk),
+                              "mass_rel_drift": abs(mass - case["c"].sum()) / case["c"].sum(),
+                              "min": cmin}
         line["cpu_baseline"] = {
             "value": cells * ks / t_cpu, "unit": "cell-updates/s", "cores": 1,
             "kind": "port",
-            "sample": f"oracle (FP64 C restatement, 1 thread, {host_cpu_model()}) on {ks} "
-                      f"steps of the same {n}^3 iord={args.iord} nonoscillatory workload",
+            "sample": f"oracle/cpu_ref.c (FP64 restatement of the reference step with the "
+                      f"same periodic + limiter options, 1 thread, {host_cpu_model()}) over "
+                      f"{ks} steps of the same {n}^3 iord={args.iord} workload",
         }
+    g.close()
+    line["parity"] = par
+    if args.ensemble:
+        line["ensemble"] = ensemble_leg(M, args.ensemble)
+    if args.large:
+        line["large"] = large_leg(M, args.large, args.iord, args.arith)
     print(json.dumps(line))
+
+
+def ensemble_leg(M, n):
+    """Config 4: n instances of uneven sizes, iord 2 nonoscillatory, 100 steps as one
+    autotuned hipGraph replay against per-instance launch sequences."""
+    import mpdata_amd.configs as K
+
+    t0 = time.perf_counter()
+    b = K.ensemble(n, seed=4)
+    cells = b.cells()
+    setup = time.perf_counter() - t0
+    tx, ty, zc, pers = b.autotune(2)
+    b.step(2, M.BATCH_GRAPH)
+    ms_graph = b.step_timed(100, M.BATCH_GRAPH) / 100
+    ms_fused = b.step_timed(10, M.BATCH_FUSED) / 10
+    b.step(1, M.BATCH_PER_INSTANCE)
+    ms_per = b.step_timed(10, M.BATCH_PER_INSTANCE) / 10
+    mass_ok = all(np.isfinite(g.stats()[0]) for g in b.instances[:4])
+    b.close()
+    return {"workload": f"{n} periodic instances from {{64^3, 96^3, 128^3, 160x160x96, "
+                        f"192x192x64}} (seed 4), rotation + 10% seeded modes, iord 2 "
+                        "nonoscillatory", "cells": cells, "setup_s": setup,
+            "autotuned": {"tile": [tx, ty], "zc": zc, "persistent": pers},
+            "graph_ms_per_step": ms_graph, "fused_launch_ms_per_step": ms_fused,
+            "per_instance_ms_per_step": ms_per,
+            "graph_cell_updates_per_s": cells / (ms_graph * 1e-3),
+            "per_instance_cell_updates_per_s": cells / (ms_per * 1e-3),
+            "speedup_graph_vs_per_instance": ms_per / ms_graph, "finite": mass_ok}
+
+
+def large_leg(M, n, iord, arith):
+    """Roofline of the single-grid step at n^3 (the size the 70%-of-HBM target names)."""
+    import mpdata_amd.configs as K
+
+    g = M.Grid(n, n, n, M.PERIODIC, iord=iord, nonosc=True)
+    g.set_arith(arith)
+    K.ring3d(g, n)
+    g.step(2)
+    ms = g.step_timed(20) / 20
+    kms = g.kernel_times(4)
+    g.close()
+    cells = n ** 3
+    kern = sum(kms)
+    ach = cells * BYTES_PER_CELL / (kern * 1e-3) / 1e9
+    return {"workload": f"periodic {n}^3 rotation, iord {iord} nonoscillatory",
+            "ms_per_step": ms, "cell_updates_per_s": cells / (ms * 1e-3),
+            "kernel_ms": kms, "achieved_GB/s": ach, "frac_of_hbm_peak": ach / HBM_PEAK_GBS}
 
 
 if __name__ == "__main__":

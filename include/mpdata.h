@@ -74,6 +74,13 @@ int mpd_synchronize(mpd_grid g);
 int mpd_field_device_ptr(mpd_grid g, double** dptr);
 /* Runs nsteps steps bracketed by HIP events on the launch stream; *ms = elapsed time. */
 int mpd_step_timed(mpd_grid g, int nsteps, double* ms);
+/* Runs nsteps steps recording an event after every kernel launch (PERIODIC grids);
+ * ms[i] = average duration of the i-th kernel of a step (i < max), *n = kernels per
+ * step. */
+int mpd_step_kernel_times(mpd_grid g, int nsteps, double* ms, int max, int* n);
+/* Device copy of n doubles timed over reps launches (*ms per launch): the plain
+ * streaming kernel against which memory-traffic counters are calibrated. */
+int mpd_calibration_copy(size_t n, int reps, double* ms);
 /* Mass (sum of c), minimum and maximum of the field, reduced on the device: the
  * failure sentinel for long runs (MPD_ERR_NUMERIC if any value is non-finite). */
 int mpd_field_stats(mpd_grid g, double* mass, double* cmin, double* cmax);
@@ -85,6 +92,60 @@ int mpd_set_path(mpd_grid g, int path);
 int mpd_set_arith(mpd_grid g, int arith);
 /* Tile shape of the fused kernel (0,0 = autotuned default). */
 int mpd_set_tile(mpd_grid g, int tile_x, int tile_y);
+
+/* ------------------------------------------------------------ generators --- */
+#define MPD_MAX_MODES 8
+#define MPD_MAX_BLOBS 64
+#define MPD_TARGET_FIELD 0
+#define MPD_TARGET_EMISSION 1
+
+/* Cell-centred wind omega*(axis x (x - centre)) + curl(A) with
+ * A_d(x) = sum_m amp[d][m] sin(2 pi (k[d][m][0] x/nx + k[d][m][1] y/ny + k[d][m][2] z/nz)
+ *                              + phase[d][m]),
+ * cell centres at integer coordinates.  Face Courant numbers follow with dx = dt = 1 and
+ * are scaled so that max |face Courant number| = cmax. */
+typedef struct {
+    double axis[3], centre[3], omega;
+    int nmodes;
+    double k[3][MPD_MAX_MODES][3];
+    double amp[3][MPD_MAX_MODES], phase[3][MPD_MAX_MODES];
+    double cmax;
+} mpd_wind_spec;
+int mpd_generate_wind(mpd_grid g, const mpd_wind_spec* w);
+/* Sum of nblobs Gaussians amp[q] exp(-|x - centre_q|^2 / (2 sigma[q]^2)) (centres as
+ * x,y,z triples; periodic = 1 adds the 26 neighbouring periodic images) written to the
+ * field (MPD_TARGET_FIELD) or the emission increment (MPD_TARGET_EMISSION). */
+int mpd_generate_blobs(mpd_grid g, int target, int nblobs, const double* centres,
+                       const double* sigma, const double* amp, int periodic);
+
+/* ------------------------------------------------------------- ensembles --- */
+typedef struct mpd_batch_s* mpd_batch;
+
+#define MPD_BATCH_PER_INSTANCE 0 /* one launch sequence per instance (baseline) */
+#define MPD_BATCH_FUSED 1        /* one launch per kernel stage for all instances */
+#define MPD_BATCH_GRAPH 2        /* the MPD_BATCH_FUSED time loop replayed as a hipGraph */
+
+/* ninst periodic instances; dims[3i..3i+2] = nx, ny, nz of instance i; iord >= 2. */
+int mpd_batch_create(mpd_batch* b, int ninst, const int* dims, int iord, int nonosc);
+int mpd_batch_destroy(mpd_batch b);
+/* Handle of instance i for mpd_set_courant / mpd_set_winds / mpd_set_field /
+ * mpd_get_field / mpd_set_emission (owned by the batch: never mpd_destroy it, and step
+ * it only through mpd_batch_step). */
+int mpd_batch_instance(mpd_batch b, int i, mpd_grid* g);
+int mpd_batch_cells(mpd_batch b, long long* cells);
+int mpd_batch_set_arith(mpd_batch b, int arith);
+/* Tile shape, z-chunk and grid kind (persistent = 1: one resident wave of workgroups
+ * striding over the work items; 0: one workgroup per item). */
+int mpd_batch_set_tile(mpd_batch b, int tile_x, int tile_y, int zc, int persistent);
+/* Times every tile shape x z-chunk x grid kind over nsteps steps on the batch itself
+ * (fields restored afterwards), applies and returns the fastest (any output may be
+ * NULL).  Cached per process; also in the file named by MPD_TUNE_CACHE when set. */
+int mpd_batch_autotune(mpd_batch b, int nsteps, int* tile_x, int* tile_y, int* zc,
+                       int* persistent);
+int mpd_batch_step(mpd_batch b, int nsteps, int mode);
+/* As mpd_batch_step, bracketed by HIP events (table upload and graph capture happen
+ * before the first event). */
+int mpd_batch_step_timed(mpd_batch b, int nsteps, int mode, double* ms);
 
 #ifdef __cplusplus
 }

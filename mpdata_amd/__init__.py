@@ -79,6 +79,17 @@ SIGNATURES = {
     "mpd_set_path": (C.c_int, [C.c_void_p, C.c_int]),
     "mpd_set_tile": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "mpd_set_arith": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpd_step_kernel_times": (C.c_int, [C.c_void_p, C.c_int, PD, C.c_int, PI]),
+    "mpd_calibration_copy": (C.c_int, [C.c_size_t, C.c_int, PD]),
+    "mpd_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, PI, C.c_int, C.c_int]),
+    "mpd_batch_destroy": (C.c_int, [C.c_void_p]),
+    "mpd_batch_instance": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "mpd_batch_cells": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong)]),
+    "mpd_batch_set_arith": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpd_batch_set_tile": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "mpd_batch_autotune": (C.c_int, [C.c_void_p, C.c_int, PI, PI, PI, PI]),
+    "mpd_batch_step": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "mpd_batch_step_timed": (C.c_int, [C.c_void_p, C.c_int, C.c_int, PD]),
 }
 
 _lib = None
@@ -117,6 +128,13 @@ def device_count():
     n = C.c_int(0)
     _check(lib().mpd_device_count(C.byref(n)))
     return n.value
+
+
+def calibration_copy(n, reps=10):
+    """ms per launch of a plain device copy of n doubles."""
+    ms = C.c_double(0)
+    _check(lib().mpd_calibration_copy(n, reps, C.byref(ms)))
+    return ms.value
 
 
 def smolar(nx, ny, ns, dx, dy, ds, dt, u0, u1, v0, v1, w0, w1, c0, c1, flow_type=DIVERGENT,
@@ -169,12 +187,17 @@ class MSmolar:
 class Grid:
     """A device-resident advection instance (mpd_grid)."""
 
-    def __init__(self, nx, ny, nz, boundary=PERIODIC, iord=2, nonosc=True):
+    def __init__(self, nx, ny, nz, boundary=PERIODIC, iord=2, nonosc=True, _handle=None):
         self.shape = (nx, ny, nz)
         self.n = nx * ny * nz
         self.boundary = boundary
+        self.owned = _handle is None
         self.h = C.c_void_p(None)
-        _check(lib().mpd_create(C.byref(self.h), nx, ny, nz, boundary, iord, int(nonosc)))
+        if _handle is not None:
+            self.h = _handle
+        else:
+            _check(lib().mpd_create(C.byref(self.h), nx, ny, nz, boundary, iord,
+                                    int(nonosc)))
 
     def face_sizes(self):
         nx, ny, nz = self.shape
@@ -239,12 +262,75 @@ class Grid:
     def set_tile(self, tx, ty):
         _check(lib().mpd_set_tile(self.h, tx, ty))
 
+    def kernel_times(self, nsteps, max_kernels=16):
+        """Average duration (ms) of each kernel of a step over nsteps steps."""
+        ms = (C.c_double * max_kernels)()
+        n = C.c_int(0)
+        _check(lib().mpd_step_kernel_times(self.h, nsteps, ms, max_kernels, C.byref(n)))
+        return [ms[i] for i in range(min(n.value, max_kernels))]
+
     def set_arith(self, arith):
         _check(lib().mpd_set_arith(self.h, arith))
 
     def close(self):
-        if self.h:
+        if self.h and self.owned:
             lib().mpd_destroy(self.h)
+        self.h = C.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+BATCH_PER_INSTANCE, BATCH_FUSED, BATCH_GRAPH = 0, 1, 2
+
+
+class Batch:
+    """An ensemble of periodic instances of uneven sizes (mpd_batch)."""
+
+    def __init__(self, dims, iord=2, nonosc=True):
+        self.dims = [tuple(int(x) for x in d) for d in dims]
+        flat = (C.c_int * (3 * len(self.dims)))(*[x for d in self.dims for x in d])
+        self.h = C.c_void_p(None)
+        _check(lib().mpd_batch_create(C.byref(self.h), len(self.dims), flat, iord,
+                                      int(nonosc)))
+        self.instances = []
+        for i, d in enumerate(self.dims):
+            gh = C.c_void_p(None)
+            _check(lib().mpd_batch_instance(self.h, i, C.byref(gh)))
+            self.instances.append(Grid(*d, PERIODIC, iord, nonosc, _handle=gh))
+
+    def cells(self):
+        n = C.c_longlong(0)
+        _check(lib().mpd_batch_cells(self.h, C.byref(n)))
+        return n.value
+
+    def set_arith(self, arith):
+        _check(lib().mpd_batch_set_arith(self.h, arith))
+
+    def set_tile(self, tx, ty, zc, persistent=True):
+        _check(lib().mpd_batch_set_tile(self.h, tx, ty, zc, int(persistent)))
+
+    def autotune(self, nsteps=2):
+        v = [C.c_int() for _ in range(4)]
+        _check(lib().mpd_batch_autotune(self.h, nsteps, *(C.byref(x) for x in v)))
+        return tuple(x.value for x in v)
+
+    def step(self, nsteps=1, mode=BATCH_GRAPH):
+        _check(lib().mpd_batch_step(self.h, nsteps, mode))
+
+    def step_timed(self, nsteps, mode=BATCH_GRAPH):
+        ms = C.c_double(0)
+        _check(lib().mpd_batch_step_timed(self.h, nsteps, mode, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self.h:
+            for g in self.instances:
+                g.h = C.c_void_p(None)
+            lib().mpd_batch_destroy(self.h)
             self.h = C.c_void_p(None)
 
     def __del__(self):

@@ -51,6 +51,19 @@ int grid_init(mpd_grid_s* g, int nx, int ny, int nz, int boundary, int iord, int
     return MPD_OK;
 }
 
+// Scratch of the fused path: unlimited / limited antidiffusive faces, pass fields.
+int ensure_fused_scratch(mpd_grid_s* g)
+{
+    int rc;
+    for (int d = 0; d < 3; d++) {
+        if ((rc = g->fa[d].alloc(g->ncell)) != MPD_OK) return rc;
+        if ((rc = g->fb[d].alloc(g->ncell)) != MPD_OK) return rc;
+    }
+    for (int b = 0; b < 2; b++)
+        if ((rc = g->cp[b].alloc(g->ncell)) != MPD_OK) return rc;
+    return MPD_OK;
+}
+
 static int step_once(mpd_grid_s* g)
 {
     if (g->boundary == MPD_FRAME) return frame_step(g);
@@ -399,6 +412,92 @@ extern "C" int mpd_step_timed(mpd_grid g, int nsteps, double* ms)
     if (rc) return rc;
     if (e != hipSuccess) return hip_fail(e, "mpd_step_timed event");
     *ms = f;
+    return MPD_OK;
+}
+
+namespace mpd {
+void trace_mark(mpd_grid_s* g)
+{
+    if (!g->trace) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    (void)hipEventRecord(e, stream());
+    g->trace->push_back(e);
+}
+}  // namespace mpd
+
+extern "C" int mpd_step_kernel_times(mpd_grid g, int nsteps, double* ms, int max, int* n)
+{
+    GRID_OR_FAIL(g);
+    if (!ms || !n || nsteps < 1 || max < 1)
+        return fail(MPD_ERR_ARG, "mpd_step_kernel_times: bad arguments");
+    if (g->boundary != MPD_PERIODIC)
+        return fail(MPD_ERR_UNSUPPORTED, "mpd_step_kernel_times: PERIODIC grids only");
+    std::vector<hipEvent_t> ev;
+    hipEvent_t start;
+    MPD_HIP(hipEventCreate(&start));
+    MPD_HIP(hipEventRecord(start, stream()));
+    g->trace = &ev;
+    int rc = mpd_step(g, nsteps);
+    g->trace = nullptr;
+    if (rc == MPD_OK && ev.empty()) rc = fail(MPD_ERR_UNSUPPORTED, "no kernel launches traced");
+    if (rc == MPD_OK) {
+        hipError_t e = hipEventSynchronize(ev.back());
+        if (e != hipSuccess) rc = hip_fail(e, "hipEventSynchronize");
+    }
+    int per = rc == MPD_OK ? (int)(ev.size() / nsteps) : 0;
+    if (rc == MPD_OK && per * nsteps != (int)ev.size())
+        rc = fail(MPD_ERR_UNSUPPORTED, "launch count varies between steps");
+    if (rc == MPD_OK) {
+        int m = per < max ? per : max;
+        for (int i = 0; i < m; i++) ms[i] = 0.0;
+        hipEvent_t prev = start;
+        for (size_t k = 0; k < ev.size(); k++) {
+            float f = 0.f;
+            (void)hipEventElapsedTime(&f, prev, ev[k]);
+            if ((int)(k % per) < m) ms[k % per] += f / nsteps;
+            prev = ev[k];
+        }
+        *n = per;
+    }
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(start);
+    return rc;
+}
+
+namespace {
+__global__ void k_copy(const double* __restrict__ src, double* __restrict__ dst, size_t n)
+{
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (size_t)gridDim.x * blockDim.x)
+        dst[t] = src[t];
+}
+}  // namespace
+
+extern "C" int mpd_calibration_copy(size_t n, int reps, double* ms)
+{
+    clear_error();
+    if (n == 0 || reps < 1 || !ms) return fail(MPD_ERR_ARG, "mpd_calibration_copy: bad arguments");
+    int rc = ensure_device();
+    if (rc) return rc;
+    DevBuf a, b;
+    if ((rc = a.alloc(n)) || (rc = b.alloc(n))) return rc;
+    hipStream_t s = stream();
+    MPD_HIP(hipMemsetAsync(a.p, 0, n * sizeof(double), s));
+    hipEvent_t e0, e1;
+    MPD_HIP(hipEventCreate(&e0));
+    MPD_HIP(hipEventCreate(&e1));
+    hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, s, a.p, b.p, n);  // warm-up
+    MPD_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; r++)
+        hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, s, a.p, b.p, n);
+    MPD_HIP(hipEventRecord(e1, s));
+    MPD_HIP(hipEventSynchronize(e1));
+    float f = 0.f;
+    MPD_HIP(hipEventElapsedTime(&f, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = f / reps;
     return MPD_OK;
 }
 

@@ -1,0 +1,470 @@
+// Ensembles of advection instances of uneven sizes (batched extension; no counterpart
+// in the reference, whose only multi-instance driver loops over subdomains on the host,
+// core/src/msmolar.c:44-180).
+//
+// Each instance is an ordinary periodic grid.  Three ways to advance them:
+//   MPD_BATCH_PER_INSTANCE  every instance runs its own launch sequence (two kernels
+//                           per corrective pass per instance): the baseline.
+//   MPD_BATCH_FUSED         one launch per kernel stage for the whole ensemble: the
+//                           fused kernels read a device table of per-instance
+//                           descriptors and a persistent grid of workgroups strides
+//                           over every (instance, tile, z-chunk) work item.
+//   MPD_BATCH_GRAPH         the MPD_BATCH_FUSED time loop captured once in a hipGraph
+//                           and replayed (no per-step host launch work).
+// Tile shape, z-chunk and grid size come from the built-in autotuner (timed trials on
+// the batch itself), or are set explicitly.
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "mpd_fused.hpp"
+#include "mpd_grid.hpp"
+
+namespace mpd {
+int periodic_step(mpd_grid_s* g);
+int ensure_fused_scratch(mpd_grid_s* g);
+}  // namespace mpd
+
+using namespace mpd;
+
+struct mpd_batch_s {
+    int ninst = 0, iord = 2, nonosc = 1, arith = MPD_ARITH_FAST;
+    int tx = 32, ty = 8, zc = 16, persistent = 1;
+    int cur = 0;
+    long long cells = 0;
+    std::vector<mpd_grid_s*> g;
+    // device tables: [parity][pass-1][0 = F1, 1 = F2][instance]
+    FusedArgs* dtab = nullptr;
+    int* ditem0 = nullptr;
+    int nitems = 0;
+    std::vector<FusedArgs> htab;
+    std::vector<int> hitem0;
+    // cached graphs of the time loop, one per starting field parity
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int g_steps[2] = {-1, -1}, g_key[2] = {-1, -1};
+    bool tuned = false;
+
+    ~mpd_batch_s()
+    {
+        for (auto& e : gexec)
+            if (e) (void)hipGraphExecDestroy(e);
+        if (dtab) (void)hipFree(dtab);
+        if (ditem0) (void)hipFree(ditem0);
+        for (auto* x : g) delete x;
+    }
+};
+
+namespace {
+
+int table_index(const mpd_batch_s* b, int parity, int pass, int f2, int inst)
+{
+    return ((parity * (b->iord - 1) + (pass - 1)) * 2 + f2) * b->ninst + inst;
+}
+
+// the key of the launch configuration a cached graph was captured with
+int config_key(const mpd_batch_s* b)
+{
+    return ((b->tx * 131 + b->ty) * 131 + b->zc) * 4 + b->persistent * 2 + b->arith;
+}
+
+// Fills the per-instance descriptors of every pass for both field parities (the buffer
+// roles of periodic_fused_step) and uploads them.
+int build_tables(mpd_batch_s* b)
+{
+    int rc;
+    const int npass = b->iord - 1;
+    b->htab.assign((size_t)2 * npass * 2 * b->ninst, FusedArgs{});
+    b->hitem0.assign(b->ninst, 0);
+    int items = 0;
+    for (int i = 0; i < b->ninst; i++) {
+        mpd_grid_s* g = b->g[i];
+        if ((rc = ensure_fused_scratch(g))) return rc;
+        FusedArgs geo{};
+        geo.nx = g->nx;
+        geo.ny = g->ny;
+        geo.nz = g->nz;
+        geo.np = (long long)g->nx * g->ny;
+        geo.zc = std::min(b->zc, g->nz);
+        fused_geometry(geo, b->tx, b->ty);
+        b->hitem0[i] = items;
+        items += fused_items(geo);
+        for (int par = 0; par < 2; par++) {
+            const double* cin = g->c[par].p;
+            double* out = g->c[1 - par].p;
+            const double* prev[3] = {g->face[0].p, g->face[1].p, g->face[2].p};
+            const double* field = cin;
+            for (int p = 1; p <= npass; p++) {
+                const bool last = (p == npass);
+                double* dst = last ? out : g->cp[p & 1].p;
+                FusedArgs f1 = geo;
+                f1.cin = field;
+                f1.U = prev[0];
+                f1.V = prev[1];
+                f1.W = prev[2];
+                f1.c1out = g->cp[0].p;
+                f1.Ax = g->fa[0].p;
+                f1.Ay = g->fa[1].p;
+                f1.Az = g->fa[2].p;
+                FusedArgs f2 = f1;
+                f2.cn = cin;
+                f2.cp = (p == 1) ? g->cp[0].p : field;
+                f2.q = (last && g->has_q) ? g->q.p : nullptr;
+                f2.cout = dst;
+                f2.Lx = last ? nullptr : g->fb[0].p;
+                f2.Ly = last ? nullptr : g->fb[1].p;
+                f2.Lz = last ? nullptr : g->fb[2].p;
+                b->htab[table_index(b, par, p, 0, i)] = f1;
+                b->htab[table_index(b, par, p, 1, i)] = f2;
+                for (int d = 0; d < 3; d++) prev[d] = g->fb[d].p;
+                field = dst;
+            }
+        }
+    }
+    b->nitems = items;
+    if (!b->dtab) {
+        MPD_HIP(hipMalloc((void**)&b->dtab, b->htab.size() * sizeof(FusedArgs)));
+        MPD_HIP(hipMalloc((void**)&b->ditem0, b->ninst * sizeof(int)));
+    }
+    hipStream_t s = stream();
+    MPD_HIP(hipMemcpyAsync(b->dtab, b->htab.data(), b->htab.size() * sizeof(FusedArgs),
+                           hipMemcpyHostToDevice, s));
+    MPD_HIP(hipMemcpyAsync(b->ditem0, b->hitem0.data(), b->ninst * sizeof(int),
+                           hipMemcpyHostToDevice, s));
+    MPD_HIP(hipStreamSynchronize(s));  // the host tables are reused on the next call
+    return MPD_OK;
+}
+
+int cu_count()
+{
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 256;
+    return n;
+}
+
+int grid_for(const mpd_batch_s* b, bool f1)
+{
+    if (!b->persistent) return b->nitems;
+    int per = b->arith == MPD_ARITH_FAST ? fused_resident_blocks_fast(b->tx, b->ty, f1)
+                                         : fused_resident_blocks_exact(b->tx, b->ty, f1);
+    long long grid = (long long)std::max(per, 1) * cu_count();
+    if (grid >= b->nitems) return b->nitems;
+    grid -= grid % 8;
+    return (int)std::max<long long>(grid, 1);
+}
+
+// one batched step from parity par: launches only (graph-capturable)
+int launch_step(mpd_batch_s* b, int par)
+{
+    hipStream_t s = stream();
+    int rc;
+    auto launch = b->arith == MPD_ARITH_FAST ? fused_batch_launch_fast : fused_batch_launch_exact;
+    for (int p = 1; p < b->iord; p++) {
+        const FusedArgs* t1 = b->dtab + table_index(b, par, p, 0, 0);
+        const FusedArgs* t2 = b->dtab + table_index(b, par, p, 1, 0);
+        if ((rc = launch(b->tx, b->ty, true, p == 1, t1, b->ditem0, b->ninst, b->nitems,
+                         grid_for(b, true), s)))
+            return rc;
+        if ((rc = launch(b->tx, b->ty, false, false, t2, b->ditem0, b->ninst, b->nitems,
+                         grid_for(b, false), s)))
+            return rc;
+    }
+    return MPD_OK;
+}
+
+void set_parity(mpd_batch_s* b, int par)
+{
+    b->cur = par;
+    for (auto* g : b->g) g->cur = par;
+}
+
+// Captures (without running) the nsteps-step loop from the current parity, unless the
+// cached graph for that parity already matches.
+int ensure_graph(mpd_batch_s* b, int nsteps)
+{
+    hipStream_t s = stream();
+    const int key = config_key(b), par = b->cur;
+    if (b->gexec[par] && b->g_steps[par] == nsteps && b->g_key[par] == key) return MPD_OK;
+    if (b->gexec[par]) {
+        (void)hipGraphExecDestroy(b->gexec[par]);
+        b->gexec[par] = nullptr;
+    }
+    hipGraph_t graph;
+    MPD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    int rc = MPD_OK, p = par;
+    for (int k = 0; k < nsteps && rc == MPD_OK; k++) {
+        rc = launch_step(b, p);
+        p = 1 - p;
+    }
+    hipError_t e = hipStreamEndCapture(s, &graph);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&b->gexec[par], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        b->gexec[par] = nullptr;
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    b->g_steps[par] = nsteps;
+    b->g_key[par] = key;
+    return MPD_OK;
+}
+
+int run_graph(mpd_batch_s* b, int nsteps)
+{
+    int rc = ensure_graph(b, nsteps);
+    if (rc) return rc;
+    MPD_HIP(hipGraphLaunch(b->gexec[b->cur], stream()));
+    if (nsteps & 1) set_parity(b, 1 - b->cur);
+    return MPD_OK;
+}
+
+int run(mpd_batch_s* b, int nsteps, int mode)
+{
+    int rc;
+    for (auto* g : b->g)
+        if (!g->has_winds || !g->has_field)
+            return fail(MPD_ERR_ARG, "batch step: every instance needs Courant numbers and a field");
+    if (mode == MPD_BATCH_PER_INSTANCE) {
+        for (int k = 0; k < nsteps; k++) {
+            for (auto* g : b->g)
+                if ((rc = periodic_step(g))) return rc;
+            b->cur = 1 - b->cur;
+        }
+        return MPD_OK;
+    }
+    if (!b->nonosc) return fail(MPD_ERR_UNSUPPORTED, "batched launches need the limiter on");
+    if ((rc = build_tables(b))) return rc;
+    if (mode == MPD_BATCH_GRAPH) return run_graph(b, nsteps);
+    for (int k = 0; k < nsteps; k++) {
+        if ((rc = launch_step(b, b->cur))) return rc;
+        set_parity(b, 1 - b->cur);
+    }
+    return MPD_OK;
+}
+
+}  // namespace
+
+#define BATCH_OR_FAIL(b)                                                 \
+    do {                                                                 \
+        clear_error();                                                   \
+        if (!(b)) return fail(MPD_ERR_ARG, "%s: NULL batch", __func__);  \
+    } while (0)
+
+extern "C" int mpd_batch_create(mpd_batch* out, int ninst, const int* dims, int iord, int nonosc)
+{
+    clear_error();
+    if (!out || !dims || ninst < 1) return fail(MPD_ERR_ARG, "mpd_batch_create: bad arguments");
+    *out = nullptr;
+    if (iord < 2) return fail(MPD_ERR_ARG, "mpd_batch_create: iord must be >= 2");
+    auto* b = new mpd_batch_s;
+    b->ninst = ninst;
+    b->iord = iord;
+    b->nonosc = nonosc ? 1 : 0;
+    for (int i = 0; i < ninst; i++) {
+        auto* g = new mpd_grid_s;
+        b->g.push_back(g);
+        int rc = grid_init(g, dims[3 * i], dims[3 * i + 1], dims[3 * i + 2], MPD_PERIODIC, iord,
+                           nonosc);
+        if (rc == MPD_OK) rc = ensure_fused_scratch(g);
+        if (rc) {
+            delete b;
+            return rc;
+        }
+        b->cells += (long long)g->ncell;
+    }
+    *out = b;
+    return MPD_OK;
+}
+
+extern "C" int mpd_batch_destroy(mpd_batch b)
+{
+    if (b) {
+        (void)hipStreamSynchronize(stream());
+        delete b;
+    }
+    return MPD_OK;
+}
+
+extern "C" int mpd_batch_instance(mpd_batch b, int i, mpd_grid* g)
+{
+    BATCH_OR_FAIL(b);
+    if (!g || i < 0 || i >= b->ninst) return fail(MPD_ERR_ARG, "mpd_batch_instance: bad index");
+    *g = b->g[i];
+    return MPD_OK;
+}
+
+extern "C" int mpd_batch_cells(mpd_batch b, long long* cells)
+{
+    BATCH_OR_FAIL(b);
+    if (!cells) return fail(MPD_ERR_ARG, "mpd_batch_cells: NULL");
+    *cells = b->cells;
+    return MPD_OK;
+}
+
+extern "C" int mpd_batch_set_arith(mpd_batch b, int arith)
+{
+    BATCH_OR_FAIL(b);
+    if (arith != MPD_ARITH_EXACT && arith != MPD_ARITH_FAST)
+        return fail(MPD_ERR_ARG, "mpd_batch_set_arith: bad value");
+    b->arith = arith;
+    for (auto* g : b->g) g->arith = arith;
+    return MPD_OK;
+}
+
+extern "C" int mpd_batch_set_tile(mpd_batch b, int tx, int ty, int zc, int persistent)
+{
+    BATCH_OR_FAIL(b);
+    bool ok = false;
+#define MPD_TILE(X, Y) ok = ok || (tx == X && ty == Y);
+    MPD_FUSED_TILES(MPD_TILE)
+#undef MPD_TILE
+    if (!ok || zc < 1) return fail(MPD_ERR_ARG, "mpd_batch_set_tile: %dx%d zc %d", tx, ty, zc);
+    b->tx = tx;
+    b->ty = ty;
+    b->zc = zc;
+    b->persistent = persistent ? 1 : 0;
+    for (auto* g : b->g) {
+        g->tile_x = tx;
+        g->tile_y = ty;
+    }
+    return MPD_OK;
+}
+
+extern "C" int mpd_batch_step(mpd_batch b, int nsteps, int mode)
+{
+    BATCH_OR_FAIL(b);
+    if (nsteps < 0 || mode < MPD_BATCH_PER_INSTANCE || mode > MPD_BATCH_GRAPH)
+        return fail(MPD_ERR_ARG, "mpd_batch_step: bad nsteps or mode");
+    if (nsteps == 0) return MPD_OK;
+    return run(b, nsteps, mode);
+}
+
+extern "C" int mpd_batch_step_timed(mpd_batch b, int nsteps, int mode, double* ms)
+{
+    BATCH_OR_FAIL(b);
+    if (!ms || nsteps < 1) return fail(MPD_ERR_ARG, "mpd_batch_step_timed: bad arguments");
+    int rc;
+    for (auto* g : b->g)
+        if (!g->has_winds || !g->has_field)
+            return fail(MPD_ERR_ARG, "batch step: every instance needs Courant numbers and a field");
+    // tables and the captured graph are prepared outside the timed region
+    if (mode != MPD_BATCH_PER_INSTANCE) {
+        if (!b->nonosc) return fail(MPD_ERR_UNSUPPORTED, "batched launches need the limiter on");
+        if ((rc = build_tables(b))) return rc;
+    }
+    if (mode == MPD_BATCH_GRAPH && (rc = ensure_graph(b, nsteps))) return rc;
+    hipEvent_t e0, e1;
+    MPD_HIP(hipEventCreate(&e0));
+    MPD_HIP(hipEventCreate(&e1));
+    hipStream_t s = stream();
+    MPD_HIP(hipEventRecord(e0, s));
+    if (mode == MPD_BATCH_GRAPH) {
+        rc = run_graph(b, nsteps);
+    } else if (mode == MPD_BATCH_FUSED) {
+        rc = MPD_OK;
+        for (int k = 0; k < nsteps && rc == MPD_OK; k++) {
+            rc = launch_step(b, b->cur);
+            set_parity(b, 1 - b->cur);
+        }
+    } else {
+        rc = run(b, nsteps, mode);
+    }
+    if (rc == MPD_OK) {
+        MPD_HIP(hipEventRecord(e1, s));
+        MPD_HIP(hipEventSynchronize(e1));
+        float f = 0.f;
+        MPD_HIP(hipEventElapsedTime(&f, e0, e1));
+        *ms = f;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+// Autotuner: times every compiled tile shape x z-chunk x grid kind on the batch itself
+// (MPD_BATCH_FUSED, nsteps steps each after one warm-up step) and keeps the fastest.  The
+// fields are saved before and restored after, so tuning leaves the ensemble unchanged.
+// Results are cached per process by the ensemble's shape signature; with MPD_TUNE_CACHE
+// set they are also appended to / read from that file.
+extern "C" int mpd_batch_autotune(mpd_batch b, int nsteps, int* tx, int* ty, int* zc,
+                                  int* persistent)
+{
+    BATCH_OR_FAIL(b);
+    if (nsteps < 1) return fail(MPD_ERR_ARG, "mpd_batch_autotune: nsteps < 1");
+    static std::map<std::string, std::vector<int>> cache;
+    std::string key = "i" + std::to_string(b->iord) + "a" + std::to_string(b->arith);
+    for (auto* g : b->g)
+        key += ":" + std::to_string(g->nx) + "x" + std::to_string(g->ny) + "x" +
+               std::to_string(g->nz);
+    const char* path = getenv("MPD_TUNE_CACHE");
+    if (!cache.count(key) && path) {
+        if (FILE* f = fopen(path, "r")) {
+            char line[1 << 16];
+            while (fgets(line, sizeof line, f)) {
+                char* sp = strchr(line, ' ');
+                if (!sp) continue;
+                *sp = 0;
+                int a, c, d, e;
+                if (key == line && sscanf(sp + 1, "%d %d %d %d", &a, &c, &d, &e) == 4)
+                    cache[key] = {a, c, d, e};
+            }
+            fclose(f);
+        }
+    }
+    int rc;
+    if (!cache.count(key)) {
+        std::vector<DevBuf> save(b->ninst);
+        hipStream_t s = stream();
+        for (int i = 0; i < b->ninst; i++) {
+            mpd_grid_s* g = b->g[i];
+            if ((rc = save[i].alloc(g->ncell))) return rc;
+            MPD_HIP(hipMemcpyAsync(save[i].p, g->c[g->cur].p, g->ncell * 8, hipMemcpyDeviceToDevice, s));
+        }
+        const int start = b->cur;
+        double best = 1e300;
+        std::vector<int> bestc = {b->tx, b->ty, b->zc, b->persistent};
+        const int shapes[][2] = {
+#define MPD_TILE(X, Y) {X, Y},
+            MPD_FUSED_TILES(MPD_TILE)
+#undef MPD_TILE
+        };
+        for (auto& sh : shapes)
+            for (int z : {8, 16, 32})
+                for (int pers : {0, 1}) {
+                    if ((rc = mpd_batch_set_tile(b, sh[0], sh[1], z, pers))) return rc;
+                    double ms = 0;
+                    if ((rc = mpd_batch_step(b, 1, MPD_BATCH_FUSED))) return rc;
+                    if ((rc = mpd_batch_step_timed(b, nsteps, MPD_BATCH_FUSED, &ms))) return rc;
+                    if (ms < best) {
+                        best = ms;
+                        bestc = {sh[0], sh[1], z, pers};
+                    }
+                }
+        set_parity(b, start);
+        for (int i = 0; i < b->ninst; i++) {
+            mpd_grid_s* g = b->g[i];
+            MPD_HIP(hipMemcpyAsync(g->c[g->cur].p, save[i].p, g->ncell * 8, hipMemcpyDeviceToDevice, s));
+        }
+        MPD_HIP(hipStreamSynchronize(s));
+        cache[key] = bestc;
+        if (path)
+            if (FILE* f = fopen(path, "a")) {
+                fprintf(f, "%s %d %d %d %d\n", key.c_str(), bestc[0], bestc[1], bestc[2], bestc[3]);
+                fclose(f);
+            }
+    }
+    const std::vector<int>& c = cache[key];
+    if ((rc = mpd_batch_set_tile(b, c[0], c[1], c[2], c[3]))) return rc;
+    if (tx) *tx = c[0];
+    if (ty) *ty = c[1];
+    if (zc) *zc = c[2];
+    if (persistent) *persistent = c[3];
+    b->tuned = true;
+    return MPD_OK;
+}

@@ -24,32 +24,18 @@
 // path and the oracle) and MPD_ARITH=fast (contracted multiply-adds, reciprocal-based
 // division; <= 1e-12 normwise per step against the oracle).
 #include "mpd_formulas.hpp"
+#include "mpd_fused.hpp"
 #include "mpd_grid.hpp"
 
 #define MPD_CAT2(a, b) a##_##b
 #define MPD_CAT(a, b) MPD_CAT2(a, b)
 
 namespace mpd {
+int ensure_fused_scratch(mpd_grid_s* g);
 using namespace MPD_ARITH;
 namespace {
 
-struct FusedArgs {
-    int nx, ny, nz;
-    long long np;
-    int tiles_x, tiles_y, zc, nchunks;
-    int xcd_remap;  // 1: blocks are remapped so each XCD gets a contiguous range
-    // F1
-    const double* cin;                       // c^n (DONOR) or the pass input field
-    const double *U, *V, *W;                 // previous pass's face velocities
-    double* c1out;                           // DONOR: the donor field (tile cells)
-    double *Ax, *Ay, *Az;                    // unlimited antidiffusive faces
-    // F2
-    const double* cn;                        // c^n (limiter bounds)
-    const double* cp;                        // pass input field
-    const double* q;                         // emission increment or null
-    double* cout;                            // pass result
-    double *Lx, *Ly, *Lz;                    // limited faces or null
-};
+
 
 template <int TX, int TY>
 struct Geo {
@@ -111,25 +97,46 @@ __device__ __forceinline__ int wrapi(int a, int n)
     return a < 0 ? a + n : a;
 }
 
-// logical tile of this workgroup (XCD-aware: the hardware deals workgroups to the 8
-// XCDs round robin, so consecutive logical tiles -- which share halos -- are gathered
-// onto one XCD and its L2)
-__device__ __forceinline__ void tile_of(const FusedArgs& a, int& bx, int& by, int& bz)
+// tile (bx, by) and z-chunk bz of work item b of an instance
+__device__ __forceinline__ void tile_of(const FusedArgs& a, int b, int& bx, int& by, int& bz)
 {
-    int b = blockIdx.x;
-    const int nb = a.tiles_x * a.tiles_y * a.nchunks;
-    if (a.xcd_remap) {
-        const int per = nb / 8;
-        b = (b % 8) * per + b / 8;
-    }
-    (void)nb;
     bx = b % a.tiles_x;
     by = (b / a.tiles_x) % a.tiles_y;
     bz = b / (a.tiles_x * a.tiles_y);
 }
 
+// Work items of a launch.  A single grid passes its descriptor by value (ninst = 0);
+// a batch of instances passes a device table of descriptors and the first item of each
+// instance.  Workgroups stride over the items (persistent when the grid is smaller than
+// the item count).  When remap is set, items are dealt so that each of the 8 XCDs --
+// which receive workgroups round robin -- works on one contiguous eighth of the item
+// list, keeping halo-sharing neighbours in the same L2.
+struct Work {
+    const FusedArgs* table;
+    const int* item0;
+    int ninst, nitems, remap;
+};
+
+__device__ __forceinline__ int find_inst(const int* item0, int n, int item)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (item0[mid] <= item)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int item_of(const Work& w, int it)
+{
+    return w.remap ? (it % 8) * (w.nitems / 8) + it / 8 : it;
+}
+
 template <int TX, int TY, bool DONOR>
-__global__ void __launch_bounds__((Geo<TX, TY>::NT)) k_f1(FusedArgs a)
+__device__ __forceinline__ void f1_tile(const FusedArgs& a, int item)
 {
     using G = Geo<TX, TY>;
     constexpr int RX = G::RX, R = G::R;
@@ -139,7 +146,7 @@ __global__ void __launch_bounds__((Geo<TX, TY>::NT)) k_f1(FusedArgs a)
     const bool live = G::cell(threadIdx.x, rx, ry);
     const int ring = live ? G::ring(rx, ry) : 3;
     int bx, by, bz;
-    tile_of(a, bx, by, bz);
+    tile_of(a, item, bx, by, bz);
     const int gx = wrapi(bx * TX + rx - 2, a.nx), gy = wrapi(by * TY + ry - 2, a.ny);
     const size_t col = (size_t)gy * a.nx + gx;
     const int L = ry * RX + rx;
@@ -260,7 +267,7 @@ __global__ void __launch_bounds__((Geo<TX, TY>::NT)) k_f1(FusedArgs a)
 }
 
 template <int TX, int TY>
-__global__ void __launch_bounds__((Geo<TX, TY>::NT)) k_f2(FusedArgs a)
+__device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
 {
     using G = Geo<TX, TY>;
     constexpr int RX = G::RX, R = G::R;
@@ -270,7 +277,7 @@ __global__ void __launch_bounds__((Geo<TX, TY>::NT)) k_f2(FusedArgs a)
     const bool live = G::cell(threadIdx.x, rx, ry);
     const int ring = live ? G::ring(rx, ry) : 3;
     int bx, by, bz;
-    tile_of(a, bx, by, bz);
+    tile_of(a, item, bx, by, bz);
     const int gx = wrapi(bx * TX + rx - 2, a.nx), gy = wrapi(by * TY + ry - 2, a.ny);
     const size_t col = (size_t)gy * a.nx + gx;
     const int L = ry * RX + rx;
@@ -376,6 +383,38 @@ __global__ void __launch_bounds__((Geo<TX, TY>::NT)) k_f2(FusedArgs a)
     }
 }
 
+template <int TX, int TY, bool DONOR>
+__global__ void __launch_bounds__((Geo<TX, TY>::NT)) k_f1(FusedArgs a, Work w)
+{
+    for (int it = blockIdx.x; it < w.nitems; it += gridDim.x) {
+        int item = item_of(w, it);
+        FusedArgs x = a;
+        if (w.ninst) {
+            int i = find_inst(w.item0, w.ninst, item);
+            x = w.table[i];
+            item -= w.item0[i];
+        }
+        f1_tile<TX, TY, DONOR>(x, item);
+        lds_barrier();  // the next item reuses the LDS planes
+    }
+}
+
+template <int TX, int TY>
+__global__ void __launch_bounds__((Geo<TX, TY>::NT)) k_f2(FusedArgs a, Work w)
+{
+    for (int it = blockIdx.x; it < w.nitems; it += gridDim.x) {
+        int item = item_of(w, it);
+        FusedArgs x = a;
+        if (w.ninst) {
+            int i = find_inst(w.item0, w.ninst, item);
+            x = w.table[i];
+            item -= w.item0[i];
+        }
+        f2_tile<TX, TY>(x, item);
+        lds_barrier();
+    }
+}
+
 // The staged donor (mpd_periodic.hip) serves the unlimited corrective pass.
 constexpr int TPB = 256;
 __global__ void k_donor_faces(int nx, int ny, int nz, const double* __restrict__ U,
@@ -401,35 +440,38 @@ __global__ void k_donor_faces(int nx, int ny, int nz, const double* __restrict__
 // ---------------------------------------------------------------- dispatch ---
 
 template <int TX, int TY>
-int launch_pair(FusedArgs& a, bool f1, bool donor, hipStream_t s)
+int launch_pair(const FusedArgs& a, const Work& w, int grid, bool f1, bool donor, hipStream_t s)
 {
     using G = Geo<TX, TY>;
-    a.tiles_x = (a.nx + TX - 1) / TX;
-    a.tiles_y = (a.ny + TY - 1) / TY;
-    a.nchunks = (a.nz + a.zc - 1) / a.zc;
-    const int nb = a.tiles_x * a.tiles_y * a.nchunks;
-    a.xcd_remap = (nb % 8 == 0) ? 1 : 0;
     if (f1) {
         if (donor)
-            hipLaunchKernelGGL((k_f1<TX, TY, true>), dim3(nb), dim3(G::NT), 0, s, a);
+            hipLaunchKernelGGL((k_f1<TX, TY, true>), dim3(grid), dim3(G::NT), 0, s, a, w);
         else
-            hipLaunchKernelGGL((k_f1<TX, TY, false>), dim3(nb), dim3(G::NT), 0, s, a);
+            hipLaunchKernelGGL((k_f1<TX, TY, false>), dim3(grid), dim3(G::NT), 0, s, a, w);
     } else {
-        hipLaunchKernelGGL((k_f2<TX, TY>), dim3(nb), dim3(G::NT), 0, s, a);
+        hipLaunchKernelGGL((k_f2<TX, TY>), dim3(grid), dim3(G::NT), 0, s, a, w);
     }
     MPD_HIP(hipGetLastError());
     return MPD_OK;
 }
 
+int launch_any(const FusedArgs& a, const Work& w, int grid, int tx, int ty, bool f1, bool donor,
+               hipStream_t s)
+{
+#define MPD_TILE(X, Y) \
+    if (tx == X && ty == Y) return launch_pair<X, Y>(a, w, grid, f1, donor, s);
+    MPD_FUSED_TILES(MPD_TILE)
+#undef MPD_TILE
+    return fail(MPD_ERR_ARG, "tile %dx%d is not one of the compiled tile shapes", tx, ty);
+}
+
+// single grid: one work item per workgroup
 int launch(FusedArgs& a, int tx, int ty, bool f1, bool donor, hipStream_t s)
 {
-    if (tx == 32 && ty == 8) return launch_pair<32, 8>(a, f1, donor, s);
-    if (tx == 64 && ty == 4) return launch_pair<64, 4>(a, f1, donor, s);
-    if (tx == 16 && ty == 16) return launch_pair<16, 16>(a, f1, donor, s);
-    if (tx == 32 && ty == 4) return launch_pair<32, 4>(a, f1, donor, s);
-    if (tx == 64 && ty == 8) return launch_pair<64, 8>(a, f1, donor, s);
-    if (tx == 16 && ty == 8) return launch_pair<16, 8>(a, f1, donor, s);
-    return fail(MPD_ERR_ARG, "tile %dx%d not compiled (32x8 64x4 16x16 32x4 64x8 16x8)", tx, ty);
+    fused_geometry(a, tx, ty);
+    const int nb = a.tiles_x * a.tiles_y * a.nchunks;
+    Work w{nullptr, nullptr, 0, nb, nb % 8 == 0 ? 1 : 0};
+    return launch_any(a, w, nb, tx, ty, f1, donor, s);
 }
 
 // z-chunk length: enough workgroups to fill 256 CUs several times over, long enough
@@ -444,24 +486,42 @@ int choose_zc(const mpd_grid_s* g, int tx, int ty)
 
 }  // namespace
 
+int MPD_CAT(fused_batch_launch, MPD_ARITH)(int tx, int ty, bool f1, bool donor,
+                                           const FusedArgs* table, const int* item0, int ninst,
+                                           int nitems, int grid, hipStream_t s)
+{
+    Work w{table, item0, ninst, nitems, (nitems % 8 == 0 && grid % 8 == 0) ? 1 : 0};
+    FusedArgs none{};
+    return launch_any(none, w, grid, tx, ty, f1, donor, s);
+}
+
+int MPD_CAT(fused_resident_blocks, MPD_ARITH)(int tx, int ty, bool f1)
+{
+    int n = 0;
+    hipError_t e = hipErrorInvalidValue;
+#define MPD_TILE(X, Y)                                                                      \
+    if (tx == X && ty == Y)                                                                 \
+        e = f1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_f1<X, Y, true>,         \
+                                                             Geo<X, Y>::NT, 0)              \
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_f2<X, Y>, Geo<X, Y>::NT, 0);
+    MPD_FUSED_TILES(MPD_TILE)
+#undef MPD_TILE
+    return e == hipSuccess ? n : 0;
+}
+
 int MPD_CAT(periodic_fused_step, MPD_ARITH)(mpd_grid_s* g)
 {
     hipStream_t s = stream();
     int rc;
     const size_t n = g->ncell;
-    for (int d = 0; d < 3; d++) {
-        if ((rc = g->fa[d].alloc(n)) != MPD_OK) return rc;
-        if ((rc = g->fb[d].alloc(n)) != MPD_OK) return rc;
-    }
-    for (int b = 0; b < 2; b++)
-        if ((rc = g->cp[b].alloc(n)) != MPD_OK) return rc;
+    if ((rc = ensure_fused_scratch(g))) return rc;
     int tx = g->tile_x ? g->tile_x : 32, ty = g->tile_y ? g->tile_y : 8;
     FusedArgs a{};
     a.nx = g->nx;
     a.ny = g->ny;
     a.nz = g->nz;
     a.np = (long long)g->nx * g->ny;
-    a.zc = choose_zc(g, tx, ty);
+    a.zc = g->zc ? g->zc : choose_zc(g, tx, ty);
     const double* q = g->has_q ? g->q.p : nullptr;
     double* cin = g->c[g->cur].p;
     double* out = g->c[1 - g->cur].p;
@@ -481,6 +541,7 @@ int MPD_CAT(periodic_fused_step, MPD_ARITH)(mpd_grid_s* g)
         a.Ay = tgt[1].p;
         a.Az = tgt[2].p;
         if ((rc = launch(a, tx, ty, true, p == 1, s))) return rc;
+        trace_mark(g);
         const double* pass_in = (p == 1) ? g->cp[0].p : field;
         if (g->nonosc) {
             a.cn = cin;
@@ -491,12 +552,14 @@ int MPD_CAT(periodic_fused_step, MPD_ARITH)(mpd_grid_s* g)
             a.Ly = last ? nullptr : g->fb[1].p;
             a.Lz = last ? nullptr : g->fb[2].p;
             if ((rc = launch(a, tx, ty, false, false, s))) return rc;
+            trace_mark(g);
             for (int d = 0; d < 3; d++) prev[d] = g->fb[d].p;
         } else {
             hipLaunchKernelGGL(k_donor_faces, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0,
                                s, g->nx, g->ny, g->nz, tgt[0].p, tgt[1].p, tgt[2].p, pass_in,
                                last ? q : (const double*)nullptr, dst);
             MPD_HIP(hipGetLastError());
+            trace_mark(g);
             for (int d = 0; d < 3; d++) prev[d] = tgt[d].p;
         }
         field = dst;

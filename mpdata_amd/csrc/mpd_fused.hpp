@@ -1,0 +1,51 @@
+// Descriptor of one fused-kernel launch over one advection instance, and the entry
+// points shared by the single-grid and batched drivers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace mpd {
+
+// Tile shapes compiled for the fused kernels (x extent, y extent of the column tile).
+#define MPD_FUSED_TILES(X) X(32, 8) X(64, 4) X(16, 16) X(32, 4) X(64, 8) X(16, 8)
+
+struct FusedArgs {
+    int nx, ny, nz;
+    long long np;
+    int tiles_x, tiles_y, zc, nchunks;
+    // F1
+    const double* cin;        // c^n (first pass: the donor field is formed on the fly) or
+                              // the pass input field
+    const double *U, *V, *W;  // previous pass's face velocities
+    double* c1out;            // first pass: the donor field (tile cells)
+    double *Ax, *Ay, *Az;     // unlimited antidiffusive faces
+    // F2
+    const double* cn;         // c^n (limiter bounds)
+    const double* cp;         // pass input field
+    const double* q;          // emission increment or null
+    double* cout;             // pass result
+    double *Lx, *Ly, *Lz;     // limited faces or null
+};
+
+// Tile counts of an instance for a tile shape (zc must be set).
+inline void fused_geometry(FusedArgs& a, int tx, int ty)
+{
+    a.tiles_x = (a.nx + tx - 1) / tx;
+    a.tiles_y = (a.ny + ty - 1) / ty;
+    a.nchunks = (a.nz + a.zc - 1) / a.zc;
+}
+
+inline int fused_items(const FusedArgs& a) { return a.tiles_x * a.tiles_y * a.nchunks; }
+
+// One F1 (f1 = true; donor = first pass) or F2 launch over a device table of instance
+// descriptors; item0[i] = first work item of instance i; grid workgroups stride over
+// the nitems items.
+int fused_batch_launch_exact(int tx, int ty, bool f1, bool donor, const FusedArgs* table,
+                             const int* item0, int ninst, int nitems, int grid, hipStream_t s);
+int fused_batch_launch_fast(int tx, int ty, bool f1, bool donor, const FusedArgs* table,
+                            const int* item0, int ninst, int nitems, int grid, hipStream_t s);
+// Resident workgroups per CU of the F1 (donor) or F2 kernel of a tile shape (0: unknown
+// shape).
+int fused_resident_blocks_exact(int tx, int ty, bool f1);
+int fused_resident_blocks_fast(int tx, int ty, bool f1);
+
+}  // namespace mpd

@@ -1,5 +1,7 @@
 // The device-resident grid object behind mpd_grid, smolar() and msmolar().
 #pragma once
+#include <vector>
+
 #include "mpd_common.hpp"
 
 struct mpd_grid_s {
@@ -9,7 +11,7 @@ struct mpd_grid_s {
     int nonosc = 0;
     int path = 0;  // 0 auto, 1 staged, 2 fused
     int arith = MPD_ARITH_FAST;
-    int tile_x = 0, tile_y = 0;
+    int tile_x = 0, tile_y = 0, zc = 0;  // fused tile shape and z-chunk (0: default)
     size_t ncell = 0;
     size_t nface[3] = {0, 0, 0};  // elements per face family (layout of the mode)
 
@@ -25,6 +27,9 @@ struct mpd_grid_s {
     mpd::DevBuf cp[2];         // intermediate pass fields
     mpd::DevBuf bup, bdn;      // limiter ratios
     mpd::DevBuf red;           // reduction partials
+
+    // when set, every kernel launch of a step is followed by an event on the stream
+    std::vector<hipEvent_t>* trace = nullptr;
 };
 
 namespace mpd {
@@ -38,5 +43,7 @@ int courant_from_winds(mpd_grid_s* g, const double* dx, const double* dy, const 
 int frame_step(mpd_grid_s* g);
 int periodic_step(mpd_grid_s* g);
 int field_stats(mpd_grid_s* g, double* mass, double* cmin, double* cmax);
+// Records a timing event after a kernel launch when g->trace is set.
+void trace_mark(mpd_grid_s* g);
 
 }  // namespace mpd
