@@ -80,6 +80,17 @@ int orc_per_step(int nx, int ny, int nz, const double* U, const double* V, const
 int orc_per_run(int nx, int ny, int nz, const double* U, const double* V, const double* W,
                 int iord, int nonosc, const double* q, double* c, int nsteps);
 
+/* ---------------- derivatives of one periodic step (iord = 2), cpu_adj.c ---------
+ * c_out = A(c_in) + q with fixed face Courant numbers; nonosc selects the limiter.
+ * TLM:     dout = A'(c_in) dc_in + dq           (dq may be NULL)
+ * adjoint: lin = A'(c_in)^T lout; lq += lout     (lq may be NULL) */
+int orc_tlm_step(int nx, int ny, int nz, const double* U, const double* V, const double* W,
+                 int nonosc, const double* c0, const double* dc0, const double* dq,
+                 double* dout);
+int orc_adj_step(int nx, int ny, int nz, const double* U, const double* V, const double* W,
+                 int nonosc, const double* c0, const double* lout, double* lin,
+                 double* lq);
+
 #ifdef __cplusplus
 }
 #endif

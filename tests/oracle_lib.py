@@ -45,6 +45,10 @@ def _bind(lib):
         lib.orc_per_antidiff.argtypes = [C.c_int] * 3 + [PD] * 7
         lib.orc_per_limiter_beta.argtypes = [C.c_int] * 3 + [PD] * 7
         lib.orc_per_limiter_apply.argtypes = [C.c_int] * 3 + [PD] * 5
+        lib.orc_tlm_step.argtypes = [C.c_int] * 3 + [PD] * 3 + [C.c_int] + [PD] * 4
+        lib.orc_tlm_step.restype = C.c_int
+        lib.orc_adj_step.argtypes = [C.c_int] * 3 + [PD] * 3 + [C.c_int] + [PD] * 4
+        lib.orc_adj_step.restype = C.c_int
     if hasattr(lib, "smolar"):
         lib.smolar.argtypes = _SMOLAR_ARGS
         lib.smolar.restype = C.c_int
@@ -112,3 +116,38 @@ def per_courant(nx, ny, nz, dx, dy, dz, dt, u0, u1, v0, v1, w0, w1):
     oracle().orc_per_courant(nx, ny, nz, ptr(dx), ptr(dy), ptr(dz), dt, ptr(u0), ptr(u1),
                              ptr(v0), ptr(v1), ptr(w0), ptr(w1), ptr(U), ptr(V), ptr(W))
     return U, V, W
+
+
+def tlm_step(nx, ny, nz, U, V, W, nonosc, c, dc, dq=None):
+    out = np.empty_like(c)
+    assert oracle().orc_tlm_step(nx, ny, nz, ptr(U), ptr(V), ptr(W), int(nonosc), ptr(c),
+                                 ptr(dc), ptr(dq), ptr(out)) == 0
+    return out
+
+
+def adj_step(nx, ny, nz, U, V, W, nonosc, c, lout, lq=None):
+    lin = np.empty_like(c)
+    assert oracle().orc_adj_step(nx, ny, nz, ptr(U), ptr(V), ptr(W), int(nonosc), ptr(c),
+                                 ptr(lout), ptr(lin), ptr(lq)) == 0
+    return lin
+
+
+def receptor_loss_and_gradient(nx, ny, nz, U, V, W, nonosc, c0, q, nsteps, receptors,
+                               sample_every, obs):
+    """J = 0.5 sum over samples s (steps sample_every, 2*sample_every, ...) and receptor
+    cells r of (c^s_r - obs[s-index, r])^2 for c^{n+1} = A(c^n) + q; returns
+    (J, dJ/dc0, dJ/dq) by a full forward sweep and the oracle adjoint backwards."""
+    states = [np.array(c0, dtype=np.float64)]
+    for _ in range(nsteps):
+        states.append(per_step(nx, ny, nz, U, V, W, 2, nonosc, states[-1], q))
+    J = 0.0
+    lam = np.zeros_like(c0)
+    lq = np.zeros_like(c0)
+    for n in range(nsteps, 0, -1):
+        if n % sample_every == 0:
+            s = n // sample_every - 1
+            r = states[n][receptors] - obs[s]
+            J += 0.5 * float(r @ r)
+            np.add.at(lam, receptors, r)
+        lam = adj_step(nx, ny, nz, U, V, W, nonosc, states[n - 1], lam, lq)
+    return J, lam, lq
