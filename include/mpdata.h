@@ -118,6 +118,25 @@ int mpd_generate_wind(mpd_grid g, const mpd_wind_spec* w);
 int mpd_generate_blobs(mpd_grid g, int target, int nblobs, const double* centres,
                        const double* sigma, const double* amp, int periodic);
 
+/* ------------------------------------------------------------- adjoint --- */
+/* PERIODIC grids with iord = 2 (limiter on or off); the face Courant numbers are fixed
+ * parameters.  One step is c_out = A(c_in) + q.
+ *
+ * One adjoint step on host arrays: lin = A'(c)^T lout, and lq += lout when lq is not
+ * NULL.  Every kink follows the branch the forward step takes. */
+int mpd_adjoint_step(mpd_grid g, const double* c, const double* lout, double* lin, double* lq);
+/* Receptor-misfit loss and gradient.  From the grid's field c^0 and its emission q (zero
+ * if unset), c^{n+1} = A(c^n) + q for n < nsteps, and
+ *   J = 1/2 sum_{s=1}^{nsteps/sample_every} sum_r (c^{s*sample_every}[rec_r] - obs[(s-1)*nrec + r])^2
+ * (rec: distinct cell indices).  The forward keeps c^n every checkpoint_every steps; the
+ * backward sweep recomputes each segment from its checkpoint and applies the adjoint
+ * steps in reverse.  grad_c0 = dJ/dc^0 and grad_q = dJ/dq (host arrays, may be NULL;
+ * both NULL: loss only).  times (may be NULL): forward and backward milliseconds.  The
+ * forward runs in exact arithmetic; on return the grid holds c^nsteps. */
+int mpd_adjoint_run(mpd_grid g, int nsteps, int checkpoint_every, int nrec,
+                    const int* receptors, int sample_every, const double* obs, double* J,
+                    double* grad_c0, double* grad_q, double* times);
+
 /* ------------------------------------------------------------- ensembles --- */
 typedef struct mpd_batch_s* mpd_batch;
 

@@ -81,6 +81,11 @@ SIGNATURES = {
     "mpd_set_arith": (C.c_int, [C.c_void_p, C.c_int]),
     "mpd_step_kernel_times": (C.c_int, [C.c_void_p, C.c_int, PD, C.c_int, PI]),
     "mpd_calibration_copy": (C.c_int, [C.c_size_t, C.c_int, PD]),
+    "mpd_generate_wind": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mpd_generate_blobs": (C.c_int, [C.c_void_p, C.c_int, C.c_int, PD, PD, PD, C.c_int]),
+    "mpd_adjoint_step": (C.c_int, [C.c_void_p, PD, PD, PD, PD]),
+    "mpd_adjoint_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, PI, C.c_int, PD, PD,
+                                  PD, PD, PD]),
     "mpd_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, PI, C.c_int, C.c_int]),
     "mpd_batch_destroy": (C.c_int, [C.c_void_p]),
     "mpd_batch_instance": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
@@ -271,6 +276,29 @@ class Grid:
 
     def set_arith(self, arith):
         _check(lib().mpd_set_arith(self.h, arith))
+
+    def adjoint_step(self, c, lout, lq=None):
+        """lin = A'(c)^T lout (one step); lq += lout in place when given."""
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        lout = np.ascontiguousarray(lout, dtype=np.float64)
+        lin = np.empty(self.n)
+        _check(lib().mpd_adjoint_step(self.h, _p(c), _p(lout), _p(lin), _p(lq)))
+        return lin
+
+    def receptor_gradient(self, nsteps, checkpoint_every, receptors, sample_every, obs,
+                          grads=True):
+        """(J, dJ/dc0, dJ/dq, (forward_ms, backward_ms)) of the receptor-misfit loss from
+        the current field; the grid ends holding c^nsteps."""
+        rec = np.ascontiguousarray(receptors, dtype=np.int32)
+        obs = np.ascontiguousarray(obs, dtype=np.float64).ravel()
+        J = C.c_double(0)
+        g0 = np.empty(self.n) if grads else None
+        gq = np.empty(self.n) if grads else None
+        t = (C.c_double * 2)()
+        _check(lib().mpd_adjoint_run(self.h, nsteps, checkpoint_every, len(rec),
+                                     rec.ctypes.data_as(PI), sample_every, _p(obs),
+                                     C.byref(J), _p(g0), _p(gq), t))
+        return J.value, g0, gq, (t[0], t[1])
 
     def close(self):
         if self.h and self.owned:

@@ -39,19 +39,6 @@ class WindSpec(C.Structure):
                 ("cmax", C.c_double)]
 
 
-def _bind():
-    L = lib()
-    if not getattr(L, "_gen_bound", False):
-        L.mpd_generate_wind.argtypes = [C.c_void_p, C.POINTER(WindSpec)]
-        L.mpd_generate_wind.restype = C.c_int
-        L.mpd_generate_blobs.argtypes = [C.c_void_p, C.c_int, C.c_int,
-                                         C.POINTER(C.c_double), C.POINTER(C.c_double),
-                                         C.POINTER(C.c_double), C.c_int]
-        L.mpd_generate_blobs.restype = C.c_int
-        L._gen_bound = True
-    return L
-
-
 class SplitMix64:
     def __init__(self, seed):
         self.s = seed & MASK
@@ -110,7 +97,7 @@ def random_modes(rng, nmodes, scale):
 
 
 def generate_wind(g, spec):
-    _check(_bind().mpd_generate_wind(g.h, C.byref(spec)))
+    _check(lib().mpd_generate_wind(g.h, C.addressof(spec)))
 
 
 def generate_blobs(g, centres, sigma, amp, periodic=True, target=TARGET_FIELD):
@@ -118,9 +105,8 @@ def generate_blobs(g, centres, sigma, amp, periodic=True, target=TARGET_FIELD):
     s = np.ascontiguousarray(np.asarray(sigma, dtype=np.float64))
     a = np.ascontiguousarray(np.asarray(amp, dtype=np.float64))
     P = C.POINTER(C.c_double)
-    _check(_bind().mpd_generate_blobs(g.h, target, len(s), c.ctypes.data_as(P),
-                                      s.ctypes.data_as(P), a.ctypes.data_as(P),
-                                      int(periodic)))
+    _check(lib().mpd_generate_blobs(g.h, target, len(s), c.ctypes.data_as(P),
+                                    s.ctypes.data_as(P), a.ctypes.data_as(P), int(periodic)))
 
 
 # ------------------------------------------------------------------ configs ---
