@@ -92,6 +92,8 @@ def main():
                     help="instances of the batched-ensemble comparison (0: skip)")
     ap.add_argument("--large", type=int, default=256,
                     help="edge of the large single-grid roofline run (0: skip)")
+    ap.add_argument("--adjoint", type=int, default=256,
+                    help="edge of the source-inversion adjoint leg (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU (every box has one); the "
@@ -214,55 +216,92 @@ def main():
         line["ensemble"] = ensemble_leg(M, args.ensemble)
     if args.large:
         line["large"] = large_leg(M, args.large, args.iord, args.arith)
+    if args.adjoint:
+        line["adjoint"] = adjoint_leg(M, args.adjoint)
     print(json.dumps(line))
 
 
-def ensemble_leg(M, n):
-    """Config 4: n instances of uneven sizes, iord 2 nonoscillatory, 100 steps as one
-    autotuned hipGraph replay against per-instance launch sequences."""
-    import mpdata_amd.configs as K
+def adjoint_leg(M, n, nsteps=512, every=16, K=32, nrec=64):
+    """Config 5: n^3 periodic, deformational wind (seed 5), 32 Gaussian blobs, emission of
+    8 seeded positive Gaussians added every step; 64 seeded receptors sampled every 16
+    steps against a twin run whose emission amplitudes are scaled by (1 + 0.2 xi);
+    gradient of J = 1/2 sum (c - obs)^2 w.r.t. c^0 and the emission, checkpoints every 32
+    steps, iord 2 with the limiter.  Reports the backward/forward time ratio, central
+    differences along a direction scaled by c^0 (limiter on, and the same with the limiter
+    off, where the step is differentiable), and the gradient against the CPU oracle's at
+    32^3 x 64 steps."""
+    import mpdata_amd.configs as K_
+    import oracle_lib as O
 
-    t0 = time.perf_counter()
-    b = K.ensemble(n, seed=4)
-    cells = b.cells()
-    setup = time.perf_counter() - t0
-    tx, ty, zc, pers = b.autotune(2)
-    b.step(2, M.BATCH_GRAPH)
-    ms_graph = b.step_timed(100, M.BATCH_GRAPH) / 100
-    ms_fused = b.step_timed(10, M.BATCH_FUSED) / 10
-    b.step(1, M.BATCH_PER_INSTANCE)
-    ms_per = b.step_timed(10, M.BATCH_PER_INSTANCE) / 10
-    mass_ok = all(np.isfinite(g.stats()[0]) for g in b.instances[:4])
-    b.close()
-    return {"workload": f"{n} periodic instances from {{64^3, 96^3, 128^3, 160x160x96, "
-                        f"192x192x64}} (seed 4), rotation + 10% seeded modes, iord 2 "
-                        "nonoscillatory", "cells": cells, "setup_s": setup,
-            "autotuned": {"tile": [tx, ty], "zc": zc, "persistent": pers},
-            "graph_ms_per_step": ms_graph, "fused_launch_ms_per_step": ms_fused,
-            "per_instance_ms_per_step": ms_per,
-            "graph_cell_updates_per_s": cells / (ms_graph * 1e-3),
-            "per_instance_cell_updates_per_s": cells / (ms_per * 1e-3),
-            "speedup_graph_vs_per_instance": ms_per / ms_graph, "finite": mass_ok}
+    rng = K_.SplitMix64(5005)
+    rec = np.array(sorted({rng.randint(n ** 3) for _ in range(nrec * 2)})[:nrec], np.int32)
+    xi = [rng.uniform(-1.0, 1.0) for _ in range(8)]
 
+    def fresh(nonosc=True, scale=None, size=n):
+        g = M.Grid(size, size, size, M.PERIODIC, iord=2, nonosc=nonosc)
+        cen, sig, amp = K_.deform(g, (size, size, size), seed=5, emission_blobs=8)
+        if scale is not None:
+            K_.generate_blobs(g, cen, sig, [a * f for a, f in zip(amp, scale)],
+                              periodic=True, target=K_.TARGET_EMISSION)
+        return g
 
-def large_leg(M, n, iord, arith):
-    """Roofline of the single-grid step at n^3 (the size the 70%-of-HBM target names)."""
-    import mpdata_amd.configs as K
-
-    g = M.Grid(n, n, n, M.PERIODIC, iord=iord, nonosc=True)
-    g.set_arith(arith)
-    K.ring3d(g, n)
-    g.step(2)
-    ms = g.step_timed(20) / 20
-    kms = g.kernel_times(4)
+    twin = fresh(scale=[1.0 + 0.2 * x for x in xi])
+    obs = twin.sample_receptors(nsteps, rec, every)
+    twin.close()
+    g = fresh()
+    c0 = g.field()
+    J, g0, gq, (tf, tb) = g.receptor_gradient(nsteps, K, rec, every, obs)
     g.close()
-    cells = n ** 3
-    kern = sum(kms)
-    ach = cells * BYTES_PER_CELL / (kern * 1e-3) / 1e9
-    return {"workload": f"periodic {n}^3 rotation, iord {iord} nonoscillatory",
-            "ms_per_step": ms, "cell_updates_per_s": cells / (ms * 1e-3),
-            "kernel_ms": kms, "achieved_GB/s": ach, "frac_of_hbm_peak": ach / HBM_PEAK_GBS}
+    d = np.random.default_rng(5).standard_normal(c0.size) * c0
 
+    def fd_check(nonosc, grad, epss):
+        res = {}
+        for e in epss:
+            vals = []
+            for sgn in (1, -1):
+                gx = fresh(nonosc)
+                gx.set_field(c0 + sgn * e * d)
+                vals.append(gx.receptor_gradient(nsteps, K, rec, every, obs, grads=False)[0])
+                gx.close()
+            fd, an = (vals[0] - vals[1]) / (2 * e), float(grad @ d)
+            res[f"{e:g}"] = {"fd": fd, "adjoint": an,
+                             "rel_diff": abs(fd - an) / max(abs(an), 1e-300)}
+        return res
 
-if __name__ == "__main__":
-    main()
+    fd_lim = fd_check(True, g0, (1e-6, 1e-8))
+    g = fresh(nonosc=False)
+    _, g0u, _, _ = g.receptor_gradient(nsteps, K, rec, every, obs)
+    g.close()
+    fd_unlim = fd_check(False, g0u, (1e-6,))
+
+    # CPU reference gradient at 32^3 x 64 steps (same generators, limiter on)
+    m, ms, mrec = 32, 64, np.array(sorted({rng.randint(32 ** 3) for _ in range(40)})[:16], np.int32)
+    gs = fresh(size=m)
+    U, V, W = gs.get_courant()
+    cs = gs.field()
+    # the emission increment is read back through a zero-wind step: c + q - c
+    gz = M.Grid(m, m, m, M.PERIODIC, iord=2, nonosc=True)
+    z = np.zeros(m ** 3)
+    gz.set_courant(z, z, z)
+    gz.set_field(z)
+    K_.deform(gz, (m, m, m), seed=5, emission_blobs=8)
+    gz.set_courant(z, z, z)
+    gz.set_field(z)
+    gz.step(1)
+    qs = gz.field()
+    gz.close()
+    obs_s = np.zeros((ms // 16, mrec.size))
+    Jg, g0g, gqg, _ = gs.receptor_gradient(ms, 16, mrec, 16, obs_s)
+    gs.close()
+    Jc, g0c, gqc = O.receptor_loss_and_gradient(m, m, m, U, V, W, True, cs, qs, ms, mrec, 16,
+                                                obs_s)
+    cpu = {"grid": [m, m, m], "steps": ms, "J_rel_diff": abs(Jg - Jc) / Jc,
+           "grad_c0_rel_diff": float(np.abs(g0g - g0c).max() / np.abs(g0c).max()),
+           "grad_q_rel_diff": float(np.abs(gqg - gqc).max() / np.abs(gqc).max())}
+    return {"workload": f"periodic {n}^3 deformational wind (seed 5), 8-source emission, "
+                        f"{nsteps} steps, {nrec} receptors every {every} steps, checkpoints "
+                        f"every {K}, iord 2 nonoscillatory",
+            "J": J, "forward_ms_per_step": tf / nsteps, "backward_ms_per_step": tb / nsteps,
+            "adjoint_over_forward": tb / tf,
+            "fd_limiter_on": fd_lim, "fd_limiter_off": fd_unlim, "cpu_reference": cpu,
+            "grad_norms": {"c0": float(np.linalg.norm(g0)), "q": float(np.linalg.norm(gq))}}

@@ -125,6 +125,14 @@ int mpd_generate_blobs(mpd_grid g, int target, int nblobs, const double* centres
  * One adjoint step on host arrays: lin = A'(c)^T lout, and lq += lout when lq is not
  * NULL.  Every kink follows the branch the forward step takes. */
 int mpd_adjoint_step(mpd_grid g, const double* c, const double* lout, double* lin, double* lq);
+/* How the adjoint sums the antidiffusive-velocity contributions: 0 (default) gathers
+ * through per-face coefficient arrays -- deterministic, bit-reproducible; 1 scatters
+ * with hardware FP64 atomics -- no coefficient arrays, last bits vary run to run. */
+int mpd_set_adjoint_scatter(mpd_grid g, int atomic);
+/* Forward run of nsteps steps from the grid's field recording the receptor values every
+ * sample_every steps: out[(s-1)*nrec + r] = c^{s*sample_every}[rec_r]. */
+int mpd_sample_receptors(mpd_grid g, int nsteps, int nrec, const int* receptors,
+                         int sample_every, double* out);
 /* Receptor-misfit loss and gradient.  From the grid's field c^0 and its emission q (zero
  * if unset), c^{n+1} = A(c^n) + q for n < nsteps, and
  *   J = 1/2 sum_{s=1}^{nsteps/sample_every} sum_r (c^{s*sample_every}[rec_r] - obs[(s-1)*nrec + r])^2

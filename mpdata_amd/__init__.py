@@ -83,6 +83,8 @@ SIGNATURES = {
     "mpd_calibration_copy": (C.c_int, [C.c_size_t, C.c_int, PD]),
     "mpd_generate_wind": (C.c_int, [C.c_void_p, C.c_void_p]),
     "mpd_generate_blobs": (C.c_int, [C.c_void_p, C.c_int, C.c_int, PD, PD, PD, C.c_int]),
+    "mpd_set_adjoint_scatter": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpd_sample_receptors": (C.c_int, [C.c_void_p, C.c_int, C.c_int, PI, C.c_int, PD]),
     "mpd_adjoint_step": (C.c_int, [C.c_void_p, PD, PD, PD, PD]),
     "mpd_adjoint_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, PI, C.c_int, PD, PD,
                                   PD, PD, PD]),
@@ -284,6 +286,16 @@ class Grid:
         lin = np.empty(self.n)
         _check(lib().mpd_adjoint_step(self.h, _p(c), _p(lout), _p(lin), _p(lq)))
         return lin
+
+    def set_adjoint_scatter(self, atomic):
+        _check(lib().mpd_set_adjoint_scatter(self.h, int(atomic)))
+
+    def sample_receptors(self, nsteps, receptors, sample_every):
+        rec = np.ascontiguousarray(receptors, dtype=np.int32)
+        out = np.empty((nsteps // sample_every) * len(rec))
+        _check(lib().mpd_sample_receptors(self.h, nsteps, len(rec), rec.ctypes.data_as(PI),
+                                          sample_every, _p(out)))
+        return out.reshape(nsteps // sample_every, len(rec))
 
     def receptor_gradient(self, nsteps, checkpoint_every, receptors, sample_every, obs,
                           grads=True):

@@ -61,12 +61,18 @@ struct Geo {
         j = (int)((t / nx) % ny);
         k = (int)(t / np);
     }
+    // wrapped index of a cell at most two cells outside the grid (every stencil here)
+    __device__ static int wrap2(int a, int n)
+    {
+        a += a < 0 ? n : 0;
+        a += a < 0 ? n : 0;
+        a -= a >= n ? n : 0;
+        a -= a >= n ? n : 0;
+        return a;
+    }
     __device__ size_t at(int i, int j, int k) const
     {
-        i = (i % nx + nx) % nx;
-        j = (j % ny + ny) % ny;
-        k = (k % nz + nz) % nz;
-        return (size_t)k * np + (size_t)j * nx + i;
+        return (size_t)wrap2(k, nz) * np + (size_t)wrap2(j, ny) * nx + wrap2(i, nx);
     }
 };
 
@@ -275,13 +281,14 @@ __global__ void k_adj_donor2(Geo g, AdjBufs b, const double* __restrict__ lout,
     if (lq) lq[t] += lout[t];
 }
 
-// limiter transposed, part 1: lA = m * lAl on the upper faces; ratio adjoints lbu, lbd
-// gathered from the six faces of the cell
+// limiter transposed, part 1: lA = m * lAl on the upper faces; the ratio adjoints,
+// gathered from the six faces of the cell, turned into bound and flux-sum adjoints
 __global__ void k_adj_limit1(Geo g, AdjBufs b, const double* __restrict__ lAl0,
                              const double* __restrict__ lAl1, const double* __restrict__ lAl2,
                              double* __restrict__ lA0, double* __restrict__ lA1,
-                             double* __restrict__ lA2, double* __restrict__ lbu,
-                             double* __restrict__ lbd)
+                             double* __restrict__ lA2, double* __restrict__ iu,
+                             double* __restrict__ id, double* __restrict__ lain,
+                             double* __restrict__ laout)
 {
     size_t t = (size_t)blockIdx.x * TPB + threadIdx.x;
     if (t >= g.n) return;
@@ -316,31 +323,21 @@ __global__ void k_adj_limit1(Geo g, AdjBufs b, const double* __restrict__ lAl0,
             if (pick(b.bup[l], b.bdn[t]) == 2) sd += a * L;
         }
     }
-    lbu[t] = su;
-    lbd[t] = sd;
-}
-
-// limiter transposed, part 2 (pointwise): ratio adjoints -> bound / flux-sum adjoints
-__global__ void k_adj_limit2(Geo g, AdjBufs b, const double* __restrict__ lbu,
-                             const double* __restrict__ lbd, double* __restrict__ iu,
-                             double* __restrict__ id, double* __restrict__ lain,
-                             double* __restrict__ laout)
-{
-    size_t t = (size_t)blockIdx.x * TPB + threadIdx.x;
-    if (t >= g.n) return;
-    double u = 0.0, dn = 0.0, li = 0.0, lo = 0.0;
-    if (lbu[t] != 0.0) {
-        u = lbu[t] / (b.ain[t] + MINV);
+    // ratio adjoints -> bound / flux-sum adjoints (pointwise).  A ratio that was not
+    // selected carries no adjoint and may be inf: skipped.
+    double u = 0.0, dn = 0.0, li = 0.0, lo2 = 0.0;
+    if (su != 0.0) {
+        u = su / (b.ain[t] + MINV);
         li = -u * b.bup[t];
     }
-    if (lbd[t] != 0.0) {
-        dn = lbd[t] / (b.aout[t] + MINV);
-        lo = -dn * b.bdn[t];
+    if (sd != 0.0) {
+        dn = sd / (b.aout[t] + MINV);
+        lo2 = -dn * b.bdn[t];
     }
     iu[t] = u;
     id[t] = dn;
     lain[t] = li;
-    laout[t] = lo;
+    laout[t] = lo2;
 }
 
 // limiter transposed, part 3 (gather): bounds -> c_in / c1; flux sums -> c1 and the
@@ -438,6 +435,50 @@ __global__ void k_adj_anti1(Geo g, AdjBufs b, const double* __restrict__ lA0,
     }
 }
 
+// Scatter variant of parts 1 + 2: each face adds its ten contributions to l1 with
+// hardware FP64 atomics (no coefficient arrays; summation order, and so the last bits,
+// vary between runs).
+__global__ void k_adj_anti_atomic(Geo g, AdjBufs b, const double* __restrict__ lA0,
+                                  const double* __restrict__ lA1, const double* __restrict__ lA2,
+                                  double* __restrict__ l1)
+{
+    size_t t = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (t >= g.n) return;
+    int i, j, k;
+    g.ijk(t, i, j, k);
+    const double* lA[3] = {lA0, lA1, lA2};
+    const double* c = b.c1;
+    for (int fam = 0; fam < 3; fam++) {
+        double L = lA[fam][t];
+        if (L == 0.0) continue;
+        FaceSt f = face_st(g, b.U, b.V, b.W, i, j, k, fam);
+        double p = c[f.p], m = c[f.m];
+        double K = ab(f.a) * (1.0 - ab(f.a));
+        double den = p + m + MINV;
+        double n1 = c[f.up1[0]] + c[f.up1[1]] - c[f.dn1[0]] - c[f.dn1[1]];
+        double D1 = c[f.up1[0]] + c[f.up1[1]] + c[f.dn1[0]] + c[f.dn1[1]] + MINV;
+        double n2 = c[f.up2[0]] + c[f.up2[1]] - c[f.dn2[0]] - c[f.dn2[1]];
+        double D2 = c[f.up2[0]] + c[f.up2[1]] + c[f.dn2[0]] + c[f.dn2[1]] + MINV;
+        double gb = -0.5 * f.a;
+        double kp = L * (K * (((2.0 * m + MINV) / den) / den));
+        double km = L * (-K * (((2.0 * p + MINV) / den) / den));
+        double ku1 = L * (gb * f.s1 * (((D1 - n1) / D1) / D1));
+        double kd1 = L * (gb * f.s1 * (((-D1 - n1) / D1) / D1));
+        double ku2 = L * (gb * f.s2 * (((D2 - n2) / D2) / D2));
+        double kd2 = L * (gb * f.s2 * (((-D2 - n2) / D2) / D2));
+        unsafeAtomicAdd(l1 + f.p, kp);
+        unsafeAtomicAdd(l1 + f.m, km);
+        unsafeAtomicAdd(l1 + f.up1[0], ku1);
+        unsafeAtomicAdd(l1 + f.up1[1], ku1);
+        unsafeAtomicAdd(l1 + f.dn1[0], kd1);
+        unsafeAtomicAdd(l1 + f.dn1[1], kd1);
+        unsafeAtomicAdd(l1 + f.up2[0], ku2);
+        unsafeAtomicAdd(l1 + f.up2[1], ku2);
+        unsafeAtomicAdd(l1 + f.dn2[0], kd2);
+        unsafeAtomicAdd(l1 + f.dn2[1], kd2);
+    }
+}
+
 // part 2: every cell gathers the coefficients of the faces whose stencil contains it
 // (the stencils of face_st read backwards)
 __global__ void k_adj_anti2(Geo g, double* const* __restrict__ coef, double* __restrict__ l1)
@@ -507,7 +548,7 @@ __global__ void k_receptors(const double* __restrict__ c, const int* __restrict_
 }
 
 struct Work {
-    DevBuf c1, A[3], bup, bdn, ain, aout, lAl[3], lA[3], lbu, lbd, iu, id, lain, laout, l1;
+    DevBuf c1, A[3], bup, bdn, ain, aout, lAl[3], lA[3], iu, id, lain, laout, l1;
     DevBuf coef[18];
     int* code = nullptr;
     double** dcoef = nullptr;
@@ -522,7 +563,7 @@ struct Work {
         int rc;
         n = cells;
         DevBuf* all[] = {&c1, &A[0], &A[1], &A[2], &bup, &bdn, &ain, &aout, &lAl[0], &lAl[1],
-                         &lAl[2], &lA[0], &lA[1], &lA[2], &lbu, &lbd, &iu, &id, &lain, &laout, &l1};
+                         &lAl[2], &lA[0], &lA[1], &lA[2], &iu, &id, &lain, &laout, &l1};
         for (DevBuf* d : all)
             if ((rc = d->alloc(cells))) return rc;
         for (auto& c : coef)
@@ -556,10 +597,8 @@ int adjoint_step(mpd_grid_s* g, Work& w, const double* cin, const double* lout, 
                        w.lAl[1].p, w.lAl[2].p, lq);
     if (g->nonosc) {
         hipLaunchKernelGGL(k_adj_limit1, dim3(nb), dim3(TPB), 0, s, geo, b, w.lAl[0].p,
-                           w.lAl[1].p, w.lAl[2].p, w.lA[0].p, w.lA[1].p, w.lA[2].p, w.lbu.p,
-                           w.lbd.p);
-        hipLaunchKernelGGL(k_adj_limit2, dim3(nb), dim3(TPB), 0, s, geo, b, w.lbu.p, w.lbd.p,
-                           w.iu.p, w.id.p, w.lain.p, w.laout.p);
+                           w.lAl[1].p, w.lAl[2].p, w.lA[0].p, w.lA[1].p, w.lA[2].p, w.iu.p,
+                           w.id.p, w.lain.p, w.laout.p);
         hipLaunchKernelGGL(k_adj_limit3, dim3(nb), dim3(TPB), 0, s, geo, b, w.iu.p, w.id.p,
                            w.lain.p, w.laout.p, lin, w.l1.p, w.lA[0].p, w.lA[1].p, w.lA[2].p);
     } else {
@@ -567,9 +606,14 @@ int adjoint_step(mpd_grid_s* g, Work& w, const double* cin, const double* lout, 
             MPD_HIP(hipMemcpyAsync(w.lA[d].p, w.lAl[d].p, g->ncell * 8, hipMemcpyDeviceToDevice, s));
         MPD_HIP(hipMemsetAsync(lin, 0, g->ncell * 8, s));
     }
-    hipLaunchKernelGGL(k_adj_anti1, dim3(nb), dim3(TPB), 0, s, geo, b, w.lA[0].p, w.lA[1].p,
-                       w.lA[2].p, w.dcoef);
-    hipLaunchKernelGGL(k_adj_anti2, dim3(nb), dim3(TPB), 0, s, geo, w.dcoef, w.l1.p);
+    if (g->adj_scatter) {
+        hipLaunchKernelGGL(k_adj_anti_atomic, dim3(nb), dim3(TPB), 0, s, geo, b, w.lA[0].p,
+                           w.lA[1].p, w.lA[2].p, w.l1.p);
+    } else {
+        hipLaunchKernelGGL(k_adj_anti1, dim3(nb), dim3(TPB), 0, s, geo, b, w.lA[0].p,
+                           w.lA[1].p, w.lA[2].p, w.dcoef);
+        hipLaunchKernelGGL(k_adj_anti2, dim3(nb), dim3(TPB), 0, s, geo, w.dcoef, w.l1.p);
+    }
     hipLaunchKernelGGL(k_adj_donor1, dim3(nb), dim3(TPB), 0, s, geo, b, w.l1.p, lin);
     MPD_HIP(hipGetLastError());
     return MPD_OK;
@@ -578,6 +622,14 @@ int adjoint_step(mpd_grid_s* g, Work& w, const double* cin, const double* lout, 
 }  // namespace
 
 // One adjoint step on host arrays (for tests): lin = A'(c)^T lout, lq += lout.
+extern "C" int mpd_set_adjoint_scatter(mpd_grid g, int atomic)
+{
+    clear_error();
+    if (!g) return fail(MPD_ERR_ARG, "mpd_set_adjoint_scatter: NULL grid");
+    g->adj_scatter = atomic ? 1 : 0;
+    return MPD_OK;
+}
+
 extern "C" int mpd_adjoint_step(mpd_grid g, const double* c, const double* lout, double* lin,
                                 double* lq)
 {
@@ -600,6 +652,47 @@ extern "C" int mpd_adjoint_step(mpd_grid g, const double* c, const double* lout,
     if ((rc = adjoint_step(g, w, dc.p, dl.p, di.p, lq ? dq.p : nullptr))) return rc;
     MPD_HIP(hipMemcpyAsync(lin, di.p, g->ncell * 8, hipMemcpyDeviceToHost, s));
     if (lq) MPD_HIP(hipMemcpyAsync(lq, dq.p, g->ncell * 8, hipMemcpyDeviceToHost, s));
+    MPD_HIP(hipStreamSynchronize(s));
+    return MPD_OK;
+}
+
+// Forward run from the grid's field recording the receptor values every sample_every
+// steps: out[(s-1)*nrec + r] = c^{s*sample_every}[rec_r].  The grid ends at c^nsteps.
+extern "C" int mpd_sample_receptors(mpd_grid g, int nsteps, int nrec, const int* receptors,
+                                    int sample_every, double* out)
+{
+    clear_error();
+    if (!g || nsteps < 1 || nrec < 1 || !receptors || sample_every < 1 || !out)
+        return fail(MPD_ERR_ARG, "mpd_sample_receptors: bad arguments");
+    if (!g->has_winds || !g->has_field)
+        return fail(MPD_ERR_ARG, "mpd_sample_receptors: set the Courant numbers and the field first");
+    for (int q = 0; q < nrec; q++)
+        if (receptors[q] < 0 || (size_t)receptors[q] >= g->ncell)
+            return fail(MPD_ERR_ARG, "receptor index out of range");
+    const int nsamp = nsteps / sample_every;
+    hipStream_t s = stream();
+    int* drec = nullptr;
+    MPD_HIP(hipMalloc((void**)&drec, nrec * sizeof(int)));
+    struct FreeI {
+        int* p;
+        ~FreeI() { if (p) (void)hipFree(p); }
+    } freei{drec};
+    MPD_HIP(hipMemcpy(drec, receptors, nrec * sizeof(int), hipMemcpyHostToDevice));
+    DevBuf zero, vals;
+    int rc;
+    if ((rc = zero.alloc(nrec)) || (rc = vals.alloc((size_t)std::max(nsamp, 1) * nrec))) return rc;
+    std::vector<double> z(nrec, 0.0);
+    MPD_HIP(hipMemcpy(zero.p, z.data(), nrec * 8, hipMemcpyHostToDevice));
+    for (int step = 0; step < nsteps; step++) {
+        if ((rc = periodic_step(g))) return rc;
+        if ((step + 1) % sample_every == 0) {
+            int sm = (step + 1) / sample_every - 1;
+            hipLaunchKernelGGL(k_receptors, dim3(nblk(nrec)), dim3(TPB), 0, s, g->c[g->cur].p,
+                               drec, nrec, zero.p, vals.p + (size_t)sm * nrec, (double*)nullptr);
+        }
+    }
+    if (nsamp)
+        MPD_HIP(hipMemcpyAsync(out, vals.p, (size_t)nsamp * nrec * 8, hipMemcpyDeviceToHost, s));
     MPD_HIP(hipStreamSynchronize(s));
     return MPD_OK;
 }

@@ -11,6 +11,7 @@ struct mpd_grid_s {
     int nonosc = 0;
     int path = 0;  // 0 auto, 1 staged, 2 fused
     int arith = MPD_ARITH_FAST;
+    int adj_scatter = 0;  // adjoint: 0 deterministic gathers, 1 atomic scatter
     int tile_x = 0, tile_y = 0, zc = 0;  // fused tile shape and z-chunk (0: default)
     size_t ncell = 0;
     size_t nface[3] = {0, 0, 0};  // elements per face family (layout of the mode)

@@ -1,0 +1,78 @@
+"""The ring2d / ring3d drivers (C programs linked against libsmolar_hip.so).
+
+CPU: with zero steps a driver only writes its initial state, so the restated set-up
+(tests/init.c:15-103 and the reference generators) is checked against the inputs the
+compiled reference generators produced for tests/golden/ring2d_32.npz -- bitwise.
+GPU: the trajectory ring2d.bin after 1, 10 and 50 steps equals the reference's, and
+ring3d's equals the oracle's frame-mode steps from the same initial state.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import mpdata_amd as M
+import oracle_lib as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "mpdata_amd", "bin")
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def run(prog, n, nt, cwd):
+    return subprocess.run([os.path.join(BIN, prog), str(n), str(nt)], cwd=cwd,
+                          capture_output=True, text=True, timeout=300)
+
+
+def test_ring2d_initial_state_matches_reference_generators(tmp_path):
+    r = run("ring2d", 32, 0, tmp_path)
+    assert r.returncode == 0, r.stderr
+    d = dict(np.load(os.path.join(GOLD, "ring2d_32.npz")))
+    raw = np.fromfile(tmp_path / "ring2d.bin")
+    n2 = 32 * 32
+    assert raw.size == 3 * n2
+    np.testing.assert_array_equal(raw[:n2], d["u"][n2:2 * n2])
+    np.testing.assert_array_equal(raw[n2:2 * n2], d["v"][n2:2 * n2])
+    np.testing.assert_array_equal(raw[2 * n2:], d["c"][n2:2 * n2])
+    assert (tmp_path / "ring2d.ctl").read_text().startswith("DSET ^ring2d.bin")
+
+
+def test_driver_usage_message(tmp_path):
+    r = subprocess.run([os.path.join(BIN, "ring3d")], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 0 and "ring3d <n> <nt>" in r.stdout
+
+
+@pytest.mark.gpu
+def test_ring2d_trajectory_matches_reference(tmp_path):
+    if M.device_count() < 1:
+        pytest.skip("no GPU")
+    r = run("ring2d", 32, 50, tmp_path)
+    assert r.returncode == 0, r.stderr
+    d = dict(np.load(os.path.join(GOLD, "ring2d_32.npz")))
+    n2 = 32 * 32
+    raw = np.fromfile(tmp_path / "ring2d.bin").reshape(51, 3, n2)
+    for s in (1, 10, 50):
+        np.testing.assert_array_equal(raw[s, 2], d[f"out_s{s}"][n2:2 * n2])
+    lines = r.stdout.splitlines()
+    assert lines[0] == "step 0" and lines[-1].startswith("step 50 ")
+
+
+@pytest.mark.gpu
+def test_ring3d_matches_oracle(tmp_path):
+    if M.device_count() < 1:
+        pytest.skip("no GPU")
+    n, nt = 16, 3
+    r = run("ring3d", n, nt, tmp_path)
+    assert r.returncode == 0, r.stderr
+    nn = n ** 3
+    raw = np.fromfile(tmp_path / "ring3d.bin")
+    u, v, w, c = (raw[i * nn:(i + 1) * nn].copy() for i in range(4))
+    one = np.ones(n)
+    c0, c1 = c.copy(), np.zeros(nn)
+    for s in range(nt):
+        _, a0, a1 = O.call_smolar(O.oracle().orc_smolar, n, n, n, one, one, one, 0.0125, u, u,
+                                  v, v, w, w, c0, c1, 3)
+        c0, c1 = a1, a0
+        np.testing.assert_array_equal(raw[(4 + s) * nn:(5 + s) * nn], c0)

@@ -128,3 +128,21 @@ def test_receptor_gradient_matches_central_differences(nonosc):
         fd, an = (Jp - Jm) / (2 * eps), float(gq @ dq)
         print(f"q direction {trial}: fd {fd:.12e} adjoint {an:.12e}", flush=True)
         assert abs(fd - an) <= 1e-6 * max(abs(an), 1e-3)
+
+
+def test_atomic_scatter_matches_gather():
+    nx, ny, nz = 14, 12, 10
+    nsteps, every = 24, 4
+    rng, U, V, W, c, q, rec = loss_setup(nx, ny, nz, 21, True)
+    obs = rng.uniform(0, 1, (nsteps // every, rec.size))
+    out = []
+    for atomic in (False, True):
+        g = grid(nx, ny, nz, True, U, V, W)
+        g.set_adjoint_scatter(atomic)
+        g.set_field(c)
+        g.set_emission(q)
+        out.append(g.receptor_gradient(nsteps, 6, rec, every, obs))
+        g.close()
+    assert out[0][0] == out[1][0]
+    for a, b in ((out[0][1], out[1][1]), (out[0][2], out[1][2])):
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
