@@ -305,3 +305,7 @@ def adjoint_leg(M, n, nsteps=512, every=16, K=32, nrec=64):
             "adjoint_over_forward": tb / tf,
             "fd_limiter_on": fd_lim, "fd_limiter_off": fd_unlim, "cpu_reference": cpu,
             "grad_norms": {"c0": float(np.linalg.norm(g0)), "q": float(np.linalg.norm(gq))}}
+
+
+if __name__ == "__main__":
+    main()
