@@ -221,6 +221,53 @@ def main():
     print(json.dumps(line))
 
 
+def ensemble_leg(M, n):
+    """Config 4: n instances of uneven sizes, iord 2 nonoscillatory, 100 steps as one
+    autotuned hipGraph replay against per-instance launch sequences."""
+    import mpdata_amd.configs as K
+
+    t0 = time.perf_counter()
+    b = K.ensemble(n, seed=4)
+    cells = b.cells()
+    setup = time.perf_counter() - t0
+    tx, ty, zc, pers = b.autotune(2)
+    b.step(2, M.BATCH_GRAPH)
+    ms_graph = b.step_timed(100, M.BATCH_GRAPH) / 100
+    ms_fused = b.step_timed(10, M.BATCH_FUSED) / 10
+    b.step(1, M.BATCH_PER_INSTANCE)
+    ms_per = b.step_timed(10, M.BATCH_PER_INSTANCE) / 10
+    mass_ok = all(np.isfinite(g.stats()[0]) for g in b.instances[:4])
+    b.close()
+    return {"workload": f"{n} periodic instances from {{64^3, 96^3, 128^3, 160x160x96, "
+                        f"192x192x64}} (seed 4), rotation + 10% seeded modes, iord 2 "
+                        "nonoscillatory", "cells": cells, "setup_s": setup,
+            "autotuned": {"tile": [tx, ty], "zc": zc, "persistent": pers},
+            "graph_ms_per_step": ms_graph, "fused_launch_ms_per_step": ms_fused,
+            "per_instance_ms_per_step": ms_per,
+            "graph_cell_updates_per_s": cells / (ms_graph * 1e-3),
+            "per_instance_cell_updates_per_s": cells / (ms_per * 1e-3),
+            "speedup_graph_vs_per_instance": ms_per / ms_graph, "finite": mass_ok}
+
+
+def large_leg(M, n, iord, arith):
+    """Roofline of the single-grid step at n^3 (the size the 70%-of-HBM target names)."""
+    import mpdata_amd.configs as K
+
+    g = M.Grid(n, n, n, M.PERIODIC, iord=iord, nonosc=True)
+    g.set_arith(arith)
+    K.ring3d(g, n)
+    g.step(2)
+    ms = g.step_timed(20) / 20
+    kms = g.kernel_times(4)
+    g.close()
+    cells = n ** 3
+    kern = sum(kms)
+    ach = cells * BYTES_PER_CELL / (kern * 1e-3) / 1e9
+    return {"workload": f"periodic {n}^3 rotation, iord {iord} nonoscillatory",
+            "ms_per_step": ms, "cell_updates_per_s": cells / (ms * 1e-3),
+            "kernel_ms": kms, "achieved_GB/s": ach, "frac_of_hbm_peak": ach / HBM_PEAK_GBS}
+
+
 def adjoint_leg(M, n, nsteps=512, every=16, K=32, nrec=64):
     """Config 5: n^3 periodic, deformational wind (seed 5), 32 Gaussian blobs, emission of
     8 seeded positive Gaussians added every step; 64 seeded receptors sampled every 16
