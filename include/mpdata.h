@@ -85,13 +85,23 @@ int mpd_calibration_copy(size_t n, int reps, double* ms);
  * failure sentinel for long runs (MPD_ERR_NUMERIC if any value is non-finite). */
 int mpd_field_stats(mpd_grid g, double* mass, double* cmin, double* cmax);
 /* Kernel-path selection for PERIODIC grids: 0 = automatic (fused), 1 = staged
- * (one kernel per stage, intermediates in HBM), 2 = fused. */
+ * (one kernel per stage, intermediates in HBM), 2 = fused.  Grids with more than
+ * 2^29 cells per z-plane always take the staged path. */
 int mpd_set_path(mpd_grid g, int path);
 /* Arithmetic of the PERIODIC fused kernels: MPD_ARITH_FAST (default) or
  * MPD_ARITH_EXACT.  FRAME grids and the staged path always compute exactly. */
 int mpd_set_arith(mpd_grid g, int arith);
-/* Tile shape of the fused kernel (0,0 = autotuned default). */
+/* Tile shape of the fused kernels (0,0 = the default 32x8); one of the compiled shapes
+ * 32x8, 16x16, 64x8, 16x8, 32x16. */
 int mpd_set_tile(mpd_grid g, int tile_x, int tile_y);
+/* Planes per workgroup of the fused kernels (0 = chosen from the grid size). */
+int mpd_set_zchunk(mpd_grid g, int zc);
+/* Times every tile shape x z-chunk length on the grid itself (one warm-up step and nsteps
+ * timed steps each), keeps the fastest and reports it; the field buffers are restored, so
+ * the grid's state is unchanged.  Results are cached per process by shape and options,
+ * and in the file named by the environment variable MPD_TUNE_CACHE when set.  PERIODIC
+ * grids with iord >= 2 on the fused path only (MPD_ERR_UNSUPPORTED otherwise). */
+int mpd_autotune(mpd_grid g, int nsteps, int* tile_x, int* tile_y, int* zc);
 
 /* ------------------------------------------------------------ generators --- */
 #define MPD_MAX_MODES 8

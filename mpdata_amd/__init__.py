@@ -78,6 +78,8 @@ SIGNATURES = {
     "mpd_field_stats": (C.c_int, [C.c_void_p, PD, PD, PD]),
     "mpd_set_path": (C.c_int, [C.c_void_p, C.c_int]),
     "mpd_set_tile": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "mpd_set_zchunk": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpd_autotune": (C.c_int, [C.c_void_p, C.c_int, PI, PI, PI]),
     "mpd_set_arith": (C.c_int, [C.c_void_p, C.c_int]),
     "mpd_step_kernel_times": (C.c_int, [C.c_void_p, C.c_int, PD, C.c_int, PI]),
     "mpd_calibration_copy": (C.c_int, [C.c_size_t, C.c_int, PD]),
@@ -268,6 +270,16 @@ class Grid:
 
     def set_tile(self, tx, ty):
         _check(lib().mpd_set_tile(self.h, tx, ty))
+
+    def set_zchunk(self, zc):
+        _check(lib().mpd_set_zchunk(self.h, zc))
+
+    def autotune(self, nsteps=2):
+        """Picks the fastest tile shape and z-chunk length for this grid (state kept);
+        returns (tile_x, tile_y, zc)."""
+        v = [C.c_int() for _ in range(3)]
+        _check(lib().mpd_autotune(self.h, nsteps, *(C.byref(x) for x in v)))
+        return tuple(x.value for x in v)
 
     def kernel_times(self, nsteps, max_kernels=16):
         """Average duration (ms) of each kernel of a step over nsteps steps."""

@@ -2,9 +2,11 @@
 // reference (smolar.h:45-48, 95-99, 132-136) and the device-resident mpd_* interface.
 #include <cstring>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "../../include/smolar.h"
+#include "mpd_fused.hpp"
 #include "mpd_grid.hpp"
 
 namespace mpd {
@@ -74,7 +76,7 @@ static int step_once(mpd_grid_s* g)
 // staged kernel already.
 int periodic_step(mpd_grid_s* g)
 {
-    if (g->path != 1 && g->iord >= 2)
+    if (g->path != 1 && g->iord >= 2 && fused_plane_fits(g->nx, g->ny))
         return g->arith == MPD_ARITH_FAST ? periodic_fused_step_fast(g)
                                           : periodic_fused_step_exact(g);
     return periodic_staged_step(g);
@@ -530,5 +532,83 @@ extern "C" int mpd_set_tile(mpd_grid g, int tile_x, int tile_y)
     if (tile_x < 0 || tile_y < 0) return fail(MPD_ERR_ARG, "mpd_set_tile: negative tile");
     g->tile_x = tile_x;
     g->tile_y = tile_y;
+    return MPD_OK;
+}
+
+extern "C" int mpd_set_zchunk(mpd_grid g, int zc)
+{
+    GRID_OR_FAIL(g);
+    if (zc < 0) return fail(MPD_ERR_ARG, "mpd_set_zchunk: negative z-chunk length");
+    g->zc = zc;
+    return MPD_OK;
+}
+
+// Autotuner of one grid: times every compiled tile shape x z-chunk length on the grid
+// itself (one warm-up step, then nsteps timed steps each), keeps the fastest, and
+// restores both field buffers and their order, so tuning leaves the grid unchanged.
+// Results are cached by the grid's shape and options (see tune_lookup).
+extern "C" int mpd_autotune(mpd_grid g, int nsteps, int* tile_x, int* tile_y, int* zc)
+{
+    GRID_OR_FAIL(g);
+    if (nsteps < 1) return fail(MPD_ERR_ARG, "mpd_autotune: nsteps < 1");
+    if (g->boundary != MPD_PERIODIC || g->iord < 2 || g->path == 1 ||
+        !fused_plane_fits(g->nx, g->ny))
+        return fail(MPD_ERR_UNSUPPORTED, "mpd_autotune: tunes the fused PERIODIC path only");
+    if (!g->has_winds || !g->has_field)
+        return fail(MPD_ERR_ARG, "mpd_autotune: set the winds (or Courant numbers) and the field first");
+    const std::string key = "grid:i" + std::to_string(g->iord) + "n" + std::to_string(g->nonosc) +
+                            "a" + std::to_string(g->arith) + "q" + std::to_string(g->has_q) +
+                            ":" + std::to_string(g->nx) + "x" + std::to_string(g->ny) + "x" +
+                            std::to_string(g->nz);
+    int rc;
+    int c[3];
+    if (!tune_lookup(key, 3, c)) {
+        hipStream_t s = stream();
+        DevBuf save[2];
+        for (int b = 0; b < 2; b++) {
+            if ((rc = save[b].alloc(g->ncell))) return rc;
+            MPD_HIP(hipMemcpyAsync(save[b].p, g->c[b].p, g->ncell * 8, hipMemcpyDeviceToDevice, s));
+        }
+        const int cur = g->cur;
+        const int shapes[][2] = {
+#define MPD_TILE(X, Y) {X, Y},
+            MPD_FUSED_TILES(MPD_TILE)
+#undef MPD_TILE
+        };
+        double best = 1e300;
+        int bestc[3] = {g->tile_x, g->tile_y, g->zc};
+        for (auto& sh : shapes) {
+            int last = 0;
+            for (int z : {8, 11, 16, 22, 32, 43, 64}) {
+                z = z < g->nz ? z : g->nz;
+                if (z == last) continue;
+                last = z;
+                g->tile_x = sh[0];
+                g->tile_y = sh[1];
+                g->zc = z;
+                double ms = 0;
+                if ((rc = mpd_step(g, 1))) return rc;
+                if ((rc = mpd_step_timed(g, nsteps, &ms))) return rc;
+                if (ms < best) {
+                    best = ms;
+                    bestc[0] = sh[0];
+                    bestc[1] = sh[1];
+                    bestc[2] = z;
+                }
+            }
+        }
+        for (int b = 0; b < 2; b++)
+            MPD_HIP(hipMemcpyAsync(g->c[b].p, save[b].p, g->ncell * 8, hipMemcpyDeviceToDevice, s));
+        g->cur = cur;
+        MPD_HIP(hipStreamSynchronize(s));
+        tune_store(key, 3, bestc);
+        for (int i = 0; i < 3; i++) c[i] = bestc[i];
+    }
+    g->tile_x = c[0];
+    g->tile_y = c[1];
+    g->zc = c[2];
+    if (tile_x) *tile_x = c[0];
+    if (tile_y) *tile_y = c[1];
+    if (zc) *zc = c[2];
     return MPD_OK;
 }

@@ -271,6 +271,9 @@ extern "C" int mpd_batch_create(mpd_batch* out, int ninst, const int* dims, int 
         b->g.push_back(g);
         int rc = grid_init(g, dims[3 * i], dims[3 * i + 1], dims[3 * i + 2], MPD_PERIODIC, iord,
                            nonosc);
+        if (rc == MPD_OK && !fused_plane_fits(g->nx, g->ny))
+            rc = fail(MPD_ERR_ARG, "mpd_batch_create: instance %d has more than 2^29 cells "
+                                   "per z-plane", i);
         if (rc == MPD_OK) rc = ensure_fused_scratch(g);
         if (rc) {
             delete b;
@@ -397,28 +400,14 @@ extern "C" int mpd_batch_autotune(mpd_batch b, int nsteps, int* tx, int* ty, int
 {
     BATCH_OR_FAIL(b);
     if (nsteps < 1) return fail(MPD_ERR_ARG, "mpd_batch_autotune: nsteps < 1");
-    static std::map<std::string, std::vector<int>> cache;
-    std::string key = "i" + std::to_string(b->iord) + "a" + std::to_string(b->arith);
+    std::string key = "batch:i" + std::to_string(b->iord) + "n" + std::to_string(b->nonosc) +
+                      "a" + std::to_string(b->arith);
     for (auto* g : b->g)
         key += ":" + std::to_string(g->nx) + "x" + std::to_string(g->ny) + "x" +
                std::to_string(g->nz);
-    const char* path = getenv("MPD_TUNE_CACHE");
-    if (!cache.count(key) && path) {
-        if (FILE* f = fopen(path, "r")) {
-            char line[1 << 16];
-            while (fgets(line, sizeof line, f)) {
-                char* sp = strchr(line, ' ');
-                if (!sp) continue;
-                *sp = 0;
-                int a, c, d, e;
-                if (key == line && sscanf(sp + 1, "%d %d %d %d", &a, &c, &d, &e) == 4)
-                    cache[key] = {a, c, d, e};
-            }
-            fclose(f);
-        }
-    }
     int rc;
-    if (!cache.count(key)) {
+    int c[4];
+    if (!tune_lookup(key, 4, c)) {
         std::vector<DevBuf> save(b->ninst);
         hipStream_t s = stream();
         for (int i = 0; i < b->ninst; i++) {
@@ -428,14 +417,14 @@ extern "C" int mpd_batch_autotune(mpd_batch b, int nsteps, int* tx, int* ty, int
         }
         const int start = b->cur;
         double best = 1e300;
-        std::vector<int> bestc = {b->tx, b->ty, b->zc, b->persistent};
+        int bestc[4] = {b->tx, b->ty, b->zc, b->persistent};
         const int shapes[][2] = {
 #define MPD_TILE(X, Y) {X, Y},
             MPD_FUSED_TILES(MPD_TILE)
 #undef MPD_TILE
         };
         for (auto& sh : shapes)
-            for (int z : {8, 16, 32})
+            for (int z : {8, 16, 32, 64})
                 for (int pers : {0, 1}) {
                     if ((rc = mpd_batch_set_tile(b, sh[0], sh[1], z, pers))) return rc;
                     double ms = 0;
@@ -443,7 +432,10 @@ extern "C" int mpd_batch_autotune(mpd_batch b, int nsteps, int* tx, int* ty, int
                     if ((rc = mpd_batch_step_timed(b, nsteps, MPD_BATCH_FUSED, &ms))) return rc;
                     if (ms < best) {
                         best = ms;
-                        bestc = {sh[0], sh[1], z, pers};
+                        bestc[0] = sh[0];
+                        bestc[1] = sh[1];
+                        bestc[2] = z;
+                        bestc[3] = pers;
                     }
                 }
         set_parity(b, start);
@@ -452,14 +444,9 @@ extern "C" int mpd_batch_autotune(mpd_batch b, int nsteps, int* tx, int* ty, int
             MPD_HIP(hipMemcpyAsync(g->c[g->cur].p, save[i].p, g->ncell * 8, hipMemcpyDeviceToDevice, s));
         }
         MPD_HIP(hipStreamSynchronize(s));
-        cache[key] = bestc;
-        if (path)
-            if (FILE* f = fopen(path, "a")) {
-                fprintf(f, "%s %d %d %d %d\n", key.c_str(), bestc[0], bestc[1], bestc[2], bestc[3]);
-                fclose(f);
-            }
+        tune_store(key, 4, bestc);
+        for (int i = 0; i < 4; i++) c[i] = bestc[i];
     }
-    const std::vector<int>& c = cache[key];
     if ((rc = mpd_batch_set_tile(b, c[0], c[1], c[2], c[3]))) return rc;
     if (tx) *tx = c[0];
     if (ty) *ty = c[1];

@@ -1,7 +1,11 @@
 #include "mpd_common.hpp"
 
 #include <cstdarg>
+#include <cstdlib>
+#include <cstring>
+#include <map>
 #include <mutex>
+#include <sstream>
 #include <vector>
 
 namespace mpd {
@@ -67,6 +71,50 @@ int DevBuf::alloc(size_t count)
     }
     n = count;
     return MPD_OK;
+}
+
+static std::mutex g_tune_mu;
+static std::map<std::string, std::vector<int>> g_tune;
+
+bool tune_lookup(const std::string& key, int n, int* v)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it == g_tune.end()) {
+        const char* path = getenv("MPD_TUNE_CACHE");
+        FILE* f = path ? fopen(path, "r") : nullptr;
+        if (!f) return false;
+        char line[1 << 16];
+        while (fgets(line, sizeof line, f)) {
+            std::istringstream in(line);
+            std::string k;
+            in >> k;
+            if (k != key) continue;
+            std::vector<int> vals(n);
+            bool ok = true;
+            for (int i = 0; i < n && ok; i++) ok = bool(in >> vals[i]);
+            if (ok) g_tune[key] = vals;
+        }
+        fclose(f);
+        it = g_tune.find(key);
+        if (it == g_tune.end()) return false;
+    }
+    if ((int)it->second.size() != n) return false;
+    for (int i = 0; i < n; i++) v[i] = it->second[i];
+    return true;
+}
+
+void tune_store(const std::string& key, int n, const int* v)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = std::vector<int>(v, v + n);
+    const char* path = getenv("MPD_TUNE_CACHE");
+    if (FILE* f = path ? fopen(path, "a") : nullptr) {
+        fprintf(f, "%s", key.c_str());
+        for (int i = 0; i < n; i++) fprintf(f, " %d", v[i]);
+        fprintf(f, "\n");
+        fclose(f);
+    }
 }
 
 void DevBuf::release()

@@ -42,4 +42,10 @@ struct DevBuf {
     void release();
 };
 
+// Autotuner results: n integers per key, cached per process and, when the environment
+// names a file in MPD_TUNE_CACHE, read from / appended to that file ("key v0 v1 ..."
+// lines).
+bool tune_lookup(const std::string& key, int n, int* v);
+void tune_store(const std::string& key, int n, const int* v);
+
 }  // namespace mpd

@@ -30,15 +30,15 @@ __device__ __forceinline__ double ab(double a) { return __builtin_fabs(a); }
 
 #ifdef MPD_FAST_DIV
 // n / d for d >= DBL_MIN (every denominator here is a sum of non-negative cell values
-// plus MINV, or a positive flux sum plus MINV): hardware reciprocal estimate, two
-// Newton-Raphson steps r <- r + r(1 - d r), then one residual correction of the
-// quotient q <- q + r(n - d q).  Error: about one ulp of the IEEE quotient.
+// plus MINV, or a positive flux sum plus MINV): hardware reciprocal estimate r0 of
+// relative error e0, one Newton-Raphson step r <- r + r(1 - d r) (error e0^2), then one
+// residual correction of the quotient q <- q + r(n - d q) (error e0^4, below one ulp
+// for any e0 < 2^-13.5; v_rcp_f64 is far more accurate than that).  Error: about one
+// ulp of the IEEE quotient.
 __device__ __forceinline__ double dv(double n, double d)
 {
     double r = __builtin_amdgcn_rcp(d);
     double e = __builtin_fma(-d, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-d, r, 1.0);
     r = __builtin_fma(r, e, r);
     double q = n * r;
     double res = __builtin_fma(-d, q, n);

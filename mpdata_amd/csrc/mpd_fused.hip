@@ -9,15 +9,15 @@
 //
 // Work decomposition.  A workgroup owns a TX x TY column tile of the (x, y) plane and a
 // chunk of ZC planes; it marches up the chunk one plane per iteration (2.5-D streaming).
-// Every thread owns one column of the tile plus a halo of 2 cells, loads that column's
-// values plane by plane (coalesced along x), keeps the values it needs from the planes
-// below and above in registers, and publishes the current plane in LDS so neighbours
-// read in-plane values at fixed LDS offsets.  Threads of the tile interior are numbered
-// first, so the halo threads form their own waves and sit out the expensive stages.
-// Stage regions shrink inward: loads on the tile + 2 ring, the donor field / limiter
-// ratios on the tile + 1 ring, the outputs on the tile.  Only the halo columns are
-// computed twice (by two workgroups); in z nothing is recomputed except the two or
-// three planes a chunk needs below its first output plane.
+// Each thread owns one column of the tile and at most one column of the 2-cell halo
+// around it.  It loads both columns plane by plane (coalesced along x), keeps the values
+// it needs from the planes below and above in registers, and publishes the current plane
+// in LDS so neighbours read in-plane values at fixed LDS offsets.  Every wave computes
+// the tile's expensive stages (no wave holds only halo cells); the threads whose halo
+// cell lies on ring 1 -- the lowest-numbered ones -- also compute the donor field or
+// the limiter ratios there.  Only the ring-1 cells are computed twice (by two
+// workgroups); in z nothing is recomputed except the two or three planes a chunk needs
+// below its first output plane.
 //
 // Arithmetic comes from mpd_formulas.hpp in the oracle's order.  This file is compiled
 // twice: MPD_ARITH=exact (-ffp-contract=off, IEEE division; bit-identical to the staged
@@ -40,43 +40,35 @@ namespace {
 template <int TX, int TY>
 struct Geo {
     static constexpr int RX = TX + 4, RY = TY + 4, R = RX * RY;
-    static constexpr int NT = ((R + 63) / 64) * 64;
-    static constexpr int NTILE = TX * TY;
-    // region cell of thread t: interior tile first, then the two halo rings
-    __device__ static bool cell(int t, int& rx, int& ry)
+    static constexpr int NT = TX * TY;                  // threads: one per tile cell
+    static constexpr int H1 = 2 * (TX + 2) + 2 * TY;    // ring-1 cells
+    static constexpr int NH = R - NT;                   // halo cells, rings 1 and 2
+    static_assert(NH <= NT, "every thread owns at most one halo cell");
+    // Region cell (rx, ry) of halo cell h: ring 1 first (the rows above and below the
+    // tile, then the two side columns), then ring 2 in the same order, so that the
+    // threads with ring-1 work are the lowest-numbered and fill as few waves as possible.
+    __device__ static void halo(int h, int& rx, int& ry)
     {
-        if (t < NTILE) {
-            rx = 2 + t % TX;
-            ry = 2 + t / TX;
-            return true;
+        if (h < H1) {
+            if (h < 2 * (TX + 2)) {
+                ry = h < TX + 2 ? 1 : TY + 2;
+                rx = 1 + h % (TX + 2);
+            } else {
+                const int r = h - 2 * (TX + 2);
+                ry = 2 + r / 2;
+                rx = (r & 1) ? TX + 2 : 1;
+            }
+        } else {
+            int r = h - H1;
+            if (r < 2 * (TX + 4)) {
+                ry = r < TX + 4 ? 0 : TY + 3;
+                rx = r % (TX + 4);
+            } else {
+                r -= 2 * (TX + 4);
+                ry = 1 + r / 2;
+                rx = (r & 1) ? TX + 3 : 0;
+            }
         }
-        int r = t - NTILE;
-        if (r < 2 * RX) {
-            ry = r / RX;
-            rx = r % RX;
-            return true;
-        }
-        r -= 2 * RX;
-        if (r < 2 * RX) {
-            ry = RY - 2 + r / RX;
-            rx = r % RX;
-            return true;
-        }
-        r -= 2 * RX;
-        if (r < 4 * TY) {
-            ry = 2 + r / 4;
-            int s = r % 4;
-            rx = (s < 2) ? s : RX - 4 + s;
-            return true;
-        }
-        rx = ry = 0;
-        return false;
-    }
-    __device__ static int ring(int rx, int ry)
-    {
-        int dx = rx < 2 ? 2 - rx : (rx > TX + 1 ? rx - TX - 1 : 0);
-        int dy = ry < 2 ? 2 - ry : (ry > TY + 1 ? ry - TY - 1 : 0);
-        return dx > dy ? dx : dy;
     }
 };
 
@@ -95,6 +87,41 @@ __device__ __forceinline__ int wrapi(int a, int n)
 {
     a %= n;
     return a < 0 ? a + n : a;
+}
+
+// Cursor over the planes of the periodic z axis: the wrapped plane index and the
+// element offset of that plane, advanced one plane at a time without a division.  Both
+// are wave-uniform (they depend on the loop counter and the instance only), so they
+// live in scalar registers and every global access is a uniform plane base plus the
+// thread's 32-bit column offset.
+struct ZCursor {
+    int z, nz;
+    size_t off, np;
+    __device__ ZCursor(int zz, const FusedArgs& a) : z(wrapi(zz, a.nz)), nz(a.nz), np((size_t)a.np)
+    {
+        off = (size_t)z * np;
+    }
+    __device__ void next()
+    {
+        if (++z == nz) {
+            z = 0;
+            off = 0;
+        } else {
+            off += np;
+        }
+    }
+};
+
+// Element of a plane at byte offset colb of the thread's column (a 32-bit offset: the
+// host keeps every plane below 4 GiB), so the access is a scalar base plus a vector
+// offset.
+__device__ __forceinline__ double ld(const double* plane, unsigned colb)
+{
+    return *(const double*)((const char*)plane + colb);
+}
+__device__ __forceinline__ void st(double* plane, unsigned colb, double v)
+{
+    *(double*)((char*)plane + colb) = v;
 }
 
 // tile (bx, by) and z-chunk bz of work item b of an instance
@@ -117,7 +144,15 @@ struct Work {
     int ninst, nitems, remap;
 };
 
-__device__ __forceinline__ int find_inst(const int* item0, int n, int item)
+// The descriptor table and the item offsets are read through the constant address
+// space: the reads are scalar loads and their results wave-uniform.  (Through a generic
+// pointer the compiler must assume each lane may read a different value, and every
+// quantity derived from the descriptor -- the plane loop included -- would be computed
+// per lane.)
+typedef const __attribute__((address_space(4))) int ConstInt;
+typedef const __attribute__((address_space(4))) FusedArgs ConstArgs;
+
+__device__ __forceinline__ int find_inst(ConstInt* item0, int n, int item)
 {
     int lo = 0, hi = n - 1;
     while (lo < hi) {
@@ -135,74 +170,138 @@ __device__ __forceinline__ int item_of(const Work& w, int it)
     return w.remap ? (it % 8) * (w.nitems / 8) + it / 8 : it;
 }
 
+// Byte offset of the column of region cell (cx, cy) of tile (bx, by) (periodic wrap).
+template <int TX, int TY>
+__device__ __forceinline__ unsigned column(const FusedArgs& a, int bx, int by, int cx, int cy)
+{
+    const int gx = wrapi(bx * TX + cx - 2, a.nx), gy = wrapi(by * TY + cy - 2, a.ny);
+    return (unsigned)(gy * a.nx + gx) * 8u;
+}
+
+// F1 over one work item.  Iteration p: publish c^n(p), U(p), V(p) -> barrier -> donor
+// field c1(p) on the tile and ring 1, publish it with W(p-1) -> barrier -> antidiffusive
+// velocities of the faces above/right of plane k = p - 1 on the tile.
+// LDS slots: c^n and W one plane each (each is read only between the two barriers that
+// follow its write), c1, U and V three planes each (plane p is written while slower waves
+// may still read planes p-2 and p-1).  The own-column values of the planes below and
+// above, and the W neighbours of plane k-1, are carried in registers.
 template <int TX, int TY, bool DONOR>
 __device__ __forceinline__ void f1_tile(const FusedArgs& a, int item)
 {
     using G = Geo<TX, TY>;
     constexpr int RX = G::RX, R = G::R;
-    __shared__ double s_c0[2][R], s_c1[3][R], s_u[3][R], s_v[3][R], s_w[4][R];
+    __shared__ double s_c0[R], s_c1[3][R], s_u[3][R], s_v[3][R], s_w[R];
 
-    int rx, ry;
-    const bool live = G::cell(threadIdx.x, rx, ry);
-    const int ring = live ? G::ring(rx, ry) : 3;
+    const int t = threadIdx.x;
     int bx, by, bz;
     tile_of(a, item, bx, by, bz);
-    const int gx = wrapi(bx * TX + rx - 2, a.nx), gy = wrapi(by * TY + ry - 2, a.ny);
-    const size_t col = (size_t)gy * a.nx + gx;
+    // own tile cell
+    const int rx = 2 + t % TX, ry = 2 + t / TX;
     const int L = ry * RX + rx;
-    const bool store = ring == 0 && bx * TX + rx - 2 < a.nx && by * TY + ry - 2 < a.ny;
+    const unsigned colb = column<TX, TY>(a, bx, by, rx, ry);
+    const bool store = bx * TX + rx - 2 < a.nx && by * TY + ry - 2 < a.ny;
+    // own halo cell: ring 1 (loads, donor) or ring 2 (loads), if any
+    const bool hl = t < G::NH, h1 = t < G::H1;
+    int hx = 0, hy = 0;
+    if (hl) G::halo(t, hx, hy);
+    const int HL = hy * RX + hx;
+    const unsigned hcolb = column<TX, TY>(a, bx, by, hx, hy);
+
     const int z0 = bz * a.zc;
     const int z1 = min(z0 + a.zc, a.nz);
-    const size_t np = (size_t)a.np;
-    auto at = [&](int z) { return (size_t)wrapi(z, a.nz) * np + col; };
+    // planes p + 2, p + 1, p, p - 1 of iteration p
+    ZCursor zp2(z0 + 1, a), zp1(z0, a), zp0(z0 - 1, a), zm1(z0 - 2, a);
 
-    // registers: own column of c^n (planes p-1, p) and W (plane p-1); the values an
-    // iteration consumes are loaded one iteration ahead (n*) so that their latency
-    // overlaps the previous plane's arithmetic
-    double c0m = 0.0, c0c = 0.0, wm1 = 0.0;
-    double nc0 = 0.0, nu = 0.0, nv = 0.0, nw = 0.0, nf = 0.0;
-    auto load_next = [&](int p) {
-        if (DONOR)
-            nc0 = a.cin[at(p + 1)];
-        else
-            nf = a.cin[at(p)];
-        nu = a.U[at(p)];
-        nv = a.V[at(p)];
-        nw = a.W[at(p)];
-    };
-    if (live) {
-        if (DONOR) {
-            c0m = a.cin[at(z0 - 2)];
-            c0c = a.cin[at(z0 - 1)];
-        }
-        wm1 = a.W[at(z0 - 2)];
-        load_next(z0 - 1);
+    // tile cell registers: c^n (p-1, p), c1 (p-2, p-1), U, V (p-1), W (p-2, p-1), the W
+    // neighbours (L+1, L+RX) of plane p-2; n*: the next iteration's values, loaded one
+    // iteration ahead so that their latency overlaps this plane's arithmetic
+    double c0m = 0, c0c = 0, c1m = 0, c1c = 0, uc = 0, vc = 0, wmm = 0, wm1 = 0, wmr = 0, wmu = 0;
+    double nc0 = 0, nu = 0, nv = 0, nw = 0, nf = 0;
+    // halo cell registers
+    double hc0m = 0, hc0c = 0, hwm1 = 0;
+    double hnc0 = 0, hnu = 0, hnv = 0, hnw = 0, hnf = 0;
+    if (DONOR) {
+        c0m = ld(a.cin + zm1.off, colb);
+        c0c = ld(a.cin + zp0.off, colb);
+        nc0 = ld(a.cin + zp1.off, colb);
+    } else {
+        nf = ld(a.cin + zp0.off, colb);
     }
-    for (int p = z0 - 1; p <= z1; p++) {
-        const int s2 = (p + 3) % 2, s3 = (p + 3) % 3, s4 = (p + 4) % 4;
+    wm1 = ld(a.W + zm1.off, colb);
+    nu = ld(a.U + zp0.off, colb);
+    nv = ld(a.V + zp0.off, colb);
+    nw = ld(a.W + zp0.off, colb);
+    if (hl) {
+        if (DONOR) {
+            hc0m = ld(a.cin + zm1.off, hcolb);
+            hc0c = ld(a.cin + zp0.off, hcolb);
+            hnc0 = ld(a.cin + zp1.off, hcolb);
+        } else if (h1) {
+            hnf = ld(a.cin + zp0.off, hcolb);
+        }
+        hnu = ld(a.U + zp0.off, hcolb);
+        hnv = ld(a.V + zp0.off, hcolb);
+        if (h1) {
+            hwm1 = ld(a.W + zm1.off, hcolb);
+            hnw = ld(a.W + zp0.off, hcolb);
+        }
+    }
+    for (int pl = z0 - 1; pl <= z1; pl++) {
+        // the plane index is the same in every lane; without the hint the compiler keeps
+        // it, and the LDS slots derived from it, in vector registers
+        const int p = __builtin_amdgcn_readfirstlane(pl);
+        const int s3 = (p + 3) % 3;
         const double c0n = nc0, un = nu, vn = nv, wn = nw, fld = nf;
-        if (live) {
-            if (p < z1) load_next(p + 1);
-            if (DONOR) s_c0[s2][L] = c0c;
-            s_u[s3][L] = un;
-            s_v[s3][L] = vn;
-            s_w[s4][L] = wn;
+        const double hc0n = hnc0, hun = hnu, hvn = hnv, hwn = hnw, hfld = hnf;
+        if (p < z1) {  // the next iteration's values: c^n at p + 2, faces at p + 1
+            if (DONOR)
+                nc0 = ld(a.cin + zp2.off, colb);
+            else
+                nf = ld(a.cin + zp1.off, colb);
+            nu = ld(a.U + zp1.off, colb);
+            nv = ld(a.V + zp1.off, colb);
+            nw = ld(a.W + zp1.off, colb);
+            if (hl) {
+                if (DONOR)
+                    hnc0 = ld(a.cin + zp2.off, hcolb);
+                else if (h1)
+                    hnf = ld(a.cin + zp1.off, hcolb);
+                hnu = ld(a.U + zp1.off, hcolb);
+                hnv = ld(a.V + zp1.off, hcolb);
+                if (h1) hnw = ld(a.W + zp1.off, hcolb);
+            }
+        }
+        if (DONOR) s_c0[L] = c0c;
+        s_u[s3][L] = un;
+        s_v[s3][L] = vn;
+        if (hl) {
+            if (DONOR) s_c0[HL] = hc0c;
+            s_u[s3][HL] = hun;
+            s_v[s3][HL] = hvn;
         }
         lds_barrier();
+        double c1p;
         if (DONOR) {
-            if (ring <= 1) {
-                const double* c0p = s_c0[s2];
-                double c1 = donor(c0c, c0p[L - 1], c0p[L + 1], c0p[L - RX], c0p[L + RX], c0m,
-                                  c0n, s_u[s3][L - 1], un, s_v[s3][L - RX], vn, wm1, wn);
-                s_c1[s3][L] = c1;
-                if (store && p >= z0 && p < z1) a.c1out[at(p)] = c1;
-            }
-        } else if (live) {
-            s_c1[s3][L] = fld;
+            c1p = donor(c0c, s_c0[L - 1], s_c0[L + 1], s_c0[L - RX], s_c0[L + RX], c0m, c0n,
+                        s_u[s3][L - 1], un, s_v[s3][L - RX], vn, wm1, wn);
+            if (store && p >= z0 && p < z1) st(a.c1out + zp0.off, colb, c1p);
+        } else {
+            c1p = fld;
+        }
+        s_c1[s3][L] = c1p;
+        s_w[L] = wm1;
+        if (h1) {
+            const double hc1 =
+                DONOR ? donor(hc0c, s_c0[HL - 1], s_c0[HL + 1], s_c0[HL - RX], s_c0[HL + RX],
+                              hc0m, hc0n, s_u[s3][HL - 1], hun, s_v[s3][HL - RX], hvn, hwm1, hwn)
+                      : hfld;
+            s_c1[s3][HL] = hc1;
+            s_w[HL] = hwm1;
         }
         lds_barrier();
         const int k = p - 1;  // plane whose faces are computed now
-        if (ring == 0 && k >= z0 && k < z1) {
+        const double wkr = s_w[L + 1], wku = s_w[L + RX];
+        if (k >= z0 && k < z1) {
             const int m3 = (p + 1) % 3, c3 = (p + 2) % 3;  // slots of planes k-1, k
             const double* cm = s_c1[m3];
             const double* cc = s_c1[c3];
@@ -211,161 +310,223 @@ __device__ __forceinline__ void f1_tile(const FusedArgs& a, int item)
             const double* uq = s_u[s3];
             const double* vk = s_v[c3];
             const double* vq = s_v[s3];
-            const double* wm = s_w[(p + 2) % 4];
-            const double* wk = s_w[(p + 3) % 4];
-            const double* wq = s_w[s4];
-            const size_t o = at(k);
+            const double c = c1c;
             {  // x-face (uu.c:81-87)
-                double av = uk[L];
-                double vr = vk[L + 1 - RX], vru = vk[L + 1], vu = vk[L], vc = vk[L - RX];
-                double wr = wm[L + 1], wrf = wk[L + 1], wf = wk[L], wc = wm[L];
-                double sv = 0.25 * (vr + vru + vu + vc);
+                double av = uc;
+                double vr = vk[L + 1 - RX], vru = vk[L + 1], vu = vc, vcc = vk[L - RX];
+                double wr = wmr, wrf = wkr, wf = wm1, wc = wmm;
+                double sv = 0.25 * (vr + vru + vu + vcc);
                 double sw = 0.25 * (wr + wrf + wf + wc);
-                double div = 0.25 * av * (uk[L + 1] - uk[L - 1] + vr - vru - vu + vc + wr - wrf -
+                double div = 0.25 * av * (uk[L + 1] - uk[L - 1] + vr - vru - vu + vcc + wr - wrf -
                                           wf + wc);
-                double c = cc[L], cr = cc[L + 1], cru = cc[L + 1 + RX], cu = cc[L + RX];
+                double cr = cc[L + 1], cru = cc[L + 1 + RX], cu = cc[L + RX];
                 double cd = cc[L - RX], crd = cc[L + 1 - RX];
-                double crf = cq[L + 1], cf = cq[L], cb = cm[L], crb = cm[L + 1];
-                a.Ax[o] = antidiff(div, av, cr, c, sv, cru + cu - cd - crd, cru + cu + cd + crd,
-                                   sw, crf + cf - cb - crb, crf + cf + cb + crb);
+                double crf = cq[L + 1], cf = c1p, cb = c1m, crb = cm[L + 1];
+                st(a.Ax + zm1.off, colb,
+                   antidiff(div, av, cr, c, sv, cru + cu - cd - crd, cru + cu + cd + crd, sw,
+                            crf + cf - cb - crb, crf + cf + cb + crb));
             }
             {  // y-face (vv.c:85-92)
-                double av = vk[L];
-                double uu = uk[L - 1 + RX], uru = uk[L + RX], ur = uk[L], uc = uk[L - 1];
+                double av = vc;
+                double uu = uk[L - 1 + RX], uru = uk[L + RX], ur = uc, ucc = uk[L - 1];
                 double ulu = uk[L - 2 + RX], ul = uk[L - 2];
-                double wu = wm[L + RX], wuf = wk[L + RX], wf = wk[L], wc = wm[L];
-                double su = 0.25 * (uu + uru + ur + uc);
+                double wu = wmu, wuf = wku, wf = wm1, wc = wmm;
+                double su = 0.25 * (uu + uru + ur + ucc);
                 double sw = 0.25 * (wu + wuf + wf + wc);
                 double div = 0.25 * av * (vk[L + RX] - vk[L - RX] + wu - wuf - wf + wc + uu - ulu -
-                                          ul + uc);
-                double c = cc[L], cu = cc[L + RX], cru = cc[L + 1 + RX], cr = cc[L + 1];
+                                          ul + ucc);
+                double cu = cc[L + RX], cru = cc[L + 1 + RX], cr = cc[L + 1];
                 double cl = cc[L - 1], clu = cc[L - 1 + RX];
-                double cuf = cq[L + RX], cf = cq[L], cb = cm[L], cub = cm[L + RX];
-                a.Ay[o] = antidiff(div, av, cu, c, su, cru + cr - cl - clu, cru + cr + cl + clu,
-                                   sw, cuf + cf - cb - cub, cuf + cf + cb + cub);
+                double cuf = cq[L + RX], cf = c1p, cb = c1m, cub = cm[L + RX];
+                st(a.Ay + zm1.off, colb,
+                   antidiff(div, av, cu, c, su, cru + cr - cl - clu, cru + cr + cl + clu, sw,
+                            cuf + cf - cb - cub, cuf + cf + cb + cub));
             }
             {  // z-face (ww.c:248-255)
-                double av = wk[L];
-                double ur = uk[L], urf = uq[L], uf = uq[L - 1], uc = uk[L - 1];
-                double vf = vq[L - RX], vuf = vq[L], vu = vk[L], vc = vk[L - RX];
-                double su = 0.25 * (ur + urf + uf + uc);
-                double sv = 0.25 * (vf + vuf + vu + vc);
-                double div = 0.25 * av * (wq[L] - wm[L] + uf - urf - ur + uc + vf - vuf - vu + vc);
-                double c = cc[L], cf = cq[L], crf = cq[L + 1], cr = cc[L + 1], cl = cc[L - 1];
+                double av = wm1;
+                double ur = uc, urf = un, uf = uq[L - 1], ucc = uk[L - 1];
+                double vf = vq[L - RX], vuf = vn, vu = vc, vcc = vk[L - RX];
+                double su = 0.25 * (ur + urf + uf + ucc);
+                double sv = 0.25 * (vf + vuf + vu + vcc);
+                double div = 0.25 * av * (wn - wmm + uf - urf - ur + ucc + vf - vuf - vu + vcc);
+                double cf = c1p, crf = cq[L + 1], cr = cc[L + 1], cl = cc[L - 1];
                 double clf = cq[L - 1], cuf = cq[L + RX], cu = cc[L + RX], cd = cc[L - RX];
                 double cdf = cq[L - RX];
-                a.Az[o] = antidiff(div, av, cf, c, su, crf + cr - cl - clf, crf + cr + cl + clf,
-                                   sv, cuf + cu - cd - cdf, cuf + cu + cd + cdf);
+                st(a.Az + zm1.off, colb,
+                   antidiff(div, av, cf, c, su, crf + cr - cl - clf, crf + cr + cl + clf, sv,
+                            cuf + cu - cd - cdf, cuf + cu + cd + cdf));
             }
         }
         if (DONOR) {
             c0m = c0c;
             c0c = c0n;
+            hc0m = hc0c;
+            hc0c = hc0n;
         }
+        c1m = c1c;
+        c1c = c1p;
+        uc = un;
+        vc = vn;
+        wmm = wm1;
         wm1 = wn;
+        wmr = wkr;
+        wmu = wku;
+        hwm1 = hwn;
+        zp2.next();
+        zp1.next();
+        zp0.next();
+        zm1.next();
     }
 }
 
+// F2 over one work item.  Iteration p: publish c^n(p), the pass input cp(p) and the
+// x/y antidiffusive faces of plane p -> barrier -> limiter ratios of plane p on the tile
+// and ring 1 -> barrier -> limited donor pass of plane k = p - 1 on the tile.
+// LDS slots: c^n, Ax and Ay one plane each (read only between the two barriers that
+// follow the write; the two face values the donor pass needs a plane later are carried
+// in registers), cp three planes, the ratios two.
 template <int TX, int TY>
 __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
 {
     using G = Geo<TX, TY>;
     constexpr int RX = G::RX, R = G::R;
-    __shared__ double s_cn[2][R], s_cp[3][R], s_ax[3][R], s_ay[3][R], s_bu[2][R], s_bd[2][R];
+    __shared__ double s_cn[R], s_cp[3][R], s_ax[R], s_ay[R], s_bu[2][R], s_bd[2][R];
 
-    int rx, ry;
-    const bool live = G::cell(threadIdx.x, rx, ry);
-    const int ring = live ? G::ring(rx, ry) : 3;
+    const int t = threadIdx.x;
     int bx, by, bz;
     tile_of(a, item, bx, by, bz);
-    const int gx = wrapi(bx * TX + rx - 2, a.nx), gy = wrapi(by * TY + ry - 2, a.ny);
-    const size_t col = (size_t)gy * a.nx + gx;
+    const int rx = 2 + t % TX, ry = 2 + t / TX;
     const int L = ry * RX + rx;
-    const bool store = ring == 0 && bx * TX + rx - 2 < a.nx && by * TY + ry - 2 < a.ny;
+    const unsigned colb = column<TX, TY>(a, bx, by, rx, ry);
+    const bool store = bx * TX + rx - 2 < a.nx && by * TY + ry - 2 < a.ny;
+    const bool hl = t < G::NH, h1 = t < G::H1;
+    int hx = 0, hy = 0;
+    if (hl) G::halo(t, hx, hy);
+    const int HL = hy * RX + hx;
+    const unsigned hcolb = column<TX, TY>(a, bx, by, hx, hy);
+
     const int z0 = bz * a.zc;
     const int z1 = min(z0 + a.zc, a.nz);
-    const size_t np = (size_t)a.np;
-    auto at = [&](int z) { return (size_t)wrapi(z, a.nz) * np + col; };
+    ZCursor zp2(z0 + 1, a), zp1(z0, a), zp0(z0 - 1, a), zm1(z0 - 2, a);
 
-    // own-column registers: cn p-1,p ; cp p-2,p-1,p ; az p-2,p-1 ; beta p-2,p-1; the
-    // values an iteration consumes are loaded one iteration ahead (n*)
+    // tile cell: c^n (p-1, p), cp (p-2 .. p), Az (p-2, p-1), Ax, Ay (p-1) with their
+    // lower neighbours (L-1, L-RX), ratios (p-2, p-1); n*: loaded one iteration ahead
     double cnm = 0, cnc = 0, cpm2 = 0, cpm = 0, cpc = 0, azm2 = 0, azm = 0;
+    double axm = 0, aym = 0, axlm = 0, aydm = 0;
     double bum2 = 0, bdm2 = 0, bum = 0, bdm = 0;
     double ncn = 0, ncp = 0, nax = 0, nay = 0, naz = 0;
-    auto load_next = [&](int p) {
-        ncn = a.cn[at(p + 1)];
-        ncp = a.cp[at(p + 1)];
-        nax = a.Ax[at(p)];
-        nay = a.Ay[at(p)];
-        naz = a.Az[at(p)];
-    };
-    if (live) {
-        cnm = a.cn[at(z0 - 2)];
-        cnc = a.cn[at(z0 - 1)];
-        cpm = a.cp[at(z0 - 2)];
-        cpc = a.cp[at(z0 - 1)];
-        azm = a.Az[at(z0 - 2)];
-        load_next(z0 - 1);
+    // halo cell
+    double hcnm = 0, hcnc = 0, hcpm = 0, hcpc = 0, hazm = 0;
+    double hncn = 0, hncp = 0, hnax = 0, hnay = 0, hnaz = 0;
+    cnm = ld(a.cn + zm1.off, colb);
+    cnc = ld(a.cn + zp0.off, colb);
+    cpm = ld(a.cp + zm1.off, colb);
+    cpc = ld(a.cp + zp0.off, colb);
+    azm = ld(a.Az + zm1.off, colb);
+    ncn = ld(a.cn + zp1.off, colb);
+    ncp = ld(a.cp + zp1.off, colb);
+    nax = ld(a.Ax + zp0.off, colb);
+    nay = ld(a.Ay + zp0.off, colb);
+    naz = ld(a.Az + zp0.off, colb);
+    if (hl) {
+        hcnc = ld(a.cn + zp0.off, hcolb);
+        hcpc = ld(a.cp + zp0.off, hcolb);
+        hncn = ld(a.cn + zp1.off, hcolb);
+        hncp = ld(a.cp + zp1.off, hcolb);
+        hnax = ld(a.Ax + zp0.off, hcolb);
+        hnay = ld(a.Ay + zp0.off, hcolb);
+        if (h1) {
+            hcnm = ld(a.cn + zm1.off, hcolb);
+            hcpm = ld(a.cp + zm1.off, hcolb);
+            hazm = ld(a.Az + zm1.off, hcolb);
+            hnaz = ld(a.Az + zp0.off, hcolb);
+        }
     }
-    for (int p = z0 - 1; p <= z1; p++) {
+    for (int pl = z0 - 1; pl <= z1; pl++) {
+        const int p = __builtin_amdgcn_readfirstlane(pl);  // uniform, as in f1_tile
         const int s2 = (p + 2) % 2, s3 = (p + 3) % 3;
         const double cnn = ncn, cpn = ncp, axn = nax, ayn = nay, azc = naz;
-        if (live) {
-            if (p < z1) load_next(p + 1);
-            s_cn[s2][L] = cnc;
-            s_cp[s3][L] = cpc;
-            s_ax[s3][L] = axn;
-            s_ay[s3][L] = ayn;
+        const double hcnn = hncn, hcpn = hncp, haxn = hnax, hayn = hnay, hazc = hnaz;
+        if (p < z1) {  // the next iteration's values: fields at p + 2, faces at p + 1
+            ncn = ld(a.cn + zp2.off, colb);
+            ncp = ld(a.cp + zp2.off, colb);
+            nax = ld(a.Ax + zp1.off, colb);
+            nay = ld(a.Ay + zp1.off, colb);
+            naz = ld(a.Az + zp1.off, colb);
+            if (hl) {
+                hncn = ld(a.cn + zp2.off, hcolb);
+                hncp = ld(a.cp + zp2.off, hcolb);
+                hnax = ld(a.Ax + zp1.off, hcolb);
+                hnay = ld(a.Ay + zp1.off, hcolb);
+                if (h1) hnaz = ld(a.Az + zp1.off, hcolb);
+            }
+        }
+        s_cn[L] = cnc;
+        s_cp[s3][L] = cpc;
+        s_ax[L] = axn;
+        s_ay[L] = ayn;
+        if (hl) {
+            s_cn[HL] = hcnc;
+            s_cp[s3][HL] = hcpc;
+            s_ax[HL] = haxn;
+            s_ay[HL] = hayn;
         }
         lds_barrier();
-        double buc = 0, bdc = 0;
-        if (ring <= 1) {  // limiter ratios of plane p (see cpu_ref.c)
-            const double* cnp = s_cn[s2];
-            const double* cpp = s_cp[s3];
-            double hi = cnc, lo = cnc;
-            double nbn[7] = {cnc, cnp[L - 1], cnp[L + 1], cnp[L - RX], cnp[L + RX], cnm, cnn};
-            double nbp[7] = {cpc, cpp[L - 1], cpp[L + 1], cpp[L - RX], cpp[L + RX], cpm, cpn};
+        // limiter ratios of plane p (see cpu_ref.c)
+        const double* cpp = s_cp[s3];
+        auto ratios = [&](int M, double cn0, double cnlo, double cnhi, double c, double cplo,
+                          double cphi, double axr, double ayr, double azlo, double azhi,
+                          double axl, double ayl, double& bu, double& bd) {
+            double hi = cn0, lo = cn0;
+            const double nbn[7] = {cn0, s_cn[M - 1], s_cn[M + 1], s_cn[M - RX], s_cn[M + RX],
+                                   cnlo, cnhi};
+            const double nbp[7] = {c, cpp[M - 1], cpp[M + 1], cpp[M - RX], cpp[M + RX], cplo,
+                                   cphi};
 #pragma unroll
             for (int m = 0; m < 7; m++) {
                 hi = mx2(hi, mx2(nbn[m], nbp[m]));
                 lo = mn2(lo, mn2(nbn[m], nbp[m]));
             }
-            const double* axp = s_ax[s3];
-            const double* ayp = s_ay[s3];
-            double c = cpc;
-            double fxl = 0.5 * F2(nbp[1], c, axp[L - 1]), fxr = 0.5 * F2(c, nbp[2], axn);
-            double fyl = 0.5 * F2(nbp[3], c, ayp[L - RX]), fyr = 0.5 * F2(c, nbp[4], ayn);
-            double fzl = 0.5 * F2(cpm, c, azm), fzr = 0.5 * F2(c, cpn, azc);
-            beta(hi, lo, c, fxl, fxr, fyl, fyr, fzl, fzr, buc, bdc);
-            s_bu[s2][L] = buc;
-            s_bd[s2][L] = bdc;
+            const double fxl = 0.5 * F2(nbp[1], c, axl), fxr = 0.5 * F2(c, nbp[2], axr);
+            const double fyl = 0.5 * F2(nbp[3], c, ayl), fyr = 0.5 * F2(c, nbp[4], ayr);
+            const double fzl = 0.5 * F2(cplo, c, azlo), fzr = 0.5 * F2(c, cphi, azhi);
+            beta(hi, lo, c, fxl, fxr, fyl, fyr, fzl, fzr, bu, bd);
+        };
+        const double axl = s_ax[L - 1], ayd = s_ay[L - RX];
+        double buc, bdc;
+        ratios(L, cnc, cnm, cnn, cpc, cpm, cpn, axn, ayn, azm, azc, axl, ayd, buc, bdc);
+        s_bu[s2][L] = buc;
+        s_bd[s2][L] = bdc;
+        if (h1) {
+            double hbu, hbd;
+            ratios(HL, hcnc, hcnm, hcnn, hcpc, hcpm, hcpn, haxn, hayn, hazm, hazc,
+                   s_ax[HL - 1], s_ay[HL - RX], hbu, hbd);
+            s_bu[s2][HL] = hbu;
+            s_bd[s2][HL] = hbd;
         }
         lds_barrier();
         const int k = p - 1;
-        if (ring == 0 && k >= z0 && k < z1) {
+        if (k >= z0 && k < z1) {
             const int k2 = (p + 1) % 2, k3 = (p + 2) % 3;  // slots of plane k
             const double* bu = s_bu[k2];
             const double* bd = s_bd[k2];
-            const double* ax = s_ax[k3];
-            const double* ay = s_ay[k3];
             const double* cq = s_cp[k3];
-            double b0u = bu[L], b0d = bd[L];
-            double uxr = limit_face(ax[L], b0u, b0d, bu[L + 1], bd[L + 1]);
-            double uxl = limit_face(ax[L - 1], bu[L - 1], bd[L - 1], b0u, b0d);
-            double vyr = limit_face(ay[L], b0u, b0d, bu[L + RX], bd[L + RX]);
-            double vyl = limit_face(ay[L - RX], bu[L - RX], bd[L - RX], b0u, b0d);
-            double wzr = limit_face(azm, b0u, b0d, buc, bdc);
-            double wzl = limit_face(azm2, bum2, bdm2, b0u, b0d);
-            double r = donor(cq[L], cq[L - 1], cq[L + 1], cq[L - RX], cq[L + RX], cpm2, cpc, uxl,
+            const double b0u = bum, b0d = bdm;
+            const double uxr = limit_face(axm, b0u, b0d, bu[L + 1], bd[L + 1]);
+            const double uxl = limit_face(axlm, bu[L - 1], bd[L - 1], b0u, b0d);
+            const double vyr = limit_face(aym, b0u, b0d, bu[L + RX], bd[L + RX]);
+            const double vyl = limit_face(aydm, bu[L - RX], bd[L - RX], b0u, b0d);
+            const double wzr = limit_face(azm, b0u, b0d, buc, bdc);
+            const double wzl = limit_face(azm2, bum2, bdm2, b0u, b0d);
+            double r = donor(cpm, cq[L - 1], cq[L + 1], cq[L - RX], cq[L + RX], cpm2, cpc, uxl,
                              uxr, vyl, vyr, wzl, wzr);
             if (store) {
-                const size_t o = at(k);
-                if (a.q) r = r + a.q[o];
-                a.cout[o] = r;
+                if (a.q) r = r + ld(a.q + zm1.off, colb);
+                st(a.cout + zm1.off, colb, r);
                 if (a.Lx) {
-                    a.Lx[o] = uxr;
-                    a.Ly[o] = vyr;
-                    a.Lz[o] = wzr;
+                    st(a.Lx + zm1.off, colb, uxr);
+                    st(a.Ly + zm1.off, colb, vyr);
+                    st(a.Lz + zm1.off, colb, wzr);
                 }
             }
         }
@@ -376,10 +537,23 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
         cpc = cpn;
         azm2 = azm;
         azm = azc;
+        axm = axn;
+        aym = ayn;
+        axlm = axl;
+        aydm = ayd;
         bum2 = bum;
         bdm2 = bdm;
         bum = buc;
         bdm = bdc;
+        hcnm = hcnc;
+        hcnc = hcnn;
+        hcpm = hcpc;
+        hcpc = hcpn;
+        hazm = hazc;
+        zp2.next();
+        zp1.next();
+        zp0.next();
+        zm1.next();
     }
 }
 
@@ -390,9 +564,10 @@ __global__ void __launch_bounds__((Geo<TX, TY>::NT)) k_f1(FusedArgs a, Work w)
         int item = item_of(w, it);
         FusedArgs x = a;
         if (w.ninst) {
-            int i = find_inst(w.item0, w.ninst, item);
-            x = w.table[i];
-            item -= w.item0[i];
+            ConstInt* item0 = (ConstInt*)w.item0;
+            int i = find_inst(item0, w.ninst, item);
+            __builtin_memcpy(&x, (const FusedArgs*)((ConstArgs*)w.table + i), sizeof x);
+            item -= item0[i];
         }
         f1_tile<TX, TY, DONOR>(x, item);
         lds_barrier();  // the next item reuses the LDS planes
@@ -406,9 +581,10 @@ __global__ void __launch_bounds__((Geo<TX, TY>::NT)) k_f2(FusedArgs a, Work w)
         int item = item_of(w, it);
         FusedArgs x = a;
         if (w.ninst) {
-            int i = find_inst(w.item0, w.ninst, item);
-            x = w.table[i];
-            item -= w.item0[i];
+            ConstInt* item0 = (ConstInt*)w.item0;
+            int i = find_inst(item0, w.ninst, item);
+            __builtin_memcpy(&x, (const FusedArgs*)((ConstArgs*)w.table + i), sizeof x);
+            item -= item0[i];
         }
         f2_tile<TX, TY>(x, item);
         lds_barrier();

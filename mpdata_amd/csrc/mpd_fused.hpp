@@ -6,7 +6,7 @@
 namespace mpd {
 
 // Tile shapes compiled for the fused kernels (x extent, y extent of the column tile).
-#define MPD_FUSED_TILES(X) X(32, 8) X(64, 4) X(16, 16) X(32, 4) X(64, 8) X(16, 8)
+#define MPD_FUSED_TILES(X) X(32, 8) X(16, 16) X(64, 8) X(16, 8) X(32, 16)
 
 struct FusedArgs {
     int nx, ny, nz;
@@ -35,6 +35,9 @@ inline void fused_geometry(FusedArgs& a, int tx, int ty)
 }
 
 inline int fused_items(const FusedArgs& a) { return a.tiles_x * a.tiles_y * a.nchunks; }
+
+// The fused kernels address a plane with 32-bit byte offsets: planes of up to 2^29 cells.
+inline bool fused_plane_fits(long long nx, long long ny) { return nx * ny <= (1LL << 29); }
 
 // One F1 (f1 = true; donor = first pass) or F2 launch over a device table of instance
 // descriptors; item0[i] = first work item of instance i; grid workgroups stride over

@@ -82,7 +82,7 @@ def make_batch(iord, arith, seed=7):
 
 
 MODES = [(M.BATCH_PER_INSTANCE, None), (M.BATCH_FUSED, (32, 8, 16, 1)),
-         (M.BATCH_FUSED, (16, 16, 8, 0)), (M.BATCH_GRAPH, (64, 4, 8, 1)),
+         (M.BATCH_FUSED, (16, 16, 8, 0)), (M.BATCH_GRAPH, (32, 16, 8, 1)),
          (M.BATCH_GRAPH, (16, 8, 32, 0))]
 
 
@@ -131,7 +131,7 @@ def test_autotune_restores_fields():
     b, _ = make_batch(2, M.ARITH_FAST)
     before = [g.field() for g in b.instances]
     tx, ty, zc, pers = b.autotune(1)
-    assert (tx, ty) in [(32, 8), (64, 4), (16, 16), (32, 4), (64, 8), (16, 8)]
+    assert (tx, ty) in [(32, 8), (16, 16), (64, 8), (16, 8), (32, 16)]
     assert zc in (8, 16, 32) and pers in (0, 1)
     for g, f in zip(b.instances, before):
         np.testing.assert_array_equal(g.field(), f)

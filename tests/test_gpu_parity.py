@@ -156,7 +156,7 @@ def test_periodic_matches_oracle(dims, path, arith):
             g.close()
 
 
-TILES = [(32, 8), (64, 4), (16, 16), (32, 4), (64, 8), (16, 8)]
+TILES = [(32, 8), (16, 16), (64, 8), (16, 8), (32, 16)]
 
 
 @pytest.mark.parametrize("tile", TILES)
@@ -235,4 +235,42 @@ def test_mass_and_positivity_at_128():
     m1, lo, hi = g.stats()
     assert abs(m1 - m0) <= 1e-12 * m0
     assert lo >= 0.0 and hi <= hi0 * (1 + 1e-14)
+    g.close()
+
+
+def test_autotune_keeps_state_and_parity():
+    nx, ny, nz = 70, 19, 9
+    U, V, W, c = per_case(nx, ny, nz, 5)
+    want = O.per_run(nx, ny, nz, U, V, W, 2, True, c, 3)
+    g = M.Grid(nx, ny, nz, M.PERIODIC, iord=2, nonosc=True)
+    g.set_arith(M.ARITH_EXACT)
+    g.set_courant(U, V, W)
+    g.set_field(c)
+    tx, ty, zc = g.autotune(1)
+    print(f"case autotune -> tile {tx}x{ty} zc {zc}", flush=True)
+    assert (tx, ty) in TILES and 1 <= zc <= nz
+    np.testing.assert_array_equal(g.field(), c)
+    g.step(3)
+    np.testing.assert_array_equal(g.field(), want)
+    g.close()
+    with pytest.raises(M.MpdError):
+        f = M.Grid(nx, ny, nz, M.FRAME, iord=2)
+        try:
+            f.autotune(1)
+        finally:
+            f.close()
+
+
+@pytest.mark.parametrize("zc", [1, 2, 5, 9, 40])
+def test_zchunk_lengths_match_oracle(zc):
+    nx, ny, nz = 33, 17, 9
+    U, V, W, c = per_case(nx, ny, nz, 8)
+    want = O.per_run(nx, ny, nz, U, V, W, 2, True, c, 2)
+    g = M.Grid(nx, ny, nz, M.PERIODIC, iord=2, nonosc=True)
+    g.set_arith(M.ARITH_EXACT)
+    g.set_zchunk(zc)
+    g.set_courant(U, V, W)
+    g.set_field(c)
+    g.step(2)
+    np.testing.assert_array_equal(g.field(), want)
     g.close()
