@@ -440,25 +440,34 @@ extern "C" int mpd_step_kernel_times(mpd_grid g, int nsteps, double* ms, int max
     MPD_HIP(hipEventCreate(&start));
     MPD_HIP(hipEventRecord(start, stream()));
     g->trace = &ev;
+    g->trace_pairs = false;
     int rc = mpd_step(g, nsteps);
     g->trace = nullptr;
-    if (rc == MPD_OK && ev.empty()) rc = fail(MPD_ERR_UNSUPPORTED, "no kernel launches traced");
+    // start/stop pairs bracket each kernel alone; single events after each launch give
+    // the time since the previous launch's event (launch gaps included)
+    const bool pairs = g->trace_pairs;
+    const size_t nk = pairs ? ev.size() / 2 : ev.size();
+    if (rc == MPD_OK && nk == 0) rc = fail(MPD_ERR_UNSUPPORTED, "no kernel launches traced");
     if (rc == MPD_OK) {
         hipError_t e = hipEventSynchronize(ev.back());
         if (e != hipSuccess) rc = hip_fail(e, "hipEventSynchronize");
     }
-    int per = rc == MPD_OK ? (int)(ev.size() / nsteps) : 0;
-    if (rc == MPD_OK && per * nsteps != (int)ev.size())
+    int per = rc == MPD_OK ? (int)(nk / nsteps) : 0;
+    if (rc == MPD_OK && per * nsteps != (int)nk)
         rc = fail(MPD_ERR_UNSUPPORTED, "launch count varies between steps");
     if (rc == MPD_OK) {
         int m = per < max ? per : max;
         for (int i = 0; i < m; i++) ms[i] = 0.0;
         hipEvent_t prev = start;
-        for (size_t k = 0; k < ev.size(); k++) {
+        for (size_t k = 0; k < nk; k++) {
             float f = 0.f;
-            (void)hipEventElapsedTime(&f, prev, ev[k]);
+            if (pairs) {
+                (void)hipEventElapsedTime(&f, ev[2 * k], ev[2 * k + 1]);
+            } else {
+                (void)hipEventElapsedTime(&f, prev, ev[k]);
+                prev = ev[k];
+            }
             if ((int)(k % per) < m) ms[k % per] += f / nsteps;
-            prev = ev[k];
         }
         *n = per;
     }

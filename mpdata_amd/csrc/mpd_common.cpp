@@ -1,5 +1,7 @@
 #include "mpd_common.hpp"
 
+#include <dlfcn.h>
+
 #include <cstdarg>
 #include <cstdlib>
 #include <cstring>
@@ -76,13 +78,25 @@ int DevBuf::alloc(size_t count)
 static std::mutex g_tune_mu;
 static std::map<std::string, std::vector<int>> g_tune;
 
+// The table shipped beside the library (tuning.txt in the directory of this shared
+// object): results measured on MI355X for the shapes of the benchmark configurations.
+static std::string shipped_table()
+{
+    Dl_info info;
+    if (!dladdr((void*)&shipped_table, &info) || !info.dli_fname) return "";
+    std::string so = info.dli_fname;
+    const size_t slash = so.rfind('/');
+    return (slash == std::string::npos ? std::string(".") : so.substr(0, slash)) + "/tuning.txt";
+}
+
 bool tune_lookup(const std::string& key, int n, int* v)
 {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune.find(key);
     if (it == g_tune.end()) {
-        const char* path = getenv("MPD_TUNE_CACHE");
-        FILE* f = path ? fopen(path, "r") : nullptr;
+        const char* env = getenv("MPD_TUNE_CACHE");
+        const std::string path = env ? std::string(env) : shipped_table();
+        FILE* f = path.empty() ? nullptr : fopen(path.c_str(), "r");
         if (!f) return false;
         char line[1 << 16];
         while (fgets(line, sizeof line, f)) {

@@ -42,9 +42,10 @@ struct DevBuf {
     void release();
 };
 
-// Autotuner results: n integers per key, cached per process and, when the environment
-// names a file in MPD_TUNE_CACHE, read from / appended to that file ("key v0 v1 ..."
-// lines).
+// Autotuner results: n integers per key ("key v0 v1 ..." lines), cached per process.
+// Lookups read the file named by the environment variable MPD_TUNE_CACHE or, when it is
+// unset, the table shipped beside the library (tuning.txt); new results are appended to
+// the MPD_TUNE_CACHE file only.
 bool tune_lookup(const std::string& key, int n, int* v);
 void tune_store(const std::string& key, int n, const int* v);
 

@@ -23,6 +23,8 @@
 // twice: MPD_ARITH=exact (-ffp-contract=off, IEEE division; bit-identical to the staged
 // path and the oracle) and MPD_ARITH=fast (contracted multiply-adds, reciprocal-based
 // division; <= 1e-12 normwise per step against the oracle).
+#include <hip/hip_ext.h>
+
 #include "mpd_formulas.hpp"
 #include "mpd_fused.hpp"
 #include "mpd_grid.hpp"
@@ -615,39 +617,66 @@ __global__ void k_donor_faces(int nx, int ny, int nz, const double* __restrict__
 
 // ---------------------------------------------------------------- dispatch ---
 
+// Launches a kernel; with ev set, between the start event ev[0] and the stop event
+// ev[1], which then bracket the kernel's execution alone.
+template <typename K, typename... Args>
+void launch_k(K kernel, dim3 grid, dim3 block, hipStream_t s, const hipEvent_t* ev,
+              Args... args)
+{
+    if (ev)
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, s, ev[0], ev[1], 0, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
+}
+
 template <int TX, int TY>
-int launch_pair(const FusedArgs& a, const Work& w, int grid, bool f1, bool donor, hipStream_t s)
+int launch_pair(const FusedArgs& a, const Work& w, int grid, bool f1, bool donor, hipStream_t s,
+                const hipEvent_t* ev)
 {
     using G = Geo<TX, TY>;
     if (f1) {
         if (donor)
-            hipLaunchKernelGGL((k_f1<TX, TY, true>), dim3(grid), dim3(G::NT), 0, s, a, w);
+            launch_k(k_f1<TX, TY, true>, dim3(grid), dim3(G::NT), s, ev, a, w);
         else
-            hipLaunchKernelGGL((k_f1<TX, TY, false>), dim3(grid), dim3(G::NT), 0, s, a, w);
+            launch_k(k_f1<TX, TY, false>, dim3(grid), dim3(G::NT), s, ev, a, w);
     } else {
-        hipLaunchKernelGGL((k_f2<TX, TY>), dim3(grid), dim3(G::NT), 0, s, a, w);
+        launch_k(k_f2<TX, TY>, dim3(grid), dim3(G::NT), s, ev, a, w);
     }
     MPD_HIP(hipGetLastError());
     return MPD_OK;
 }
 
 int launch_any(const FusedArgs& a, const Work& w, int grid, int tx, int ty, bool f1, bool donor,
-               hipStream_t s)
+               hipStream_t s, const hipEvent_t* ev = nullptr)
 {
 #define MPD_TILE(X, Y) \
-    if (tx == X && ty == Y) return launch_pair<X, Y>(a, w, grid, f1, donor, s);
+    if (tx == X && ty == Y) return launch_pair<X, Y>(a, w, grid, f1, donor, s, ev);
     MPD_FUSED_TILES(MPD_TILE)
 #undef MPD_TILE
     return fail(MPD_ERR_ARG, "tile %dx%d is not one of the compiled tile shapes", tx, ty);
 }
 
 // single grid: one work item per workgroup
-int launch(FusedArgs& a, int tx, int ty, bool f1, bool donor, hipStream_t s)
+int launch(FusedArgs& a, int tx, int ty, bool f1, bool donor, hipStream_t s, const hipEvent_t* ev)
 {
     fused_geometry(a, tx, ty);
     const int nb = a.tiles_x * a.tiles_y * a.nchunks;
     Work w{nullptr, nullptr, 0, nb, nb % 8 == 0 ? 1 : 0};
-    return launch_any(a, w, nb, tx, ty, f1, donor, s);
+    return launch_any(a, w, nb, tx, ty, f1, donor, s, ev);
+}
+
+// The start/stop events of the next launch when the grid's step is being timed.
+int span(mpd_grid_s* g, hipEvent_t ev[2], const hipEvent_t** use)
+{
+    *use = nullptr;
+    if (!g->trace) return MPD_OK;
+    MPD_HIP(hipEventCreate(&ev[0]));
+    MPD_HIP(hipEventCreate(&ev[1]));
+    g->trace->push_back(ev[0]);
+    g->trace->push_back(ev[1]);
+    g->trace_pairs = true;
+    *use = ev;
+    return MPD_OK;
 }
 
 // z-chunk length: enough workgroups to fill 256 CUs several times over, long enough
@@ -716,9 +745,12 @@ int MPD_CAT(periodic_fused_step, MPD_ARITH)(mpd_grid_s* g)
         a.Ax = tgt[0].p;
         a.Ay = tgt[1].p;
         a.Az = tgt[2].p;
-        if ((rc = launch(a, tx, ty, true, p == 1, s))) return rc;
-        trace_mark(g);
+        hipEvent_t ev[2];
+        const hipEvent_t* use;
+        if ((rc = span(g, ev, &use))) return rc;
+        if ((rc = launch(a, tx, ty, true, p == 1, s, use))) return rc;
         const double* pass_in = (p == 1) ? g->cp[0].p : field;
+        if ((rc = span(g, ev, &use))) return rc;
         if (g->nonosc) {
             a.cn = cin;
             a.cp = pass_in;
@@ -727,15 +759,13 @@ int MPD_CAT(periodic_fused_step, MPD_ARITH)(mpd_grid_s* g)
             a.Lx = last ? nullptr : g->fb[0].p;
             a.Ly = last ? nullptr : g->fb[1].p;
             a.Lz = last ? nullptr : g->fb[2].p;
-            if ((rc = launch(a, tx, ty, false, false, s))) return rc;
-            trace_mark(g);
+            if ((rc = launch(a, tx, ty, false, false, s, use))) return rc;
             for (int d = 0; d < 3; d++) prev[d] = g->fb[d].p;
         } else {
-            hipLaunchKernelGGL(k_donor_faces, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0,
-                               s, g->nx, g->ny, g->nz, tgt[0].p, tgt[1].p, tgt[2].p, pass_in,
-                               last ? q : (const double*)nullptr, dst);
+            launch_k(k_donor_faces, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), s, use,
+                     g->nx, g->ny, g->nz, (const double*)tgt[0].p, (const double*)tgt[1].p,
+                     (const double*)tgt[2].p, pass_in, last ? q : (const double*)nullptr, dst);
             MPD_HIP(hipGetLastError());
-            trace_mark(g);
             for (int d = 0; d < 3; d++) prev[d] = tgt[d].p;
         }
         field = dst;

@@ -29,8 +29,11 @@ struct mpd_grid_s {
     mpd::DevBuf bup, bdn;      // limiter ratios
     mpd::DevBuf red;           // reduction partials
 
-    // when set, every kernel launch of a step is followed by an event on the stream
+    // when set, the kernel launches of a step are timed: the fused path launches each
+    // kernel between a start and a stop event of its own (pairs appended to *trace,
+    // trace_pairs set); the other paths record one event after each launch
     std::vector<hipEvent_t>* trace = nullptr;
+    bool trace_pairs = false;
 };
 
 namespace mpd {
