@@ -163,7 +163,10 @@ def main():
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_rotation{n}_iord{args.iord}.json")
     if os.path.exists(tfile) and args.path == 0 and args.arith == 1:
-        traffic = json.load(open(tfile)).get("hbm_bytes_per_step")
+        t = json.load(open(tfile))
+        # counters measured with the same tile shape and z-chunk length only
+        if t.get("config") == tuned:
+            traffic = t.get("hbm_bytes_per_step")
 
     line = {
         "metric": "cell-updates/s at <=1e-12 rel err (FP64 MPDATA step, 1 GPU)",
