@@ -8,8 +8,10 @@
 //   bup, bdn = ratios(c_in, c1, A)                 limiter (nonoscillatory option)
 //   Al  = limit(A; bup, bdn)
 //   c_out = donor(c1; Al) + q                      corrective pass
-// The adjoint runs these stages backwards, each as a gather over the cells or faces
-// that depend on a value (no atomics, deterministic).  Every kink (|v|, max/min,
+// The forward intermediates (c1, A, the limiter's ratios, flux sums and arg-max codes)
+// come from the fused forward kernels, which store them during the recompute of each
+// checkpoint segment.  The adjoint runs the stages backwards, each as a gather over the
+// cells or faces that depend on a value (no atomics, deterministic).  Every kink (|v|, max/min,
 // positive/negative part, the limiter's min-selection) follows the branch the forward
 // took, with the oracle's conventions (oracle/cpu_adj.c), so the GPU adjoint is the
 // transpose of the oracle's tangent-linear model.  Arithmetic is unfused IEEE FP64.
@@ -17,6 +19,7 @@
 #include <vector>
 
 #include "mpd_formulas.hpp"
+#include "mpd_fused.hpp"
 #include "mpd_grid.hpp"
 
 namespace mpd {
@@ -75,22 +78,6 @@ struct Geo {
         return (size_t)wrap2(k, nz) * np + (size_t)wrap2(j, ny) * nx + wrap2(i, nx);
     }
 };
-
-// ---------------------------------------------------------------- forward parts ---
-
-__global__ void k_donor(Geo g, const double* __restrict__ U, const double* __restrict__ V,
-                        const double* __restrict__ W, const double* __restrict__ c,
-                        double* __restrict__ out)
-{
-    size_t t = (size_t)blockIdx.x * TPB + threadIdx.x;
-    if (t >= g.n) return;
-    int i, j, k;
-    g.ijk(t, i, j, k);
-    size_t xl = g.at(i - 1, j, k), xr = g.at(i + 1, j, k), yd = g.at(i, j - 1, k);
-    size_t yu = g.at(i, j + 1, k), zb = g.at(i, j, k - 1), zf = g.at(i, j, k + 1);
-    out[t] = donor(c[t], c[xl], c[xr], c[yd], c[yu], c[zb], c[zf], U[xl], U[t], V[yd], V[t],
-                   W[zb], W[t]);
-}
 
 // The three faces above cell (i,j,k): stencil cells and transverse strokes
 // (the interior formulas of uu.c:81-87, vv.c:85-92, ww.c:248-255).
@@ -156,69 +143,10 @@ __device__ FaceSt face_st(const Geo& g, const double* U, const double* V, const 
     return f;
 }
 
-__global__ void k_antidiff(Geo g, const double* __restrict__ U, const double* __restrict__ V,
-                           const double* __restrict__ W, const double* __restrict__ c,
-                           double* __restrict__ A0, double* __restrict__ A1, double* __restrict__ A2)
-{
-    size_t t = (size_t)blockIdx.x * TPB + threadIdx.x;
-    if (t >= g.n) return;
-    int i, j, k;
-    g.ijk(t, i, j, k);
-    double* out[3] = {A0, A1, A2};
-    for (int fam = 0; fam < 3; fam++) {
-        FaceSt f = face_st(g, U, V, W, i, j, k, fam);
-        double n1 = c[f.up1[0]] + c[f.up1[1]] - c[f.dn1[0]] - c[f.dn1[1]];
-        double d1 = c[f.up1[0]] + c[f.up1[1]] + c[f.dn1[0]] + c[f.dn1[1]];
-        double n2 = c[f.up2[0]] + c[f.up2[1]] - c[f.dn2[0]] - c[f.dn2[1]];
-        double d2 = c[f.up2[0]] + c[f.up2[1]] + c[f.dn2[0]] + c[f.dn2[1]];
-        out[fam][t] = antidiff(f.div, f.a, c[f.p], c[f.m], f.s1, n1, d1, f.s2, n2, d2);
-    }
-}
-
-// Limiter ratios plus what the adjoint needs: the flux sums and which element of the
-// 7-point (c_in | c1) neighbourhood holds the maximum / minimum (code = 2*q + field,
-// q over {self, x-, x+, y-, y+, z-, z+}, field 0 = c_in, 1 = c1).
-__global__ void k_ratios(Geo g, const double* __restrict__ c0, const double* __restrict__ c1,
-                         const double* __restrict__ A0, const double* __restrict__ A1,
-                         const double* __restrict__ A2, double* __restrict__ bup,
-                         double* __restrict__ bdn, double* __restrict__ ain,
-                         double* __restrict__ aout, int* __restrict__ code)
-{
-    size_t t = (size_t)blockIdx.x * TPB + threadIdx.x;
-    if (t >= g.n) return;
-    int i, j, k;
-    g.ijk(t, i, j, k);
-    const size_t nb[7] = {t, g.at(i - 1, j, k), g.at(i + 1, j, k), g.at(i, j - 1, k),
-                          g.at(i, j + 1, k), g.at(i, j, k - 1), g.at(i, j, k + 1)};
-    double hi = c0[t], lo = c0[t];
-    int hc = 0, lc = 0;
-    for (int q = 0; q < 7; q++) {
-        double a = c0[nb[q]], b = c1[nb[q]];
-        double mh = a > b ? a : b;
-        int mhc = 2 * q + (a > b ? 0 : 1);
-        if (!(hi > mh)) { hi = mh; hc = mhc; }
-        double ml = a < b ? a : b;
-        int mlc = 2 * q + (a < b ? 0 : 1);
-        if (!(lo < ml)) { lo = ml; lc = mlc; }
-    }
-    double c = c1[t];
-    double fxl = 0.5 * F2(c1[nb[1]], c, A0[nb[1]]), fxr = 0.5 * F2(c, c1[nb[2]], A0[t]);
-    double fyl = 0.5 * F2(c1[nb[3]], c, A1[nb[3]]), fyr = 0.5 * F2(c, c1[nb[4]], A1[t]);
-    double fzl = 0.5 * F2(c1[nb[5]], c, A2[nb[5]]), fzr = 0.5 * F2(c, c1[nb[6]], A2[t]);
-    double si = ((pos(fxl) - neg(fxr)) + (pos(fyl) - neg(fyr))) + (pos(fzl) - neg(fzr));
-    double so = ((pos(fxr) - neg(fxl)) + (pos(fyr) - neg(fyl))) + (pos(fzr) - neg(fzl));
-    ain[t] = si;
-    aout[t] = so;
-    bup[t] = (hi - c) / (si + MINV);
-    bdn[t] = (c - lo) / (so + MINV);
-    code[t] = hc | (lc << 8);
-}
-
 // limited velocity of a face with unlimited velocity a between lower cell l, upper r
 __device__ __forceinline__ double limited(double a, double bu_l, double bd_l, double bu_r,
-                                          double bd_r, bool nonosc)
+                                          double bd_r)
 {
-    if (!nonosc) return a;
     if (a > 0) {
         int w = pick(bd_l, bu_r);
         return (w == 0 ? 1.0 : (w == 1 ? bd_l : bu_r)) * a;
@@ -266,14 +194,17 @@ __global__ void k_adj_donor2(Geo g, AdjBufs b, const double* __restrict__ lout,
     double acc = lout[t];
     for (int d = 0; d < 3; d++) {
         double gl, gr, gv;
-        // upper face (t, up): lam_f = lout[up] - lout[t]
-        double au = limited(fam_a(b, d, t), b.bup[t], b.bdn[t], b.bup[up[d]], b.bdn[up[d]], b.nonosc);
+        // upper face (t, up): lam_f = lout[up] - lout[t].  The ratio arrays exist only
+        // with the limiter: read them only then.
+        double au = fam_a(b, d, t);
+        if (b.nonosc) au = limited(au, b.bup[t], b.bdn[t], b.bup[up[d]], b.bdn[up[d]]);
         fpart(b.c1[t], b.c1[up[d]], au, gl, gr, gv);
         double lf = lout[up[d]] - lout[t];
         acc += gl * lf;
         lAl[d][t] = gv * lf;
         // lower face (lo, t)
-        double al = limited(fam_a(b, d, lo[d]), b.bup[lo[d]], b.bdn[lo[d]], b.bup[t], b.bdn[t], b.nonosc);
+        double al = fam_a(b, d, lo[d]);
+        if (b.nonosc) al = limited(al, b.bup[lo[d]], b.bdn[lo[d]], b.bup[t], b.bdn[t]);
         fpart(b.c1[lo[d]], b.c1[t], al, gl, gr, gv);
         acc += gr * (lout[t] - lout[lo[d]]);
     }
@@ -547,28 +478,52 @@ __global__ void k_receptors(const double* __restrict__ c, const int* __restrict_
     if (lam) lam[rec[r]] += d;  // receptor cells are distinct (checked on the host)
 }
 
-struct Work {
-    DevBuf c1, A[3], bup, bdn, ain, aout, lAl[3], lA[3], iu, id, lain, laout, l1;
-    DevBuf coef[18];
+// Forward intermediates of one step, stored by the fused forward kernels.
+struct Fwd {
+    DevBuf c1, A[3], bup, bdn, ain, aout;
     int* code = nullptr;
-    double** dcoef = nullptr;
-    size_t n = 0;
-    ~Work()
+    Fwd() = default;
+    Fwd(const Fwd&) = delete;
+    Fwd& operator=(const Fwd&) = delete;
+    ~Fwd()
     {
         if (code) (void)hipFree(code);
+    }
+    int alloc(size_t cells, bool nonosc)
+    {
+        int rc;
+        DevBuf* all[] = {&c1, &A[0], &A[1], &A[2], &bup, &bdn, &ain, &aout};
+        for (int i = 0; i < (nonosc ? 8 : 4); i++)
+            if ((rc = all[i]->alloc(cells))) return rc;
+        if (nonosc) MPD_HIP(hipMalloc((void**)&code, cells * sizeof(int)));
+        return MPD_OK;
+    }
+    // one exact step cin -> cout storing this step's intermediates
+    int step(mpd_grid_s* g, const double* cin, double* cout)
+    {
+        double* const A3[3] = {A[0].p, A[1].p, A[2].p};
+        return fused_store_step_exact(g, cin, cout, c1.p, A3, bup.p, bdn.p, ain.p, aout.p, code);
+    }
+};
+
+// Adjoint intermediates of one step.
+struct Work {
+    DevBuf lAl[3], lA[3], iu, id, lain, laout, l1;
+    DevBuf coef[18];
+    double** dcoef = nullptr;
+    ~Work()
+    {
         if (dcoef) (void)hipFree(dcoef);
     }
     int alloc(size_t cells)
     {
         int rc;
-        n = cells;
-        DevBuf* all[] = {&c1, &A[0], &A[1], &A[2], &bup, &bdn, &ain, &aout, &lAl[0], &lAl[1],
-                         &lAl[2], &lA[0], &lA[1], &lA[2], &iu, &id, &lain, &laout, &l1};
+        DevBuf* all[] = {&lAl[0], &lAl[1], &lAl[2], &lA[0], &lA[1], &lA[2], &iu, &id, &lain,
+                         &laout, &l1};
         for (DevBuf* d : all)
             if ((rc = d->alloc(cells))) return rc;
         for (auto& c : coef)
             if ((rc = c.alloc(cells))) return rc;
-        MPD_HIP(hipMalloc((void**)&code, cells * sizeof(int)));
         MPD_HIP(hipMalloc((void**)&dcoef, 18 * sizeof(double*)));
         double* h[18];
         for (int q = 0; q < 18; q++) h[q] = coef[q].p;
@@ -577,22 +532,16 @@ struct Work {
     }
 };
 
-// lin = A'(c_in)^T lout; lq += lout (lq may be null)
-int adjoint_step(mpd_grid_s* g, Work& w, const double* cin, const double* lout, double* lin,
-                 double* lq)
+// lin = A'(c_in)^T lout given the step's stored forward f; lq += lout (lq may be null)
+int adjoint_step(mpd_grid_s* g, Work& w, const Fwd& f, const double* cin, const double* lout,
+                 double* lin, double* lq)
 {
     hipStream_t s = stream();
     Geo geo{g->nx, g->ny, g->nz, (size_t)g->nx * g->ny, g->ncell};
     const unsigned nb = nblk(g->ncell);
     const double *U = g->face[0].p, *V = g->face[1].p, *W = g->face[2].p;
-    hipLaunchKernelGGL(k_donor, dim3(nb), dim3(TPB), 0, s, geo, U, V, W, cin, w.c1.p);
-    hipLaunchKernelGGL(k_antidiff, dim3(nb), dim3(TPB), 0, s, geo, U, V, W, w.c1.p, w.A[0].p,
-                       w.A[1].p, w.A[2].p);
-    if (g->nonosc)
-        hipLaunchKernelGGL(k_ratios, dim3(nb), dim3(TPB), 0, s, geo, cin, w.c1.p, w.A[0].p,
-                           w.A[1].p, w.A[2].p, w.bup.p, w.bdn.p, w.ain.p, w.aout.p, w.code);
-    AdjBufs b{U, V, W, cin, w.c1.p, w.A[0].p, w.A[1].p, w.A[2].p, w.bup.p, w.bdn.p,
-              w.ain.p, w.aout.p, w.code, g->nonosc};
+    AdjBufs b{U, V, W, cin, f.c1.p, f.A[0].p, f.A[1].p, f.A[2].p, f.bup.p, f.bdn.p,
+              f.ain.p, f.aout.p, f.code, g->nonosc};
     hipLaunchKernelGGL(k_adj_donor2, dim3(nb), dim3(TPB), 0, s, geo, b, lout, w.l1.p, w.lAl[0].p,
                        w.lAl[1].p, w.lAl[2].p, lq);
     if (g->nonosc) {
@@ -619,6 +568,16 @@ int adjoint_step(mpd_grid_s* g, Work& w, const double* cin, const double* lout, 
     return MPD_OK;
 }
 
+// The adjoint linearises the exact fused step; grids it cannot run on are refused.
+int adjoint_supported(const mpd_grid_s* g)
+{
+    if (g->boundary != MPD_PERIODIC || g->iord != 2)
+        return fail(MPD_ERR_UNSUPPORTED, "the adjoint covers PERIODIC grids with iord = 2");
+    if (!fused_plane_fits(g->nx, g->ny))
+        return fail(MPD_ERR_UNSUPPORTED, "the adjoint needs z-planes of at most 2^29 cells");
+    return MPD_OK;
+}
+
 }  // namespace
 
 // One adjoint step on host arrays (for tests): lin = A'(c)^T lout, lq += lout.
@@ -635,21 +594,23 @@ extern "C" int mpd_adjoint_step(mpd_grid g, const double* c, const double* lout,
 {
     clear_error();
     if (!g || !c || !lout || !lin) return fail(MPD_ERR_ARG, "mpd_adjoint_step: NULL argument");
-    if (g->boundary != MPD_PERIODIC || g->iord != 2)
-        return fail(MPD_ERR_UNSUPPORTED, "the adjoint covers PERIODIC grids with iord = 2");
+    int rc;
+    if ((rc = adjoint_supported(g))) return rc;
     if (!g->has_winds) return fail(MPD_ERR_ARG, "mpd_adjoint_step: set the Courant numbers first");
+    if ((rc = ensure_fused_scratch(g))) return rc;
     Work w;
-    int rc = w.alloc(g->ncell);
-    if (rc) return rc;
-    DevBuf dc, dl, di, dq;
+    Fwd f;
+    if ((rc = w.alloc(g->ncell)) || (rc = f.alloc(g->ncell, g->nonosc))) return rc;
+    DevBuf dc, dl, di, dq, dout;
     hipStream_t s = stream();
     if ((rc = dc.alloc(g->ncell)) || (rc = dl.alloc(g->ncell)) || (rc = di.alloc(g->ncell)) ||
-        (rc = dq.alloc(g->ncell)))
+        (rc = dq.alloc(g->ncell)) || (rc = dout.alloc(g->ncell)))
         return rc;
     MPD_HIP(hipMemcpyAsync(dc.p, c, g->ncell * 8, hipMemcpyHostToDevice, s));
     MPD_HIP(hipMemcpyAsync(dl.p, lout, g->ncell * 8, hipMemcpyHostToDevice, s));
     if (lq) MPD_HIP(hipMemcpyAsync(dq.p, lq, g->ncell * 8, hipMemcpyHostToDevice, s));
-    if ((rc = adjoint_step(g, w, dc.p, dl.p, di.p, lq ? dq.p : nullptr))) return rc;
+    if ((rc = f.step(g, dc.p, dout.p))) return rc;
+    if ((rc = adjoint_step(g, w, f, dc.p, dl.p, di.p, lq ? dq.p : nullptr))) return rc;
     MPD_HIP(hipMemcpyAsync(lin, di.p, g->ncell * 8, hipMemcpyDeviceToHost, s));
     if (lq) MPD_HIP(hipMemcpyAsync(lq, dq.p, g->ncell * 8, hipMemcpyDeviceToHost, s));
     MPD_HIP(hipStreamSynchronize(s));
@@ -714,8 +675,8 @@ extern "C" int mpd_adjoint_run(mpd_grid g, int nsteps, int checkpoint_every, int
     if (!g || nsteps < 1 || checkpoint_every < 1 || nrec < 1 || !receptors ||
         sample_every < 1 || !obs || !J)
         return fail(MPD_ERR_ARG, "mpd_adjoint_run: bad arguments");
-    if (g->boundary != MPD_PERIODIC || g->iord != 2)
-        return fail(MPD_ERR_UNSUPPORTED, "the adjoint covers PERIODIC grids with iord = 2");
+    int rc;
+    if ((rc = adjoint_supported(g))) return rc;
     if (!g->has_winds || !g->has_field)
         return fail(MPD_ERR_ARG, "mpd_adjoint_run: set the Courant numbers and the field first");
     {
@@ -730,7 +691,6 @@ extern "C" int mpd_adjoint_run(mpd_grid g, int nsteps, int checkpoint_every, int
     const int K = checkpoint_every;
     const int nck = (nsteps + K - 1) / K;  // checkpoints at 0, K, 2K, ...
     hipStream_t s = stream();
-    int rc;
     const size_t n = g->ncell;
     // arithmetic of the forward: the adjoint linearises exact-arithmetic steps
     const int saved_arith = g->arith;
@@ -792,22 +752,28 @@ extern "C" int mpd_adjoint_run(mpd_grid g, int nsteps, int checkpoint_every, int
         DevBuf lam, lnext, gq, tmp;
         if ((rc = lam.alloc(n)) || (rc = lnext.alloc(n)) || (rc = gq.alloc(n)) || (rc = tmp.alloc(n)))
             return rc;
+        // a segment's states and the intermediates of its steps (about 10 arrays per step)
         std::vector<DevBuf> seg(K);
-        for (auto& b : seg)
-            if ((rc = b.alloc(n))) return rc;
+        std::vector<Fwd> fw(K);
+        for (int m = 0; m < K; m++)
+            if ((rc = seg[m].alloc(n)) || (rc = fw[m].alloc(n, g->nonosc)))
+                return rc == MPD_ERR_NOMEM
+                           ? fail(rc, "mpd_adjoint_run: device memory for %d stored steps of "
+                                      "%zu cells exhausted; use a shorter checkpoint interval",
+                                  K, n)
+                           : rc;
         MPD_HIP(hipMemsetAsync(lam.p, 0, n * 8, s));
         MPD_HIP(hipMemsetAsync(gq.p, 0, n * 8, s));
         // a scratch grid view that steps the segment states: reuse g's buffers
         const int cur0 = g->cur;
         for (int c = nck - 1; c >= 0; c--) {
             const int s0 = c * K, s1 = std::min(nsteps, s0 + K);
-            // recompute c^{s0} .. c^{s1-1} into seg[0 .. s1-s0-1]
+            // recompute c^{s0} .. c^{s1-1} into seg[0 .. s1-s0-1] with each step's
+            // intermediates (the last step's result, c^{s1}, is not needed again)
             MPD_HIP(hipMemcpyAsync(seg[0].p, ck[c].p, n * 8, hipMemcpyDeviceToDevice, s));
-            for (int m = 1; m < s1 - s0; m++) {
-                MPD_HIP(hipMemcpyAsync(g->c[g->cur].p, seg[m - 1].p, n * 8, hipMemcpyDeviceToDevice, s));
-                if ((rc = periodic_step(g))) return rc;
-                MPD_HIP(hipMemcpyAsync(seg[m].p, g->c[g->cur].p, n * 8, hipMemcpyDeviceToDevice, s));
-            }
+            for (int m = 0; m < s1 - s0; m++)
+                if ((rc = fw[m].step(g, seg[m].p, m + 1 < s1 - s0 ? seg[m + 1].p : tmp.p)))
+                    return rc;
             for (int step = s1; step > s0; step--) {  // lam holds lam^{step}
                 if (step % sample_every == 0) {
                     // the residual needs c^{step}: c^{s1} is the next segment's first state
@@ -822,7 +788,9 @@ extern "C" int mpd_adjoint_run(mpd_grid g, int nsteps, int checkpoint_every, int
                     hipLaunchKernelGGL(k_receptors, dim3(nblk(nrec)), dim3(TPB), 0, s, cs, drec,
                                        nrec, dobs.p + (size_t)sm * nrec, (double*)nullptr, lam.p);
                 }
-                if ((rc = adjoint_step(g, w, seg[step - 1 - s0].p, lam.p, lnext.p, gq.p))) return rc;
+                if ((rc = adjoint_step(g, w, fw[step - 1 - s0], seg[step - 1 - s0].p, lam.p,
+                                       lnext.p, gq.p)))
+                    return rc;
                 std::swap(lam.p, lnext.p);
             }
         }

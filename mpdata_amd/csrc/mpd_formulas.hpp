@@ -96,12 +96,19 @@ __device__ __forceinline__ double limit_face(double u, double bup_l, double bdn_
 // fluxes (0.5 F2) of the pass: bup = (hi - c)/(Ain + MINV), bdn = (c - lo)/(Aout + MINV).
 __device__ __forceinline__ void beta(double hi, double lo, double c, double fxl, double fxr,
                                      double fyl, double fyr, double fzl, double fzr,
-                                     double& bup, double& bdn)
+                                     double& bup, double& bdn, double& ain, double& aout)
 {
-    double ain = ((pos(fxl) - neg(fxr)) + (pos(fyl) - neg(fyr))) + (pos(fzl) - neg(fzr));
-    double aout = ((pos(fxr) - neg(fxl)) + (pos(fyr) - neg(fyl))) + (pos(fzr) - neg(fzl));
+    ain = ((pos(fxl) - neg(fxr)) + (pos(fyl) - neg(fyr))) + (pos(fzl) - neg(fzr));
+    aout = ((pos(fxr) - neg(fxl)) + (pos(fyr) - neg(fyl))) + (pos(fzr) - neg(fzl));
     bup = dv(hi - c, ain + MINV);
     bdn = dv(c - lo, aout + MINV);
+}
+__device__ __forceinline__ void beta(double hi, double lo, double c, double fxl, double fxr,
+                                     double fyl, double fyr, double fzl, double fzr,
+                                     double& bup, double& bdn)
+{
+    double ain, aout;
+    beta(hi, lo, c, fxl, fxr, fyl, fyr, fzl, fzr, bup, bdn, ain, aout);
 }
 
 }  // namespace MPD_ARITH

@@ -478,7 +478,8 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
         const double* cpp = s_cp[s3];
         auto ratios = [&](int M, double cn0, double cnlo, double cnhi, double c, double cplo,
                           double cphi, double axr, double ayr, double azlo, double azhi,
-                          double axl, double ayl, double& bu, double& bd) {
+                          double axl, double ayl, double& bu, double& bd, double& ain,
+                          double& aout, int* code) {
             double hi = cn0, lo = cn0;
             const double nbn[7] = {cn0, s_cn[M - 1], s_cn[M + 1], s_cn[M - RX], s_cn[M + RX],
                                    cnlo, cnhi};
@@ -492,17 +493,44 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
             const double fxl = 0.5 * F2(nbp[1], c, axl), fxr = 0.5 * F2(c, nbp[2], axr);
             const double fyl = 0.5 * F2(nbp[3], c, ayl), fyr = 0.5 * F2(c, nbp[4], ayr);
             const double fzl = 0.5 * F2(cplo, c, azlo), fzr = 0.5 * F2(c, cphi, azhi);
-            beta(hi, lo, c, fxl, fxr, fyl, fyr, fzl, fzr, bu, bd);
+            beta(hi, lo, c, fxl, fxr, fyl, fyr, fzl, fzr, bu, bd, ain, aout);
+            if (code) {  // which element holds each bound (later elements win ties)
+                double h = cn0, l = cn0;
+                int hc = 0, lc = 0;
+#pragma unroll
+                for (int m = 0; m < 7; m++) {
+                    const double x = nbn[m], y = nbp[m];
+                    const double mh = x > y ? x : y, ml = x < y ? x : y;
+                    if (!(h > mh)) {
+                        h = mh;
+                        hc = 2 * m + (x > y ? 0 : 1);
+                    }
+                    if (!(l < ml)) {
+                        l = ml;
+                        lc = 2 * m + (x < y ? 0 : 1);
+                    }
+                }
+                *code = hc | (lc << 8);
+            }
         };
         const double axl = s_ax[L - 1], ayd = s_ay[L - RX];
-        double buc, bdc;
-        ratios(L, cnc, cnm, cnn, cpc, cpm, cpn, axn, ayn, azm, azc, axl, ayd, buc, bdc);
+        double buc, bdc, ain, aout;
+        int code = 0;
+        ratios(L, cnc, cnm, cnn, cpc, cpm, cpn, axn, ayn, azm, azc, axl, ayd, buc, bdc, ain, aout,
+               a.code ? &code : nullptr);
+        if (a.bu && store && p >= z0 && p < z1) {
+            st(a.bu + zp0.off, colb, buc);
+            st(a.bd + zp0.off, colb, bdc);
+            st(a.ain + zp0.off, colb, ain);
+            st(a.aout + zp0.off, colb, aout);
+            *(int*)((char*)(a.code + zp0.off) + colb / 2) = code;
+        }
         s_bu[s2][L] = buc;
         s_bd[s2][L] = bdc;
         if (h1) {
-            double hbu, hbd;
+            double hbu, hbd, hain, haout;
             ratios(HL, hcnc, hcnm, hcnn, hcpc, hcpm, hcpn, haxn, hayn, hazm, hazc,
-                   s_ax[HL - 1], s_ay[HL - RX], hbu, hbd);
+                   s_ax[HL - 1], s_ay[HL - RX], hbu, hbd, hain, haout, nullptr);
             s_bu[s2][HL] = hbu;
             s_bd[s2][HL] = hbd;
         }
@@ -712,6 +740,49 @@ int MPD_CAT(fused_resident_blocks, MPD_ARITH)(int tx, int ty, bool f1)
     MPD_FUSED_TILES(MPD_TILE)
 #undef MPD_TILE
     return e == hipSuccess ? n : 0;
+}
+
+int MPD_CAT(fused_store_step, MPD_ARITH)(mpd_grid_s* g, const double* cin, double* cout,
+                                         double* c1, double* const A[3], double* bu,
+                                         double* bd, double* ain, double* aout, int* code)
+{
+    hipStream_t s = stream();
+    int rc;
+    const int tx = g->tile_x ? g->tile_x : 32, ty = g->tile_y ? g->tile_y : 8;
+    FusedArgs a{};
+    a.nx = g->nx;
+    a.ny = g->ny;
+    a.nz = g->nz;
+    a.np = (long long)g->nx * g->ny;
+    a.zc = g->zc ? g->zc : choose_zc(g, tx, ty);
+    a.cin = cin;
+    a.U = g->face[0].p;
+    a.V = g->face[1].p;
+    a.W = g->face[2].p;
+    a.c1out = c1;
+    a.Ax = A[0];
+    a.Ay = A[1];
+    a.Az = A[2];
+    if ((rc = launch(a, tx, ty, true, true, s, nullptr))) return rc;
+    const double* q = g->has_q ? g->q.p : nullptr;
+    if (g->nonosc) {
+        a.cn = cin;
+        a.cp = c1;
+        a.q = q;
+        a.cout = cout;
+        a.bu = bu;
+        a.bd = bd;
+        a.ain = ain;
+        a.aout = aout;
+        a.code = code;
+        return launch(a, tx, ty, false, false, s, nullptr);
+    }
+    const size_t n = g->ncell;
+    launch_k(k_donor_faces, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), s, nullptr, g->nx,
+             g->ny, g->nz, (const double*)A[0], (const double*)A[1], (const double*)A[2],
+             (const double*)c1, q, cout);
+    MPD_HIP(hipGetLastError());
+    return MPD_OK;
 }
 
 int MPD_CAT(periodic_fused_step, MPD_ARITH)(mpd_grid_s* g)

@@ -3,6 +3,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+struct mpd_grid_s;
+
 namespace mpd {
 
 // Tile shapes compiled for the fused kernels (x extent, y extent of the column tile).
@@ -24,6 +26,12 @@ struct FusedArgs {
     const double* q;          // emission increment or null
     double* cout;             // pass result
     double *Lx, *Ly, *Lz;     // limited faces or null
+    // F2, optional (the adjoint's stored forward): limiter data of the tile cells -- the
+    // ratios, the flux sums and which neighbourhood element holds the bounds
+    // (code = hc | lc << 8, hc / lc = 2*q + field, q over {self, x-, x+, y-, y+, z-, z+},
+    // field 0 = c^n, 1 = pass input; ties go to the later element)
+    double *bu, *bd, *ain, *aout;
+    int* code;
 };
 
 // Tile counts of an instance for a tile shape (zc must be set).
@@ -46,6 +54,12 @@ int fused_batch_launch_exact(int tx, int ty, bool f1, bool donor, const FusedArg
                              const int* item0, int ninst, int nitems, int grid, hipStream_t s);
 int fused_batch_launch_fast(int tx, int ty, bool f1, bool donor, const FusedArgs* table,
                             const int* item0, int ninst, int nitems, int grid, hipStream_t s);
+// One exact-arithmetic iord-2 step cin -> cout of a PERIODIC grid that also stores the
+// forward's intermediates for the adjoint: the donor field c1, the antidiffusive faces
+// A[3] and, with the limiter, its data (see FusedArgs; null pointers when unlimited).
+int fused_store_step_exact(mpd_grid_s* g, const double* cin, double* cout, double* c1,
+                           double* const A[3], double* bu, double* bd, double* ain,
+                           double* aout, int* code);
 // Resident workgroups per CU of the F1 (donor) or F2 kernel of a tile shape (0: unknown
 // shape).
 int fused_resident_blocks_exact(int tx, int ty, bool f1);
