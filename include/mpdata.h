@@ -136,9 +136,12 @@ int mpd_generate_blobs(mpd_grid g, int target, int nblobs, const double* centres
  * NULL.  Every kink follows the branch the forward step takes. */
 int mpd_adjoint_step(mpd_grid g, const double* c, const double* lout, double* lin, double* lq);
 /* How the adjoint sums the antidiffusive-velocity contributions: 0 (default) gathers
- * through per-face coefficient arrays -- deterministic, bit-reproducible; 1 scatters
- * with hardware FP64 atomics -- no coefficient arrays, last bits vary run to run. */
+ * them in an LDS-tiled kernel -- deterministic, bit-reproducible; 1 scatters with
+ * hardware FP64 atomics -- last bits vary run to run. */
 int mpd_set_adjoint_scatter(mpd_grid g, int atomic);
+/* Kernels of the adjoint's limiter stages: 1 (default) one LDS-tiled kernel, 0 two
+ * per-cell passes through device arrays (the same arithmetic, for cross-checks). */
+int mpd_set_adjoint_tiled(mpd_grid g, int tiled);
 /* Forward run of nsteps steps from the grid's field recording the receptor values every
  * sample_every steps: out[(s-1)*nrec + r] = c^{s*sample_every}[rec_r]. */
 int mpd_sample_receptors(mpd_grid g, int nsteps, int nrec, const int* receptors,

@@ -86,6 +86,7 @@ SIGNATURES = {
     "mpd_generate_wind": (C.c_int, [C.c_void_p, C.c_void_p]),
     "mpd_generate_blobs": (C.c_int, [C.c_void_p, C.c_int, C.c_int, PD, PD, PD, C.c_int]),
     "mpd_set_adjoint_scatter": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpd_set_adjoint_tiled": (C.c_int, [C.c_void_p, C.c_int]),
     "mpd_sample_receptors": (C.c_int, [C.c_void_p, C.c_int, C.c_int, PI, C.c_int, PD]),
     "mpd_adjoint_step": (C.c_int, [C.c_void_p, PD, PD, PD, PD]),
     "mpd_adjoint_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, PI, C.c_int, PD, PD,
@@ -301,6 +302,9 @@ class Grid:
 
     def set_adjoint_scatter(self, atomic):
         _check(lib().mpd_set_adjoint_scatter(self.h, int(atomic)))
+
+    def set_adjoint_tiled(self, tiled):
+        _check(lib().mpd_set_adjoint_tiled(self.h, int(tiled)))
 
     def sample_receptors(self, nsteps, receptors, sample_every):
         rec = np.ascontiguousarray(receptors, dtype=np.int32)

@@ -21,6 +21,7 @@
 #include "mpd_formulas.hpp"
 #include "mpd_fused.hpp"
 #include "mpd_grid.hpp"
+#include "mpd_tile.hpp"
 
 namespace mpd {
 int ensure_fused_scratch(mpd_grid_s* g);
@@ -180,90 +181,94 @@ __device__ __forceinline__ double fam_a(const AdjBufs& b, int d, size_t o)
     return d == 0 ? b.A0[o] : (d == 1 ? b.A1[o] : b.A2[o]);
 }
 
-// corrective donor pass transposed: l1 = lout + gather; lAl of the upper faces; lq += lout
-__global__ void k_adj_donor2(Geo g, AdjBufs b, const double* __restrict__ lout,
-                             double* __restrict__ l1, double* __restrict__ lAl0,
-                             double* __restrict__ lAl1, double* __restrict__ lAl2,
-                             double* __restrict__ lq)
+// Corrective donor pass and limiter part 1, transposed, in one pass over the cells:
+//   l1 = lout + the donor gather over the six faces (limited velocities);
+//   lam_Al of each face of the cell = gv * (lout[upper] - lout[lower]) (recomputed for
+//   the lower faces rather than read back); lA = m * lam_Al on the upper faces, m the
+//   limiter's multiplier; the multipliers' ratio adjoints, gathered from the six faces,
+//   turned into bound and flux-sum adjoints (iu, id, lain, laout).  Without the limiter
+//   lA = lam_Al and the ratio outputs are untouched.  lq += lout.
+__global__ void k_adj_donor_limit(Geo g, AdjBufs b, const double* __restrict__ lout,
+                                  double* __restrict__ l1, double* __restrict__ lA0,
+                                  double* __restrict__ lA1, double* __restrict__ lA2,
+                                  double* __restrict__ iu, double* __restrict__ id,
+                                  double* __restrict__ lain, double* __restrict__ laout,
+                                  double* __restrict__ lq)
 {
     size_t t = (size_t)blockIdx.x * TPB + threadIdx.x;
     if (t >= g.n) return;
     size_t lo[3], up[3];
     nbrs(g, t, lo, up);
-    double* lAl[3] = {lAl0, lAl1, lAl2};
-    double acc = lout[t];
+    double* lA[3] = {lA0, lA1, lA2};
+    const double lt = lout[t], ct = b.c1[t];
+    double bu_t = 0.0, bd_t = 0.0;
+    if (b.nonosc) {
+        bu_t = b.bup[t];
+        bd_t = b.bdn[t];
+    }
+    double acc = lt, su = 0.0, sd = 0.0;
     for (int d = 0; d < 3; d++) {
         double gl, gr, gv;
-        // upper face (t, up): lam_f = lout[up] - lout[t].  The ratio arrays exist only
-        // with the limiter: read them only then.
-        double au = fam_a(b, d, t);
-        if (b.nonosc) au = limited(au, b.bup[t], b.bdn[t], b.bup[up[d]], b.bdn[up[d]]);
-        fpart(b.c1[t], b.c1[up[d]], au, gl, gr, gv);
-        double lf = lout[up[d]] - lout[t];
-        acc += gl * lf;
-        lAl[d][t] = gv * lf;
-        // lower face (lo, t)
-        double al = fam_a(b, d, lo[d]);
-        if (b.nonosc) al = limited(al, b.bup[lo[d]], b.bdn[lo[d]], b.bup[t], b.bdn[t]);
-        fpart(b.c1[lo[d]], b.c1[t], al, gl, gr, gv);
-        acc += gr * (lout[t] - lout[lo[d]]);
+        {  // upper face (t, r): t is the lower cell
+            const size_t r = up[d];
+            const double a = fam_a(b, d, t);
+            double au = a, m = 1.0;
+            if (b.nonosc) {
+                const double bu_r = b.bup[r], bd_r = b.bdn[r];
+                if (a > 0) {
+                    const int w = pick(bd_t, bu_r);
+                    m = w == 0 ? 1.0 : (w == 1 ? bd_t : bu_r);
+                } else if (a < 0) {
+                    const int w = pick(bu_t, bd_r);
+                    m = w == 0 ? 1.0 : (w == 1 ? bu_t : bd_r);
+                } else {
+                    m = 0.0;
+                }
+                au = limited(a, bu_t, bd_t, bu_r, bd_r);
+            }
+            fpart(ct, b.c1[r], au, gl, gr, gv);
+            const double lf = lout[r] - lt;
+            acc += gl * lf;
+            const double L = gv * lf;
+            if (b.nonosc) {
+                if (a > 0 && pick(bd_t, b.bup[r]) == 1) sd += a * L;
+                if (a < 0 && pick(bu_t, b.bdn[r]) == 1) su += a * L;
+            }
+            lA[d][t] = m * L;
+        }
+        {  // lower face (l, t): t is the upper cell
+            const size_t l = lo[d];
+            const double a = fam_a(b, d, l);
+            double al = a;
+            double bu_l = 0.0, bd_l = 0.0;
+            if (b.nonosc) {
+                bu_l = b.bup[l];
+                bd_l = b.bdn[l];
+                al = limited(a, bu_l, bd_l, bu_t, bd_t);
+            }
+            fpart(b.c1[l], ct, al, gl, gr, gv);
+            const double lf = lt - lout[l];
+            acc += gr * lf;
+            if (b.nonosc) {
+                const double L = gv * lf;
+                if (a > 0 && pick(bd_l, bu_t) == 2) su += a * L;
+                if (a < 0 && pick(bu_l, bd_t) == 2) sd += a * L;
+            }
+        }
     }
     l1[t] = acc;
-    if (lq) lq[t] += lout[t];
-}
-
-// limiter transposed, part 1: lA = m * lAl on the upper faces; the ratio adjoints,
-// gathered from the six faces of the cell, turned into bound and flux-sum adjoints
-__global__ void k_adj_limit1(Geo g, AdjBufs b, const double* __restrict__ lAl0,
-                             const double* __restrict__ lAl1, const double* __restrict__ lAl2,
-                             double* __restrict__ lA0, double* __restrict__ lA1,
-                             double* __restrict__ lA2, double* __restrict__ iu,
-                             double* __restrict__ id, double* __restrict__ lain,
-                             double* __restrict__ laout)
-{
-    size_t t = (size_t)blockIdx.x * TPB + threadIdx.x;
-    if (t >= g.n) return;
-    size_t lo[3], up[3];
-    nbrs(g, t, lo, up);
-    const double* lAl[3] = {lAl0, lAl1, lAl2};
-    double* lA[3] = {lA0, lA1, lA2};
-    double su = 0.0, sd = 0.0;
-    for (int d = 0; d < 3; d++) {
-        // upper face: t is the lower cell
-        size_t r = up[d];
-        double a = fam_a(b, d, t), L = lAl[d][t], m = 1.0;
-        if (a > 0) {
-            int w = pick(b.bdn[t], b.bup[r]);
-            m = w == 0 ? 1.0 : (w == 1 ? b.bdn[t] : b.bup[r]);
-            if (w == 1) sd += a * L;
-        } else if (a < 0) {
-            int w = pick(b.bup[t], b.bdn[r]);
-            m = w == 0 ? 1.0 : (w == 1 ? b.bup[t] : b.bdn[r]);
-            if (w == 1) su += a * L;
-        } else {
-            m = 0.0;
-        }
-        lA[d][t] = m * L;
-        // lower face: t is the upper cell
-        size_t l = lo[d];
-        a = fam_a(b, d, l);
-        L = lAl[d][l];
-        if (a > 0) {
-            if (pick(b.bdn[l], b.bup[t]) == 2) su += a * L;
-        } else if (a < 0) {
-            if (pick(b.bup[l], b.bdn[t]) == 2) sd += a * L;
-        }
-    }
+    if (lq) lq[t] += lt;
+    if (!b.nonosc) return;
     // ratio adjoints -> bound / flux-sum adjoints (pointwise).  A ratio that was not
     // selected carries no adjoint and may be inf: skipped.
     double u = 0.0, dn = 0.0, li = 0.0, lo2 = 0.0;
     if (su != 0.0) {
         u = su / (b.ain[t] + MINV);
-        li = -u * b.bup[t];
+        li = -u * bu_t;
     }
     if (sd != 0.0) {
         dn = sd / (b.aout[t] + MINV);
-        lo2 = -dn * b.bdn[t];
+        lo2 = -dn * bd_t;
     }
     iu[t] = u;
     id[t] = dn;
@@ -333,41 +338,8 @@ __global__ void k_adj_limit3(Geo g, AdjBufs b, const double* __restrict__ iu,
     l1[t] += s1 + acc;
 }
 
-// antidiffusive velocities transposed, part 1: per face the coefficients of its ten
-// cells times lam_A: kp, km, ku1, kd1, ku2, kd2 (coef[fam*6 + role] arrays)
-__global__ void k_adj_anti1(Geo g, AdjBufs b, const double* __restrict__ lA0,
-                            const double* __restrict__ lA1, const double* __restrict__ lA2,
-                            double* const* __restrict__ coef)
-{
-    size_t t = (size_t)blockIdx.x * TPB + threadIdx.x;
-    if (t >= g.n) return;
-    int i, j, k;
-    g.ijk(t, i, j, k);
-    const double* lA[3] = {lA0, lA1, lA2};
-    const double* c = b.c1;
-    for (int fam = 0; fam < 3; fam++) {
-        FaceSt f = face_st(g, b.U, b.V, b.W, i, j, k, fam);
-        double L = lA[fam][t];
-        double p = c[f.p], m = c[f.m];
-        double K = ab(f.a) * (1.0 - ab(f.a));
-        double den = p + m + MINV;
-        double n1 = c[f.up1[0]] + c[f.up1[1]] - c[f.dn1[0]] - c[f.dn1[1]];
-        double D1 = c[f.up1[0]] + c[f.up1[1]] + c[f.dn1[0]] + c[f.dn1[1]] + MINV;
-        double n2 = c[f.up2[0]] + c[f.up2[1]] - c[f.dn2[0]] - c[f.dn2[1]];
-        double D2 = c[f.up2[0]] + c[f.up2[1]] + c[f.dn2[0]] + c[f.dn2[1]] + MINV;
-        double gb = -0.5 * f.a;
-        double* o = nullptr;
-        o = coef[fam * 6 + 0]; o[t] = L * (K * (((2.0 * m + MINV) / den) / den));
-        o = coef[fam * 6 + 1]; o[t] = L * (-K * (((2.0 * p + MINV) / den) / den));
-        o = coef[fam * 6 + 2]; o[t] = L * (gb * f.s1 * (((D1 - n1) / D1) / D1));
-        o = coef[fam * 6 + 3]; o[t] = L * (gb * f.s1 * (((-D1 - n1) / D1) / D1));
-        o = coef[fam * 6 + 4]; o[t] = L * (gb * f.s2 * (((D2 - n2) / D2) / D2));
-        o = coef[fam * 6 + 5]; o[t] = L * (gb * f.s2 * (((-D2 - n2) / D2) / D2));
-    }
-}
-
-// Scatter variant of parts 1 + 2: each face adds its ten contributions to l1 with
-// hardware FP64 atomics (no coefficient arrays; summation order, and so the last bits,
+// Scatter variant of the transposed antidiffusive stage: each face adds its ten
+// contributions to l1 with hardware FP64 atomics (summation order, and so the last bits,
 // vary between runs).
 __global__ void k_adj_anti_atomic(Geo g, AdjBufs b, const double* __restrict__ lA0,
                                   const double* __restrict__ lA1, const double* __restrict__ lA2,
@@ -410,41 +382,652 @@ __global__ void k_adj_anti_atomic(Geo g, AdjBufs b, const double* __restrict__ l
     }
 }
 
-// part 2: every cell gathers the coefficients of the faces whose stencil contains it
-// (the stencils of face_st read backwards)
-__global__ void k_adj_anti2(Geo g, double* const* __restrict__ coef, double* __restrict__ l1)
+// ------------------------------ corrective donor pass + limiter transposed, tiled ---
+//
+// k_adj_donor_limit followed by k_adj_limit3 in one LDS-tiled pass (with the limiter).
+// Iteration p: publish the plane-p inputs -> barrier -> stage 1 on the tile and ring 1 of
+// plane p (the donor gather, lam_Al of the six faces, the ratio adjoints iu, id, lain,
+// laout) -> barrier -> stage 2 on the tile of plane k = p - 1 (the bound gather over the
+// 7-point neighbourhood and the flux-sum adjoints of the six faces).  LDS slots: lout,
+// bup, bdn one plane (read in stage 1 only), c1, A0, A1 three (stage 1 of plane p, stage
+// 2 of plane p - 1), the stage-1 results and the codes two.
+struct LimArgs {
+    int nx, ny, nz;
+    long long np;
+    int tiles_x, tiles_y, zc;
+    const double *lout, *c1, *A0, *A1, *A2, *bup, *bdn, *ain, *aout;
+    const int* code;
+    double *l1, *lA0, *lA1, *lA2, *lin, *lq;
+};
+
+// own-column values of one region cell: planes p-1, p, p+1 of lout, c1, the ratios; the
+// z-face velocity below (p-1) and above (p); plane p of the rest; n*: the next plane
+struct LimCol {
+    double lo[3], c1[3], bu[3], bd[3], a2m, a2c, a0, a1, ain, aout;
+    int cd;
+    double nlo, nc1, nbu, nbd, na0, na1, na2, nain, naout;
+    int ncd;
+};
+
+__device__ __forceinline__ int ld_code(const int* plane, unsigned colb)
 {
-    size_t t = (size_t)blockIdx.x * TPB + threadIdx.x;
-    if (t >= g.n) return;
-    int i, j, k;
-    g.ijk(t, i, j, k);
-    const double *const *C = coef;
-    auto K = [&](int fam, int role, int di, int dj, int dk) {
-        return C[fam * 6 + role][g.at(i + di, j + dj, k + dk)];
+    return *(const int*)((const char*)plane + colb / 2);
+}
+
+template <int TX, int TY>
+__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(2))) k_adj_lim_t(LimArgs a)
+{
+    using G = tile::Geo<TX, TY>;
+    using tile::ld;
+    using tile::st;
+    constexpr int RX = G::RX, R = G::R;
+    __shared__ double s_lo[R], s_bu[R], s_bd[R], s_c[3][R], s_a0[3][R], s_a1[3][R];
+    __shared__ double s_iu[2][R], s_id[2][R], s_li[2][R], s_lx[2][R];
+    __shared__ int s_cd[2][R];
+
+    const int t = threadIdx.x;
+    const int item = blockIdx.x;
+    const int bx = item % a.tiles_x, by = (item / a.tiles_x) % a.tiles_y;
+    const int bz = item / (a.tiles_x * a.tiles_y);
+    const int rx = 2 + t % TX, ry = 2 + t / TX;
+    const int L = ry * RX + rx;
+    const unsigned colb = tile::column<TX, TY>(a.nx, a.ny, bx, by, rx, ry);
+    const bool store = bx * TX + rx - 2 < a.nx && by * TY + ry - 2 < a.ny;
+    const bool hl = t < G::NH, h1 = t < G::H1;
+    int hx = 0, hy = 0;
+    if (hl) G::halo(t, hx, hy);
+    const int HL = hy * RX + hx;
+    const unsigned hcolb = tile::column<TX, TY>(a.nx, a.ny, bx, by, hx, hy);
+    const int z0 = bz * a.zc;
+    const int z1 = min(z0 + a.zc, a.nz);
+    // planes p + 2, p + 1, p, p - 1, p - 2 of iteration p
+    tile::ZCursor zp2(z0 + 1, a.nz, a.np), zp1(z0, a.nz, a.np), zp0(z0 - 1, a.nz, a.np);
+    tile::ZCursor zm1(z0 - 2, a.nz, a.np);
+
+    // loads of a column: everything for a tile or ring-1 cell, the in-plane values for a
+    // ring-2 cell
+    auto load_next = [&](LimCol& q, unsigned cb, bool full, const tile::ZCursor& zn,
+                         const tile::ZCursor& zc1) {
+        q.nlo = ld(a.lout + zn.off, cb);
+        q.nc1 = ld(a.c1 + zn.off, cb);
+        q.nbu = ld(a.bup + zn.off, cb);
+        q.nbd = ld(a.bdn + zn.off, cb);
+        q.na0 = ld(a.A0 + zc1.off, cb);
+        q.na1 = ld(a.A1 + zc1.off, cb);
+        if (full) {
+            q.na2 = ld(a.A2 + zc1.off, cb);
+            q.nain = ld(a.ain + zc1.off, cb);
+            q.naout = ld(a.aout + zc1.off, cb);
+            q.ncd = ld_code(a.code + zc1.off, cb);
+        }
     };
-    double s = 0.0;
-    // x-faces (i+1/2): p=(i+1), m=(i); up1 (i+1,j+1),(i,j+1); dn1 (i,j-1),(i+1,j-1);
-    // up2 (i+1,k+1),(i,k+1); dn2 (i,k-1),(i+1,k-1)
-    s += K(0, 0, -1, 0, 0) + K(0, 1, 0, 0, 0);
-    s += K(0, 2, -1, -1, 0) + K(0, 2, 0, -1, 0);
-    s += K(0, 3, 0, 1, 0) + K(0, 3, -1, 1, 0);
-    s += K(0, 4, -1, 0, -1) + K(0, 4, 0, 0, -1);
-    s += K(0, 5, 0, 0, 1) + K(0, 5, -1, 0, 1);
-    // y-faces (j+1/2): p=(j+1), m=(j); up1 (i+1,j+1),(i+1,j); dn1 (i-1,j),(i-1,j+1);
-    // up2 (j+1,k+1),(j,k+1); dn2 (j,k-1),(j+1,k-1)
-    s += K(1, 0, 0, -1, 0) + K(1, 1, 0, 0, 0);
-    s += K(1, 2, -1, -1, 0) + K(1, 2, -1, 0, 0);
-    s += K(1, 3, 1, 0, 0) + K(1, 3, 1, -1, 0);
-    s += K(1, 4, 0, -1, -1) + K(1, 4, 0, 0, -1);
-    s += K(1, 5, 0, 0, 1) + K(1, 5, 0, -1, 1);
-    // z-faces (k+1/2): p=(k+1), m=(k); up1 (i+1,k+1),(i+1,k); dn1 (i-1,k),(i-1,k+1);
-    // up2 (j+1,k+1),(j+1,k); dn2 (j-1,k),(j-1,k+1)
-    s += K(2, 0, 0, 0, -1) + K(2, 1, 0, 0, 0);
-    s += K(2, 2, -1, 0, -1) + K(2, 2, -1, 0, 0);
-    s += K(2, 3, 1, 0, 0) + K(2, 3, 1, 0, -1);
-    s += K(2, 4, 0, -1, -1) + K(2, 4, 0, -1, 0);
-    s += K(2, 5, 0, 1, 0) + K(2, 5, 0, 1, -1);
-    l1[t] += s;
+    auto init = [&](LimCol& q, unsigned cb, bool full) {
+        q.lo[0] = ld(a.lout + zm1.off, cb);
+        q.lo[1] = ld(a.lout + zp0.off, cb);
+        q.c1[0] = ld(a.c1 + zm1.off, cb);
+        q.c1[1] = ld(a.c1 + zp0.off, cb);
+        q.bu[0] = ld(a.bup + zm1.off, cb);
+        q.bu[1] = ld(a.bup + zp0.off, cb);
+        q.bd[0] = ld(a.bdn + zm1.off, cb);
+        q.bd[1] = ld(a.bdn + zp0.off, cb);
+        q.a0 = ld(a.A0 + zp0.off, cb);
+        q.a1 = ld(a.A1 + zp0.off, cb);
+        q.a2m = q.a2c = q.ain = q.aout = 0.0;
+        q.cd = 0;
+        if (full) {
+            q.a2m = ld(a.A2 + zm1.off, cb);
+            q.a2c = ld(a.A2 + zp0.off, cb);
+            q.ain = ld(a.ain + zp0.off, cb);
+            q.aout = ld(a.aout + zp0.off, cb);
+            q.cd = ld_code(a.code + zp0.off, cb);
+        }
+        // plane p + 1 of the first iteration, then the prefetch of the second
+        q.lo[2] = ld(a.lout + zp1.off, cb);
+        q.c1[2] = ld(a.c1 + zp1.off, cb);
+        q.bu[2] = ld(a.bup + zp1.off, cb);
+        q.bd[2] = ld(a.bdn + zp1.off, cb);
+        load_next(q, cb, full, zp2, zp1);
+    };
+    auto rotate = [&](LimCol& q) {
+        q.lo[0] = q.lo[1];
+        q.lo[1] = q.lo[2];
+        q.lo[2] = q.nlo;
+        q.c1[0] = q.c1[1];
+        q.c1[1] = q.c1[2];
+        q.c1[2] = q.nc1;
+        q.bu[0] = q.bu[1];
+        q.bu[1] = q.bu[2];
+        q.bu[2] = q.nbu;
+        q.bd[0] = q.bd[1];
+        q.bd[1] = q.bd[2];
+        q.bd[2] = q.nbd;
+        q.a2m = q.a2c;
+        q.a2c = q.na2;
+        q.a0 = q.na0;
+        q.a1 = q.na1;
+        q.ain = q.nain;
+        q.aout = q.naout;
+        q.cd = q.ncd;
+    };
+    LimCol P, H;
+    init(P, colb, true);
+    if (hl) init(H, hcolb, h1);
+
+    // tile cell: stage-1 results of planes p-2, p-1 (index 0, 1; 2 = plane p), the codes,
+    // the carried donor sum and lam_Al * m of plane p-1, c1 and A2 of plane p-2
+    double siu[3] = {0, 0, 0}, sid[3] = {0, 0, 0}, sli[3] = {0, 0, 0}, slx[3] = {0, 0, 0};
+    int scd[3] = {0, 0, 0};
+    double dacc_m = 0.0, lam_m[3] = {0, 0, 0}, c1_m2 = 0.0, a2_m2 = 0.0;
+
+    for (int pl = z0 - 1; pl <= z1; pl++) {
+        const int p = __builtin_amdgcn_readfirstlane(pl);
+        const int s3 = (p + 3) % 3, k3 = (p + 2) % 3;  // slots of planes p, p-1 (c1, A0, A1)
+        const int s2 = (p + 2) % 2, k2 = (p + 1) % 2;  // slots of planes p, p-1 (stage 1)
+        s_lo[L] = P.lo[1];
+        s_bu[L] = P.bu[1];
+        s_bd[L] = P.bd[1];
+        s_c[s3][L] = P.c1[1];
+        s_a0[s3][L] = P.a0;
+        s_a1[s3][L] = P.a1;
+        s_cd[s2][L] = P.cd;
+        if (hl) {
+            s_lo[HL] = H.lo[1];
+            s_bu[HL] = H.bu[1];
+            s_bd[HL] = H.bd[1];
+            s_c[s3][HL] = H.c1[1];
+            s_a0[s3][HL] = H.a0;
+            s_a1[s3][HL] = H.a1;
+            s_cd[s2][HL] = H.cd;
+        }
+        tile::lds_barrier();
+        // stage 1 on cell M of plane p (the merged k_adj_donor_limit, same operation order)
+        auto stage1 = [&](int M, const LimCol& q, double& dacc, double lam[3], double& iu,
+                          double& id, double& li, double& lx) {
+            const double* C = s_c[s3];
+            const double lt = q.lo[1], ct = q.c1[1], bu_t = q.bu[1], bd_t = q.bd[1];
+            double acc = lt, su = 0.0, sd = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                double gl, gr, gv;
+                {  // upper face (M, r)
+                    double a_, lo_r, c_r, bu_r, bd_r;
+                    if (d == 0) {
+                        a_ = q.a0; lo_r = s_lo[M + 1]; c_r = C[M + 1]; bu_r = s_bu[M + 1]; bd_r = s_bd[M + 1];
+                    } else if (d == 1) {
+                        a_ = q.a1; lo_r = s_lo[M + RX]; c_r = C[M + RX]; bu_r = s_bu[M + RX]; bd_r = s_bd[M + RX];
+                    } else {
+                        a_ = q.a2c; lo_r = q.lo[2]; c_r = q.c1[2]; bu_r = q.bu[2]; bd_r = q.bd[2];
+                    }
+                    double m = 1.0;
+                    if (a_ > 0) {
+                        const int w = pick(bd_t, bu_r);
+                        m = w == 0 ? 1.0 : (w == 1 ? bd_t : bu_r);
+                    } else if (a_ < 0) {
+                        const int w = pick(bu_t, bd_r);
+                        m = w == 0 ? 1.0 : (w == 1 ? bu_t : bd_r);
+                    } else {
+                        m = 0.0;
+                    }
+                    const double au = limited(a_, bu_t, bd_t, bu_r, bd_r);
+                    fpart(ct, c_r, au, gl, gr, gv);
+                    const double lf = lo_r - lt;
+                    acc += gl * lf;
+                    const double Lf = gv * lf;
+                    if (a_ > 0 && pick(bd_t, bu_r) == 1) sd += a_ * Lf;
+                    if (a_ < 0 && pick(bu_t, bd_r) == 1) su += a_ * Lf;
+                    lam[d] = m * Lf;
+                }
+                {  // lower face (l, M)
+                    double a_, lo_l, c_l, bu_l, bd_l;
+                    if (d == 0) {
+                        a_ = s_a0[s3][M - 1]; lo_l = s_lo[M - 1]; c_l = C[M - 1]; bu_l = s_bu[M - 1]; bd_l = s_bd[M - 1];
+                    } else if (d == 1) {
+                        a_ = s_a1[s3][M - RX]; lo_l = s_lo[M - RX]; c_l = C[M - RX]; bu_l = s_bu[M - RX]; bd_l = s_bd[M - RX];
+                    } else {
+                        a_ = q.a2m; lo_l = q.lo[0]; c_l = q.c1[0]; bu_l = q.bu[0]; bd_l = q.bd[0];
+                    }
+                    const double al = limited(a_, bu_l, bd_l, bu_t, bd_t);
+                    fpart(c_l, ct, al, gl, gr, gv);
+                    const double lf = lt - lo_l;
+                    acc += gr * lf;
+                    const double Lf = gv * lf;
+                    if (a_ > 0 && pick(bd_l, bu_t) == 2) su += a_ * Lf;
+                    if (a_ < 0 && pick(bu_l, bd_t) == 2) sd += a_ * Lf;
+                }
+            }
+            dacc = acc;
+            iu = id = li = lx = 0.0;
+            if (su != 0.0) {
+                iu = su / (q.ain + MINV);
+                li = -iu * bu_t;
+            }
+            if (sd != 0.0) {
+                id = sd / (q.aout + MINV);
+                lx = -id * bd_t;
+            }
+        };
+        double dacc, lam[3];
+        stage1(L, P, dacc, lam, siu[2], sid[2], sli[2], slx[2]);
+        scd[2] = P.cd;
+        s_iu[s2][L] = siu[2];
+        s_id[s2][L] = sid[2];
+        s_li[s2][L] = sli[2];
+        s_lx[s2][L] = slx[2];
+        if (store && p >= z0 && p < z1 && a.lq) st(a.lq + zp0.off, colb, ld(a.lq + zp0.off, colb) + P.lo[1]);
+        if (h1) {
+            double hd, hl3[3], hiu, hid, hli, hlx;
+            stage1(HL, H, hd, hl3, hiu, hid, hli, hlx);
+            s_iu[s2][HL] = hiu;
+            s_id[s2][HL] = hid;
+            s_li[s2][HL] = hli;
+            s_lx[s2][HL] = hlx;
+        }
+        tile::lds_barrier();
+        // stage 2 on the tile cell of plane k = p - 1 (k_adj_limit3, same operation order)
+        const int k = p - 1;
+        if (k >= z0 && k < z1) {
+            const double* C = s_c[k3];
+            const double *IU = s_iu[k2], *ID = s_id[k2], *LI = s_li[k2], *LX = s_lx[k2];
+            const int* CD = s_cd[k2];
+            const double ct = P.c1[0];  // c1 of plane k
+            // neighbour y, the code q under which y sees t, whether y is t itself (extents
+            // of 1 wrap a neighbour onto the cell)
+            double s0 = 0.0, s1 = 0.0;
+            const bool fx = a.nx == 1, fy = a.ny == 1, fz = a.nz == 1;
+            auto bound = [&](int cq, double iuy, double idy, int q, bool folds) {
+                const int hc = cq & 0xff, lc = (cq >> 8) & 0xff;
+                if (hc >> 1 == q && !(folds && (hc & 1))) {
+                    if (hc & 1) s1 += iuy; else s0 += iuy;
+                }
+                if (lc >> 1 == q && !(folds && (lc & 1))) {
+                    if (lc & 1) s1 -= idy; else s0 -= idy;
+                }
+            };
+            bound(scd[1], siu[1], sid[1], 0, true);
+            bound(CD[L + 1], IU[L + 1], ID[L + 1], 1, fx);
+            bound(CD[L - 1], IU[L - 1], ID[L - 1], 2, fx);
+            bound(CD[L + RX], IU[L + RX], ID[L + RX], 3, fy);
+            bound(CD[L - RX], IU[L - RX], ID[L - RX], 4, fy);
+            bound(scd[2], siu[2], sid[2], 5, fz);
+            bound(scd[0], siu[0], sid[0], 6, fz);
+            {
+                const int c = scd[1], hc = c & 0xff, lc = (c >> 8) & 0xff;
+                auto self = [&](int q) {
+                    return q == 0 || (q <= 2 && fx) || ((q == 3 || q == 4) && fy) || (q >= 5 && fz);
+                };
+                const bool hself = self(hc >> 1) && (hc & 1), lself = self(lc >> 1) && (lc & 1);
+                if (!hself) s1 -= siu[1];
+                if (!lself) s1 += sid[1];
+            }
+            // flux sums of the six faces
+            const double li_t = sli[1], lx_t = slx[1];
+            double acc = 0.0, lAf[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                double gl, gr, gv;
+                {  // upper face (t, r)
+                    double a_, c_r, li_r, lx_r;
+                    if (d == 0) {
+                        a_ = s_a0[k3][L]; c_r = C[L + 1]; li_r = LI[L + 1]; lx_r = LX[L + 1];
+                    } else if (d == 1) {
+                        a_ = s_a1[k3][L]; c_r = C[L + RX]; li_r = LI[L + RX]; lx_r = LX[L + RX];
+                    } else {
+                        a_ = P.a2m; c_r = P.c1[1]; li_r = sli[2]; lx_r = slx[2];
+                    }
+                    const double f = 0.5 * F2(ct, c_r, a_);
+                    const double lf = (f > 0 ? lx_t : 0.0) - (f < 0 ? li_t : 0.0) +
+                                      (f > 0 ? li_r : 0.0) - (f < 0 ? lx_r : 0.0);
+                    fpart(ct, c_r, a_, gl, gr, gv);
+                    acc += gl * lf;
+                    lAf[d] = lam_m[d] + gv * lf;
+                }
+                {  // lower face (l, t)
+                    double a_, c_l, li_l, lx_l;
+                    if (d == 0) {
+                        a_ = s_a0[k3][L - 1]; c_l = C[L - 1]; li_l = LI[L - 1]; lx_l = LX[L - 1];
+                    } else if (d == 1) {
+                        a_ = s_a1[k3][L - RX]; c_l = C[L - RX]; li_l = LI[L - RX]; lx_l = LX[L - RX];
+                    } else {
+                        a_ = a2_m2; c_l = c1_m2; li_l = sli[0]; lx_l = slx[0];
+                    }
+                    const double f = 0.5 * F2(c_l, ct, a_);
+                    const double lf = (f > 0 ? lx_l : 0.0) - (f < 0 ? li_l : 0.0) +
+                                      (f > 0 ? li_t : 0.0) - (f < 0 ? lx_t : 0.0);
+                    fpart(c_l, ct, a_, gl, gr, gv);
+                    acc += gr * lf;
+                }
+            }
+            if (store) {
+                st(a.lin + zm1.off, colb, s0);
+                st(a.l1 + zm1.off, colb, dacc_m + (s1 + acc));
+                st(a.lA0 + zm1.off, colb, lAf[0]);
+                st(a.lA1 + zm1.off, colb, lAf[1]);
+                st(a.lA2 + zm1.off, colb, lAf[2]);
+            }
+        }
+        // shift the tile cell's history by one plane
+        siu[0] = siu[1]; siu[1] = siu[2];
+        sid[0] = sid[1]; sid[1] = sid[2];
+        sli[0] = sli[1]; sli[1] = sli[2];
+        slx[0] = slx[1]; slx[1] = slx[2];
+        scd[0] = scd[1]; scd[1] = scd[2];
+        dacc_m = dacc;
+        lam_m[0] = lam[0];
+        lam_m[1] = lam[1];
+        lam_m[2] = lam[2];
+        c1_m2 = P.c1[0];
+        a2_m2 = P.a2m;
+        rotate(P);
+        if (hl) rotate(H);
+        zp2.next();
+        zp1.next();
+        zp0.next();
+        zm1.next();
+        if (p + 2 <= z1) {  // for the rotation after the next iteration: fields at p + 3,
+                            // faces at p + 2 (the cursors now stand one plane higher)
+            load_next(P, colb, true, zp2, zp1);
+            if (hl) load_next(H, hcolb, h1, zp2, zp1);
+        }
+    }
+}
+
+int lim_tiled(const mpd_grid_s* g, const double* const in[9], const int* code,
+              const double* lout, double* l1, double* const lA[3], double* lin, double* lq,
+              hipStream_t s)
+{
+    constexpr int TX = 32, TY = 8;
+    LimArgs a{};
+    a.nx = g->nx;
+    a.ny = g->ny;
+    a.nz = g->nz;
+    a.np = (long long)g->nx * g->ny;
+    a.tiles_x = (g->nx + TX - 1) / TX;
+    a.tiles_y = (g->ny + TY - 1) / TY;
+    const long long cols = (long long)a.tiles_x * a.tiles_y;
+    int zc = g->nz;
+    while (zc > 8 && cols * ((g->nz + zc - 1) / zc) < 2048) zc = (zc + 1) / 2;
+    a.zc = zc;
+    const int nchunks = (g->nz + zc - 1) / zc;
+    a.lout = lout;
+    a.c1 = in[0];
+    a.A0 = in[1];
+    a.A1 = in[2];
+    a.A2 = in[3];
+    a.bup = in[4];
+    a.bdn = in[5];
+    a.ain = in[6];
+    a.aout = in[7];
+    a.code = code;
+    a.l1 = l1;
+    a.lA0 = lA[0];
+    a.lA1 = lA[1];
+    a.lA2 = lA[2];
+    a.lin = lin;
+    a.lq = lq;
+    hipLaunchKernelGGL((k_adj_lim_t<TX, TY>), dim3((unsigned)(cols * nchunks)), dim3(TX * TY), 0,
+                       s, a);
+    MPD_HIP(hipGetLastError());
+    return MPD_OK;
+}
+
+// ---------------------------------------- antidiffusive velocities transposed, tiled ---
+//
+// For every face the derivatives of its antidiffusive velocity (uu.c:18-32 pattern) with
+// respect to its ten cell values, times the face adjoint L:
+//   kp, km     the cells above / below the face:  +-K (2 m|p + MINV) / den^2,
+//              K = |a| (1 - |a|), den = p + m + MINV
+//   ku1, kd1   the two upper / two lower cells of the first transverse ratio n1/D1:
+//              -a/2 s1 (+-D1 - n1) / D1^2;  ku2, kd2 likewise for the second ratio.
+// Each cell then sums the coefficients of the faces whose stencils contain it (the
+// stencils of face_st read backwards: 30 terms from the faces of planes k-1, k, k+1).
+struct FaceK {
+    double kp, km, ku1, kd1, ku2, kd2;
+};
+
+__device__ __forceinline__ FaceK face_k(double L, double a, double p, double m, double s1,
+                                        double u1a, double u1b, double d1a, double d1b,
+                                        double s2, double u2a, double u2b, double d2a,
+                                        double d2b)
+{
+    FaceK k;
+    const double K = ab(a) * (1.0 - ab(a));
+    const double r0 = 1.0 / (p + m + MINV);  // (x r0) r0: no underflow of den^2
+    k.kp = L * (K * (((2.0 * m + MINV) * r0) * r0));
+    k.km = L * (-K * (((2.0 * p + MINV) * r0) * r0));
+    const double gb = -0.5 * a;
+    const double n1 = u1a + u1b - d1a - d1b, D1 = u1a + u1b + d1a + d1b + MINV;
+    const double r1 = 1.0 / D1;
+    k.ku1 = L * (gb * s1 * (((D1 - n1) * r1) * r1));
+    k.kd1 = L * (gb * s1 * (((-D1 - n1) * r1) * r1));
+    const double n2 = u2a + u2b - d2a - d2b, D2 = u2a + u2b + d2a + d2b + MINV;
+    const double r2 = 1.0 / D2;
+    k.ku2 = L * (gb * s2 * (((D2 - n2) * r2) * r2));
+    k.kd2 = L * (gb * s2 * (((-D2 - n2) * r2) * r2));
+    return k;
+}
+
+struct AntiArgs {
+    int nx, ny, nz;
+    long long np;
+    int tiles_x, tiles_y, zc;
+    const double *c1, *U, *V, *W, *lA0, *lA1, *lA2;
+    double* l1;  // the coefficient sums are added to it on every cell
+};
+
+// One workgroup per (tile, z-chunk), 2.5-D streaming as the forward kernels (mpd_tile.hpp):
+// each thread owns a tile column and at most one halo column.  Iteration p: publish c1
+// (p+1), U, V (p+1), W (p) -> barrier -> coefficients of the faces of plane p on the tile
+// and ring 1; the 14 that other columns gather go to LDS -> barrier -> each tile cell
+// adds the terms of plane-p faces to its running sums for planes p-1, p, p+1; the sum
+// of plane p-1 is then complete.  LDS slots: c1 three planes, U, V, W two, the
+// coefficients one (written after the first barrier of an iteration, read before the
+// first barrier of the next).
+template <int TX, int TY>
+__global__ void __launch_bounds__(TX * TY) k_adj_anti_t(AntiArgs a)
+{
+    using G = tile::Geo<TX, TY>;
+    using tile::ld;
+    using tile::st;
+    constexpr int RX = G::RX, R = G::R;
+    __shared__ double s_c[3][R], s_u[2][R], s_v[2][R], s_w[2][R];
+    __shared__ double s_k[14][R];
+
+    const int t = threadIdx.x;
+    const int item = blockIdx.x;
+    const int bx = item % a.tiles_x, by = (item / a.tiles_x) % a.tiles_y;
+    const int bz = item / (a.tiles_x * a.tiles_y);
+    const int rx = 2 + t % TX, ry = 2 + t / TX;
+    const int L = ry * RX + rx;
+    const unsigned colb = tile::column<TX, TY>(a.nx, a.ny, bx, by, rx, ry);
+    const bool store = bx * TX + rx - 2 < a.nx && by * TY + ry - 2 < a.ny;
+    const bool hl = t < G::NH, h1 = t < G::H1;
+    int hx = 0, hy = 0;
+    if (hl) G::halo(t, hx, hy);
+    const int HL = hy * RX + hx;
+    const unsigned hcolb = tile::column<TX, TY>(a.nx, a.ny, bx, by, hx, hy);
+    const int z0 = bz * a.zc;
+    const int z1 = min(z0 + a.zc, a.nz);
+    // planes p + 2, p + 1, p, p - 1 of iteration p
+    tile::ZCursor zp2(z0 + 1, a.nz, a.np), zp1(z0, a.nz, a.np), zp0(z0 - 1, a.nz, a.np);
+    tile::ZCursor zm1(z0 - 2, a.nz, a.np);
+
+    // planes before the first iteration: c1 (z0-2, z0-1), U, V (z0-1), W (z0-2)
+    {
+        const int cs0 = (z0 + 1) % 3, cs1 = (z0 + 2) % 3, us = (z0 + 1) % 2, ws = z0 % 2;
+        s_c[cs0][L] = ld(a.c1 + zm1.off, colb);
+        s_c[cs1][L] = ld(a.c1 + zp0.off, colb);
+        s_u[us][L] = ld(a.U + zp0.off, colb);
+        s_v[us][L] = ld(a.V + zp0.off, colb);
+        s_w[ws][L] = ld(a.W + zm1.off, colb);
+        if (hl) {
+            s_c[cs0][HL] = ld(a.c1 + zm1.off, hcolb);
+            s_c[cs1][HL] = ld(a.c1 + zp0.off, hcolb);
+            s_u[us][HL] = ld(a.U + zp0.off, hcolb);
+            s_v[us][HL] = ld(a.V + zp0.off, hcolb);
+            s_w[ws][HL] = ld(a.W + zm1.off, hcolb);
+        }
+    }
+    // values published / consumed by the next iteration: c1, U, V at p+1, W and the face
+    // adjoints at p (tile and halo column), l1 at p (tile column)
+    double nc = ld(a.c1 + zp1.off, colb), nu = ld(a.U + zp1.off, colb);
+    double nv = ld(a.V + zp1.off, colb), nw = ld(a.W + zp0.off, colb);
+    double nl0 = ld(a.lA0 + zp0.off, colb), nl1 = ld(a.lA1 + zp0.off, colb);
+    double nl2 = ld(a.lA2 + zp0.off, colb), nl1c = ld(a.l1 + zp0.off, colb);
+    double hnc = 0, hnu = 0, hnv = 0, hnw = 0, hnl0 = 0, hnl1 = 0, hnl2 = 0;
+    if (hl) {
+        hnc = ld(a.c1 + zp1.off, hcolb);
+        hnu = ld(a.U + zp1.off, hcolb);
+        hnv = ld(a.V + zp1.off, hcolb);
+        hnw = ld(a.W + zp0.off, hcolb);
+        if (h1) {
+            hnl0 = ld(a.lA0 + zp0.off, hcolb);
+            hnl1 = ld(a.lA1 + zp0.off, hcolb);
+            hnl2 = ld(a.lA2 + zp0.off, hcolb);
+        }
+    }
+    double accm = 0.0, accc = 0.0, accn = 0.0, l1m = 0.0;
+    for (int pl = z0 - 1; pl <= z1; pl++) {
+        const int p = __builtin_amdgcn_readfirstlane(pl);
+        const int cm = (p + 2) % 3, cc = (p + 3) % 3, cq = (p + 4) % 3;  // c1 p-1, p, p+1
+        const int up = (p + 2) % 2, uq = (p + 3) % 2;                     // U, V p, p+1
+        const int wm = (p + 1) % 2, wc = (p + 2) % 2;                     // W p-1, p
+        const double la0 = nl0, la1 = nl1, la2 = nl2, hla0 = hnl0, hla1 = hnl1, hla2 = hnl2;
+        const double l1c = nl1c;
+        s_c[cq][L] = nc;
+        s_u[uq][L] = nu;
+        s_v[uq][L] = nv;
+        s_w[wc][L] = nw;
+        if (hl) {
+            s_c[cq][HL] = hnc;
+            s_u[uq][HL] = hnu;
+            s_v[uq][HL] = hnv;
+            s_w[wc][HL] = hnw;
+        }
+        if (p < z1) {
+            nc = ld(a.c1 + zp2.off, colb);
+            nu = ld(a.U + zp2.off, colb);
+            nv = ld(a.V + zp2.off, colb);
+            nw = ld(a.W + zp1.off, colb);
+            nl0 = ld(a.lA0 + zp1.off, colb);
+            nl1 = ld(a.lA1 + zp1.off, colb);
+            nl2 = ld(a.lA2 + zp1.off, colb);
+            nl1c = ld(a.l1 + zp1.off, colb);
+            if (hl) {
+                hnc = ld(a.c1 + zp2.off, hcolb);
+                hnu = ld(a.U + zp2.off, hcolb);
+                hnv = ld(a.V + zp2.off, hcolb);
+                hnw = ld(a.W + zp1.off, hcolb);
+                if (h1) {
+                    hnl0 = ld(a.lA0 + zp1.off, hcolb);
+                    hnl1 = ld(a.lA1 + zp1.off, hcolb);
+                    hnl2 = ld(a.lA2 + zp1.off, hcolb);
+                }
+            }
+        }
+        tile::lds_barrier();
+        // coefficients of the three faces above/right of cell M in plane p
+        auto faces = [&](int M, double lx, double ly, double lz, FaceK& kx, FaceK& ky,
+                         FaceK& kz) {
+            const double *C = s_c[cc], *Cm = s_c[cm], *Cq = s_c[cq];
+            const double *Up = s_u[up], *Uq = s_u[uq], *Vp = s_v[up], *Vq = s_v[uq];
+            const double *Wm = s_w[wm], *Wc = s_w[wc];
+            const double m = C[M];
+            {  // x-face (uu.c:81-87)
+                const double s1 = 0.25 * (Vp[M + 1 - RX] + Vp[M + 1] + Vp[M] + Vp[M - RX]);
+                const double s2 = 0.25 * (Wm[M + 1] + Wc[M + 1] + Wc[M] + Wm[M]);
+                kx = face_k(lx, Up[M], C[M + 1], m, s1, C[M + 1 + RX], C[M + RX], C[M - RX],
+                            C[M + 1 - RX], s2, Cq[M + 1], Cq[M], Cm[M], Cm[M + 1]);
+            }
+            {  // y-face (vv.c:85-92)
+                const double s1 = 0.25 * (Up[M - 1 + RX] + Up[M + RX] + Up[M] + Up[M - 1]);
+                const double s2 = 0.25 * (Wm[M + RX] + Wc[M + RX] + Wc[M] + Wm[M]);
+                ky = face_k(ly, Vp[M], C[M + RX], m, s1, C[M + 1 + RX], C[M + 1], C[M - 1],
+                            C[M - 1 + RX], s2, Cq[M + RX], Cq[M], Cm[M], Cm[M + RX]);
+            }
+            {  // z-face (ww.c:248-255)
+                const double s1 = 0.25 * (Up[M] + Uq[M] + Uq[M - 1] + Up[M - 1]);
+                const double s2 = 0.25 * (Vq[M - RX] + Vq[M] + Vp[M] + Vp[M - RX]);
+                kz = face_k(lz, Wc[M], Cq[M], m, s1, Cq[M + 1], C[M + 1], C[M - 1], Cq[M - 1],
+                            s2, Cq[M + RX], C[M + RX], C[M - RX], Cq[M - RX]);
+            }
+        };
+        auto publish = [&](int M, const FaceK& kx, const FaceK& ky, const FaceK& kz) {
+            s_k[0][M] = kx.kp;
+            s_k[1][M] = kx.ku1;
+            s_k[2][M] = kx.kd1;
+            s_k[3][M] = kx.ku2;
+            s_k[4][M] = kx.kd2;
+            s_k[5][M] = ky.kp;
+            s_k[6][M] = ky.ku1;
+            s_k[7][M] = ky.kd1;
+            s_k[8][M] = ky.ku2;
+            s_k[9][M] = ky.kd2;
+            s_k[10][M] = kz.ku1;
+            s_k[11][M] = kz.kd1;
+            s_k[12][M] = kz.ku2;
+            s_k[13][M] = kz.kd2;
+        };
+        FaceK kx, ky, kz;
+        faces(L, la0, la1, la2, kx, ky, kz);
+        publish(L, kx, ky, kz);
+        if (h1) {
+            FaceK hx_, hy_, hz_;
+            faces(HL, hla0, hla1, hla2, hx_, hy_, hz_);
+            publish(HL, hx_, hy_, hz_);
+        }
+        tile::lds_barrier();
+        // terms of the faces of plane p, by target plane (own-column terms from registers)
+        accm += (kx.kd2 + s_k[4][L - 1]) + (ky.kd2 + s_k[9][L - RX]);
+        accc += kx.km + s_k[0][L - 1] + s_k[1][L - 1 - RX] + s_k[1][L - RX] + s_k[2][L + RX] +
+                s_k[2][L - 1 + RX];
+        accc += ky.km + s_k[5][L - RX] + s_k[6][L - 1 - RX] + s_k[6][L - 1] + s_k[7][L + 1] +
+                s_k[7][L + 1 - RX];
+        const double zu1 = s_k[10][L - 1], zd1 = s_k[11][L + 1], zu2 = s_k[12][L - RX];
+        const double zd2 = s_k[13][L + RX];
+        accc += kz.km + zu1 + zd1 + zu2 + zd2;
+        accn += (kx.ku2 + s_k[3][L - 1]) + (ky.ku2 + s_k[8][L - RX]) + kz.kp + zu1 + zd1 + zu2 +
+                zd2;
+        if (store && p - 1 >= z0 && p - 1 < z1) st(a.l1 + zm1.off, colb, l1m + accm);
+        accm = accc;
+        accc = accn;
+        accn = 0.0;
+        l1m = l1c;
+        zp2.next();
+        zp1.next();
+        zp0.next();
+        zm1.next();
+    }
+}
+
+int anti_tiled(const mpd_grid_s* g, const double* c1, double* const lA[3], double* l1,
+               hipStream_t s)
+{
+    constexpr int TX = 32, TY = 8;
+    AntiArgs a{};
+    a.nx = g->nx;
+    a.ny = g->ny;
+    a.nz = g->nz;
+    a.np = (long long)g->nx * g->ny;
+    a.tiles_x = (g->nx + TX - 1) / TX;
+    a.tiles_y = (g->ny + TY - 1) / TY;
+    // z-chunks: enough workgroups to fill the chip a few times over, at least 8 planes
+    const long long cols = (long long)a.tiles_x * a.tiles_y;
+    int zc = g->nz;
+    while (zc > 8 && cols * ((g->nz + zc - 1) / zc) < 2048) zc = (zc + 1) / 2;
+    a.zc = zc;
+    const int nchunks = (g->nz + zc - 1) / zc;
+    a.c1 = c1;
+    a.U = g->face[0].p;
+    a.V = g->face[1].p;
+    a.W = g->face[2].p;
+    a.lA0 = lA[0];
+    a.lA1 = lA[1];
+    a.lA2 = lA[2];
+    a.l1 = l1;
+    hipLaunchKernelGGL((k_adj_anti_t<TX, TY>), dim3((unsigned)(cols * nchunks)), dim3(TX * TY), 0,
+                       s, a);
+    MPD_HIP(hipGetLastError());
+    return MPD_OK;
 }
 
 // donor pass transposed: lin += l1 + gather over the Courant faces
@@ -508,26 +1091,13 @@ struct Fwd {
 
 // Adjoint intermediates of one step.
 struct Work {
-    DevBuf lAl[3], lA[3], iu, id, lain, laout, l1;
-    DevBuf coef[18];
-    double** dcoef = nullptr;
-    ~Work()
-    {
-        if (dcoef) (void)hipFree(dcoef);
-    }
+    DevBuf lA[3], iu, id, lain, laout, l1;
     int alloc(size_t cells)
     {
         int rc;
-        DevBuf* all[] = {&lAl[0], &lAl[1], &lAl[2], &lA[0], &lA[1], &lA[2], &iu, &id, &lain,
-                         &laout, &l1};
+        DevBuf* all[] = {&lA[0], &lA[1], &lA[2], &iu, &id, &lain, &laout, &l1};
         for (DevBuf* d : all)
             if ((rc = d->alloc(cells))) return rc;
-        for (auto& c : coef)
-            if ((rc = c.alloc(cells))) return rc;
-        MPD_HIP(hipMalloc((void**)&dcoef, 18 * sizeof(double*)));
-        double* h[18];
-        for (int q = 0; q < 18; q++) h[q] = coef[q].p;
-        MPD_HIP(hipMemcpy(dcoef, h, sizeof h, hipMemcpyHostToDevice));
         return MPD_OK;
     }
 };
@@ -542,26 +1112,30 @@ int adjoint_step(mpd_grid_s* g, Work& w, const Fwd& f, const double* cin, const 
     const double *U = g->face[0].p, *V = g->face[1].p, *W = g->face[2].p;
     AdjBufs b{U, V, W, cin, f.c1.p, f.A[0].p, f.A[1].p, f.A[2].p, f.bup.p, f.bdn.p,
               f.ain.p, f.aout.p, f.code, g->nonosc};
-    hipLaunchKernelGGL(k_adj_donor2, dim3(nb), dim3(TPB), 0, s, geo, b, lout, w.l1.p, w.lAl[0].p,
-                       w.lAl[1].p, w.lAl[2].p, lq);
-    if (g->nonosc) {
-        hipLaunchKernelGGL(k_adj_limit1, dim3(nb), dim3(TPB), 0, s, geo, b, w.lAl[0].p,
-                           w.lAl[1].p, w.lAl[2].p, w.lA[0].p, w.lA[1].p, w.lA[2].p, w.iu.p,
-                           w.id.p, w.lain.p, w.laout.p);
-        hipLaunchKernelGGL(k_adj_limit3, dim3(nb), dim3(TPB), 0, s, geo, b, w.iu.p, w.id.p,
-                           w.lain.p, w.laout.p, lin, w.l1.p, w.lA[0].p, w.lA[1].p, w.lA[2].p);
+    if (g->nonosc && g->adj_tiled) {
+        const double* in[9] = {f.c1.p, f.A[0].p, f.A[1].p, f.A[2].p, f.bup.p, f.bdn.p, f.ain.p,
+                               f.aout.p, nullptr};
+        double* const lA[3] = {w.lA[0].p, w.lA[1].p, w.lA[2].p};
+        int rc;
+        if ((rc = lim_tiled(g, in, f.code, lout, w.l1.p, lA, lin, lq, s))) return rc;
     } else {
-        for (int d = 0; d < 3; d++)
-            MPD_HIP(hipMemcpyAsync(w.lA[d].p, w.lAl[d].p, g->ncell * 8, hipMemcpyDeviceToDevice, s));
-        MPD_HIP(hipMemsetAsync(lin, 0, g->ncell * 8, s));
+        hipLaunchKernelGGL(k_adj_donor_limit, dim3(nb), dim3(TPB), 0, s, geo, b, lout, w.l1.p,
+                           w.lA[0].p, w.lA[1].p, w.lA[2].p, w.iu.p, w.id.p, w.lain.p,
+                           w.laout.p, lq);
+        if (g->nonosc)
+            hipLaunchKernelGGL(k_adj_limit3, dim3(nb), dim3(TPB), 0, s, geo, b, w.iu.p, w.id.p,
+                               w.lain.p, w.laout.p, lin, w.l1.p, w.lA[0].p, w.lA[1].p,
+                               w.lA[2].p);
+        else
+            MPD_HIP(hipMemsetAsync(lin, 0, g->ncell * 8, s));
     }
     if (g->adj_scatter) {
         hipLaunchKernelGGL(k_adj_anti_atomic, dim3(nb), dim3(TPB), 0, s, geo, b, w.lA[0].p,
                            w.lA[1].p, w.lA[2].p, w.l1.p);
     } else {
-        hipLaunchKernelGGL(k_adj_anti1, dim3(nb), dim3(TPB), 0, s, geo, b, w.lA[0].p,
-                           w.lA[1].p, w.lA[2].p, w.dcoef);
-        hipLaunchKernelGGL(k_adj_anti2, dim3(nb), dim3(TPB), 0, s, geo, w.dcoef, w.l1.p);
+        double* const lA[3] = {w.lA[0].p, w.lA[1].p, w.lA[2].p};
+        int rc;
+        if ((rc = anti_tiled(g, f.c1.p, lA, w.l1.p, s))) return rc;
     }
     hipLaunchKernelGGL(k_adj_donor1, dim3(nb), dim3(TPB), 0, s, geo, b, w.l1.p, lin);
     MPD_HIP(hipGetLastError());
@@ -586,6 +1160,14 @@ extern "C" int mpd_set_adjoint_scatter(mpd_grid g, int atomic)
     clear_error();
     if (!g) return fail(MPD_ERR_ARG, "mpd_set_adjoint_scatter: NULL grid");
     g->adj_scatter = atomic ? 1 : 0;
+    return MPD_OK;
+}
+
+extern "C" int mpd_set_adjoint_tiled(mpd_grid g, int tiled)
+{
+    clear_error();
+    if (!g) return fail(MPD_ERR_ARG, "mpd_set_adjoint_tiled: NULL grid");
+    g->adj_tiled = tiled ? 1 : 0;
     return MPD_OK;
 }
 

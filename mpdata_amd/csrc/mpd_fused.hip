@@ -28,6 +28,7 @@
 #include "mpd_formulas.hpp"
 #include "mpd_fused.hpp"
 #include "mpd_grid.hpp"
+#include "mpd_tile.hpp"
 
 #define MPD_CAT2(a, b) a##_##b
 #define MPD_CAT(a, b) MPD_CAT2(a, b)
@@ -39,92 +40,7 @@ namespace {
 
 
 
-template <int TX, int TY>
-struct Geo {
-    static constexpr int RX = TX + 4, RY = TY + 4, R = RX * RY;
-    static constexpr int NT = TX * TY;                  // threads: one per tile cell
-    static constexpr int H1 = 2 * (TX + 2) + 2 * TY;    // ring-1 cells
-    static constexpr int NH = R - NT;                   // halo cells, rings 1 and 2
-    static_assert(NH <= NT, "every thread owns at most one halo cell");
-    // Region cell (rx, ry) of halo cell h: ring 1 first (the rows above and below the
-    // tile, then the two side columns), then ring 2 in the same order, so that the
-    // threads with ring-1 work are the lowest-numbered and fill as few waves as possible.
-    __device__ static void halo(int h, int& rx, int& ry)
-    {
-        if (h < H1) {
-            if (h < 2 * (TX + 2)) {
-                ry = h < TX + 2 ? 1 : TY + 2;
-                rx = 1 + h % (TX + 2);
-            } else {
-                const int r = h - 2 * (TX + 2);
-                ry = 2 + r / 2;
-                rx = (r & 1) ? TX + 2 : 1;
-            }
-        } else {
-            int r = h - H1;
-            if (r < 2 * (TX + 4)) {
-                ry = r < TX + 4 ? 0 : TY + 3;
-                rx = r % (TX + 4);
-            } else {
-                r -= 2 * (TX + 4);
-                ry = 1 + r / 2;
-                rx = (r & 1) ? TX + 3 : 0;
-            }
-        }
-    }
-};
-
-// Workgroup barrier ordering LDS only.  A full __syncthreads() also fences global
-// memory, which makes every wave wait for its outstanding global loads at the barrier
-// and would defeat the one-plane-ahead prefetch below; these kernels exchange data
-// between threads through LDS only.
-__device__ __forceinline__ void lds_barrier()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
-__device__ __forceinline__ int wrapi(int a, int n)
-{
-    a %= n;
-    return a < 0 ? a + n : a;
-}
-
-// Cursor over the planes of the periodic z axis: the wrapped plane index and the
-// element offset of that plane, advanced one plane at a time without a division.  Both
-// are wave-uniform (they depend on the loop counter and the instance only), so they
-// live in scalar registers and every global access is a uniform plane base plus the
-// thread's 32-bit column offset.
-struct ZCursor {
-    int z, nz;
-    size_t off, np;
-    __device__ ZCursor(int zz, const FusedArgs& a) : z(wrapi(zz, a.nz)), nz(a.nz), np((size_t)a.np)
-    {
-        off = (size_t)z * np;
-    }
-    __device__ void next()
-    {
-        if (++z == nz) {
-            z = 0;
-            off = 0;
-        } else {
-            off += np;
-        }
-    }
-};
-
-// Element of a plane at byte offset colb of the thread's column (a 32-bit offset: the
-// host keeps every plane below 4 GiB), so the access is a scalar base plus a vector
-// offset.
-__device__ __forceinline__ double ld(const double* plane, unsigned colb)
-{
-    return *(const double*)((const char*)plane + colb);
-}
-__device__ __forceinline__ void st(double* plane, unsigned colb, double v)
-{
-    *(double*)((char*)plane + colb) = v;
-}
+using namespace tile;
 
 // tile (bx, by) and z-chunk bz of work item b of an instance
 __device__ __forceinline__ void tile_of(const FusedArgs& a, int b, int& bx, int& by, int& bz)
@@ -172,14 +88,6 @@ __device__ __forceinline__ int item_of(const Work& w, int it)
     return w.remap ? (it % 8) * (w.nitems / 8) + it / 8 : it;
 }
 
-// Byte offset of the column of region cell (cx, cy) of tile (bx, by) (periodic wrap).
-template <int TX, int TY>
-__device__ __forceinline__ unsigned column(const FusedArgs& a, int bx, int by, int cx, int cy)
-{
-    const int gx = wrapi(bx * TX + cx - 2, a.nx), gy = wrapi(by * TY + cy - 2, a.ny);
-    return (unsigned)(gy * a.nx + gx) * 8u;
-}
-
 // F1 over one work item.  Iteration p: publish c^n(p), U(p), V(p) -> barrier -> donor
 // field c1(p) on the tile and ring 1, publish it with W(p-1) -> barrier -> antidiffusive
 // velocities of the faces above/right of plane k = p - 1 on the tile.
@@ -200,19 +108,19 @@ __device__ __forceinline__ void f1_tile(const FusedArgs& a, int item)
     // own tile cell
     const int rx = 2 + t % TX, ry = 2 + t / TX;
     const int L = ry * RX + rx;
-    const unsigned colb = column<TX, TY>(a, bx, by, rx, ry);
+    const unsigned colb = column<TX, TY>(a.nx, a.ny, bx, by, rx, ry);
     const bool store = bx * TX + rx - 2 < a.nx && by * TY + ry - 2 < a.ny;
     // own halo cell: ring 1 (loads, donor) or ring 2 (loads), if any
     const bool hl = t < G::NH, h1 = t < G::H1;
     int hx = 0, hy = 0;
     if (hl) G::halo(t, hx, hy);
     const int HL = hy * RX + hx;
-    const unsigned hcolb = column<TX, TY>(a, bx, by, hx, hy);
+    const unsigned hcolb = column<TX, TY>(a.nx, a.ny, bx, by, hx, hy);
 
     const int z0 = bz * a.zc;
     const int z1 = min(z0 + a.zc, a.nz);
     // planes p + 2, p + 1, p, p - 1 of iteration p
-    ZCursor zp2(z0 + 1, a), zp1(z0, a), zp0(z0 - 1, a), zm1(z0 - 2, a);
+    ZCursor zp2(z0 + 1, a.nz, a.np), zp1(z0, a.nz, a.np), zp0(z0 - 1, a.nz, a.np), zm1(z0 - 2, a.nz, a.np);
 
     // tile cell registers: c^n (p-1, p), c1 (p-2, p-1), U, V (p-1), W (p-2, p-1), the W
     // neighbours (L+1, L+RX) of plane p-2; n*: the next iteration's values, loaded one
@@ -399,17 +307,17 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
     tile_of(a, item, bx, by, bz);
     const int rx = 2 + t % TX, ry = 2 + t / TX;
     const int L = ry * RX + rx;
-    const unsigned colb = column<TX, TY>(a, bx, by, rx, ry);
+    const unsigned colb = column<TX, TY>(a.nx, a.ny, bx, by, rx, ry);
     const bool store = bx * TX + rx - 2 < a.nx && by * TY + ry - 2 < a.ny;
     const bool hl = t < G::NH, h1 = t < G::H1;
     int hx = 0, hy = 0;
     if (hl) G::halo(t, hx, hy);
     const int HL = hy * RX + hx;
-    const unsigned hcolb = column<TX, TY>(a, bx, by, hx, hy);
+    const unsigned hcolb = column<TX, TY>(a.nx, a.ny, bx, by, hx, hy);
 
     const int z0 = bz * a.zc;
     const int z1 = min(z0 + a.zc, a.nz);
-    ZCursor zp2(z0 + 1, a), zp1(z0, a), zp0(z0 - 1, a), zm1(z0 - 2, a);
+    ZCursor zp2(z0 + 1, a.nz, a.np), zp1(z0, a.nz, a.np), zp0(z0 - 1, a.nz, a.np), zm1(z0 - 2, a.nz, a.np);
 
     // tile cell: c^n (p-1, p), cp (p-2 .. p), Az (p-2, p-1), Ax, Ay (p-1) with their
     // lower neighbours (L-1, L-RX), ratios (p-2, p-1); n*: loaded one iteration ahead

@@ -1,0 +1,109 @@
+// Device helpers of the 2.5-D streaming kernels (the fused forward kernels and the
+// tiled adjoint): the tile + halo geometry, the LDS-only barrier, the periodic plane
+// cursor and 32-bit column addressing.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace mpd {
+namespace tile {
+
+template <int TX, int TY>
+struct Geo {
+    static constexpr int RX = TX + 4, RY = TY + 4, R = RX * RY;
+    static constexpr int NT = TX * TY;                  // threads: one per tile cell
+    static constexpr int H1 = 2 * (TX + 2) + 2 * TY;    // ring-1 cells
+    static constexpr int NH = R - NT;                   // halo cells, rings 1 and 2
+    static_assert(NH <= NT, "every thread owns at most one halo cell");
+    // Region cell (rx, ry) of halo cell h: ring 1 first (the rows above and below the
+    // tile, then the two side columns), then ring 2 in the same order, so that the
+    // threads with ring-1 work are the lowest-numbered and fill as few waves as possible.
+    __device__ static void halo(int h, int& rx, int& ry)
+    {
+        if (h < H1) {
+            if (h < 2 * (TX + 2)) {
+                ry = h < TX + 2 ? 1 : TY + 2;
+                rx = 1 + h % (TX + 2);
+            } else {
+                const int r = h - 2 * (TX + 2);
+                ry = 2 + r / 2;
+                rx = (r & 1) ? TX + 2 : 1;
+            }
+        } else {
+            int r = h - H1;
+            if (r < 2 * (TX + 4)) {
+                ry = r < TX + 4 ? 0 : TY + 3;
+                rx = r % (TX + 4);
+            } else {
+                r -= 2 * (TX + 4);
+                ry = 1 + r / 2;
+                rx = (r & 1) ? TX + 3 : 0;
+            }
+        }
+    }
+};
+
+// Workgroup barrier ordering LDS only.  A full __syncthreads() also fences global
+// memory, which makes every wave wait for its outstanding global loads at the barrier
+// and would defeat the one-plane-ahead prefetch below; these kernels exchange data
+// between threads through LDS only.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+__device__ __forceinline__ int wrapi(int a, int n)
+{
+    a %= n;
+    return a < 0 ? a + n : a;
+}
+
+// Cursor over the planes of the periodic z axis: the wrapped plane index and the
+// element offset of that plane, advanced one plane at a time without a division.  Both
+// are wave-uniform (they depend on the loop counter and the instance only), so they
+// live in scalar registers and every global access is a uniform plane base plus the
+// thread's 32-bit column offset.
+struct ZCursor {
+    int z, nz;
+    size_t off, np;
+    __device__ ZCursor(int zz, int nz_, long long np_) : z(wrapi(zz, nz_)), nz(nz_), np((size_t)np_)
+    {
+        off = (size_t)z * np;
+    }
+    __device__ void next()
+    {
+        if (++z == nz) {
+            z = 0;
+            off = 0;
+        } else {
+            off += np;
+        }
+    }
+};
+
+// Element of a plane at byte offset colb of the thread's column (a 32-bit offset: the
+// host keeps every plane below 4 GiB), so the access is a scalar base plus a vector
+// offset.
+__device__ __forceinline__ double ld(const double* plane, unsigned colb)
+{
+    return *(const double*)((const char*)plane + colb);
+}
+__device__ __forceinline__ void st(double* plane, unsigned colb, double v)
+{
+    *(double*)((char*)plane + colb) = v;
+}
+
+// Byte offset of the column of region cell (cx, cy) of tile (bx, by) of an nx x ny plane
+// (periodic wrap).
+template <int TX, int TY>
+__device__ __forceinline__ unsigned column(int nx, int ny, int bx, int by, int cx, int cy)
+{
+    const int gx = wrapi(bx * TX + cx - 2, nx), gy = wrapi(by * TY + cy - 2, ny);
+    return (unsigned)(gy * nx + gx) * 8u;
+}
+
+}  // namespace tile
+}  // namespace mpd

@@ -146,3 +146,23 @@ def test_atomic_scatter_matches_gather():
     assert out[0][0] == out[1][0]
     for a, b in ((out[0][1], out[1][1]), (out[0][2], out[1][2])):
         assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
+
+
+@pytest.mark.parametrize("dims", [(12, 10, 8), (1, 8, 6), (2, 6, 4), (40, 9, 5)])
+def test_tiled_limiter_adjoint_matches_per_cell_passes(dims):
+    nx, ny, nz = dims
+    rng, U, V, W, c = case(nx, ny, nz, 77 + nx)
+    c[rng.uniform(0, 1, c.size) < 0.2] = 0.0  # ties with the bounds
+    lam = rng.standard_normal(c.size)
+    out = []
+    for tiled in (True, False):
+        g = grid(nx, ny, nz, True, U, V, W)
+        g.set_adjoint_tiled(tiled)
+        lq = np.zeros(c.size)
+        out.append((g.adjoint_step(c, lam, lq), lq))
+        g.close()
+    (a, qa), (b, qb) = out
+    assert np.abs(a - b).max() <= 1e-13 * np.abs(b).max()
+    np.testing.assert_array_equal(qa, qb)
+    want = O.adj_step(nx, ny, nz, U, V, W, True, c, lam)
+    assert np.abs(a - want).max() <= 1e-12 * np.abs(want).max()
