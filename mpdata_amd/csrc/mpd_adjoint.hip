@@ -512,12 +512,16 @@ __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(2)
     LimCol P, H;
     init(P, colb, true);
     if (hl) init(H, hcolb, h1);
+    // c1 of plane z0 - 2: stage 1 of the first iteration reads it from LDS (the loop
+    // publishes planes from z0 - 1 on)
+    s_c[(z0 + 1) % 3][L] = P.c1[0];
+    if (hl) s_c[(z0 + 1) % 3][HL] = H.c1[0];
 
     // tile cell: stage-1 results of planes p-2, p-1 (index 0, 1; 2 = plane p), the codes,
     // the carried donor sum and lam_Al * m of plane p-1, c1 and A2 of plane p-2
-    double siu[3] = {0, 0, 0}, sid[3] = {0, 0, 0}, sli[3] = {0, 0, 0}, slx[3] = {0, 0, 0};
-    int scd[3] = {0, 0, 0};
-    double dacc_m = 0.0, lam_m[3] = {0, 0, 0}, c1_m2 = 0.0, a2_m2 = 0.0;
+    double siu0 = 0, sid0 = 0, sli0 = 0, slx0 = 0, dacc_m = 0.0, lam_m[3] = {0, 0, 0};
+    double a2_m2 = 0.0;
+    int scd0 = 0;
 
     for (int pl = z0 - 1; pl <= z1; pl++) {
         const int p = __builtin_amdgcn_readfirstlane(pl);
@@ -584,7 +588,7 @@ __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(2)
                     } else if (d == 1) {
                         a_ = s_a1[s3][M - RX]; lo_l = s_lo[M - RX]; c_l = C[M - RX]; bu_l = s_bu[M - RX]; bd_l = s_bd[M - RX];
                     } else {
-                        a_ = q.a2m; lo_l = q.lo[0]; c_l = q.c1[0]; bu_l = q.bu[0]; bd_l = q.bd[0];
+                        a_ = q.a2m; lo_l = q.lo[0]; c_l = s_c[k3][M]; bu_l = q.bu[0]; bd_l = q.bd[0];
                     }
                     const double al = limited(a_, bu_l, bd_l, bu_t, bd_t);
                     fpart(c_l, ct, al, gl, gr, gv);
@@ -607,12 +611,14 @@ __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(2)
             }
         };
         double dacc, lam[3];
-        stage1(L, P, dacc, lam, siu[2], sid[2], sli[2], slx[2]);
-        scd[2] = P.cd;
-        s_iu[s2][L] = siu[2];
-        s_id[s2][L] = sid[2];
-        s_li[s2][L] = sli[2];
-        s_lx[s2][L] = slx[2];
+        {
+            double iu, id, li, lx;
+            stage1(L, P, dacc, lam, iu, id, li, lx);
+            s_iu[s2][L] = iu;
+            s_id[s2][L] = id;
+            s_li[s2][L] = li;
+            s_lx[s2][L] = lx;
+        }
         if (store && p >= z0 && p < z1 && a.lq) st(a.lq + zp0.off, colb, ld(a.lq + zp0.off, colb) + P.lo[1]);
         if (h1) {
             double hd, hl3[3], hiu, hid, hli, hlx;
@@ -629,7 +635,11 @@ __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(2)
             const double* C = s_c[k3];
             const double *IU = s_iu[k2], *ID = s_id[k2], *LI = s_li[k2], *LX = s_lx[k2];
             const int* CD = s_cd[k2];
-            const double ct = P.c1[0];  // c1 of plane k
+            const double ct = C[L];  // c1 of plane k
+            // the cell's own column: stage-1 results and codes of planes k (LDS slot k2),
+            // k + 1 (slot s2) and k - 1 (registers)
+            const double iu_k = IU[L], id_k = ID[L], iu_n = s_iu[s2][L], id_n = s_id[s2][L];
+            const int cd_k = CD[L], cd_n = s_cd[s2][L];
             // neighbour y, the code q under which y sees t, whether y is t itself (extents
             // of 1 wrap a neighbour onto the cell)
             double s0 = 0.0, s1 = 0.0;
@@ -643,24 +653,24 @@ __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(2)
                     if (lc & 1) s1 -= idy; else s0 -= idy;
                 }
             };
-            bound(scd[1], siu[1], sid[1], 0, true);
+            bound(cd_k, iu_k, id_k, 0, true);
             bound(CD[L + 1], IU[L + 1], ID[L + 1], 1, fx);
             bound(CD[L - 1], IU[L - 1], ID[L - 1], 2, fx);
             bound(CD[L + RX], IU[L + RX], ID[L + RX], 3, fy);
             bound(CD[L - RX], IU[L - RX], ID[L - RX], 4, fy);
-            bound(scd[2], siu[2], sid[2], 5, fz);
-            bound(scd[0], siu[0], sid[0], 6, fz);
+            bound(cd_n, iu_n, id_n, 5, fz);
+            bound(scd0, siu0, sid0, 6, fz);
             {
-                const int c = scd[1], hc = c & 0xff, lc = (c >> 8) & 0xff;
+                const int c = cd_k, hc = c & 0xff, lc = (c >> 8) & 0xff;
                 auto self = [&](int q) {
                     return q == 0 || (q <= 2 && fx) || ((q == 3 || q == 4) && fy) || (q >= 5 && fz);
                 };
                 const bool hself = self(hc >> 1) && (hc & 1), lself = self(lc >> 1) && (lc & 1);
-                if (!hself) s1 -= siu[1];
-                if (!lself) s1 += sid[1];
+                if (!hself) s1 -= iu_k;
+                if (!lself) s1 += id_k;
             }
             // flux sums of the six faces
-            const double li_t = sli[1], lx_t = slx[1];
+            const double li_t = LI[L], lx_t = LX[L];
             double acc = 0.0, lAf[3];
 #pragma unroll
             for (int d = 0; d < 3; d++) {
@@ -672,7 +682,7 @@ __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(2)
                     } else if (d == 1) {
                         a_ = s_a1[k3][L]; c_r = C[L + RX]; li_r = LI[L + RX]; lx_r = LX[L + RX];
                     } else {
-                        a_ = P.a2m; c_r = P.c1[1]; li_r = sli[2]; lx_r = slx[2];
+                        a_ = P.a2m; c_r = s_c[s3][L]; li_r = s_li[s2][L]; lx_r = s_lx[s2][L];
                     }
                     const double f = 0.5 * F2(ct, c_r, a_);
                     const double lf = (f > 0 ? lx_t : 0.0) - (f < 0 ? li_t : 0.0) +
@@ -688,7 +698,7 @@ __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(2)
                     } else if (d == 1) {
                         a_ = s_a1[k3][L - RX]; c_l = C[L - RX]; li_l = LI[L - RX]; lx_l = LX[L - RX];
                     } else {
-                        a_ = a2_m2; c_l = c1_m2; li_l = sli[0]; lx_l = slx[0];
+                        a_ = a2_m2; c_l = s_c[(p + 1) % 3][L]; li_l = sli0; lx_l = slx0;
                     }
                     const double f = 0.5 * F2(c_l, ct, a_);
                     const double lf = (f > 0 ? lx_l : 0.0) - (f < 0 ? li_l : 0.0) +
@@ -705,17 +715,17 @@ __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(2)
                 st(a.lA2 + zm1.off, colb, lAf[2]);
             }
         }
-        // shift the tile cell's history by one plane
-        siu[0] = siu[1]; siu[1] = siu[2];
-        sid[0] = sid[1]; sid[1] = sid[2];
-        sli[0] = sli[1]; sli[1] = sli[2];
-        slx[0] = slx[1]; slx[1] = slx[2];
-        scd[0] = scd[1]; scd[1] = scd[2];
+        // shift the tile cell's history by one plane: plane p - 1 becomes p - 2 (its LDS
+        // slot is overwritten by the next iteration's stage 1)
+        siu0 = s_iu[k2][L];
+        sid0 = s_id[k2][L];
+        sli0 = s_li[k2][L];
+        slx0 = s_lx[k2][L];
+        scd0 = s_cd[k2][L];
         dacc_m = dacc;
         lam_m[0] = lam[0];
         lam_m[1] = lam[1];
         lam_m[2] = lam[2];
-        c1_m2 = P.c1[0];
         a2_m2 = P.a2m;
         rotate(P);
         if (hl) rotate(H);
@@ -1297,10 +1307,30 @@ extern "C" int mpd_adjoint_run(mpd_grid g, int nsteps, int checkpoint_every, int
     std::vector<DevBuf> ck(want_grad ? nck : 0);
     for (auto& b : ck)
         if ((rc = b.alloc(n))) return rc;
+    // storage of the backward sweep, allocated before the timed sweeps: the adjoint
+    // intermediates, a segment's states and the intermediates of its steps (about ten
+    // arrays per step)
+    Work w;
+    DevBuf lam, lnext, gq, tmp, final_state;
+    std::vector<DevBuf> seg(want_grad ? K : 0);
+    std::vector<Fwd> fw(want_grad ? K : 0);
+    if (want_grad) {
+        if ((rc = w.alloc(n)) || (rc = lam.alloc(n)) || (rc = lnext.alloc(n)) ||
+            (rc = gq.alloc(n)) || (rc = tmp.alloc(n)) || (rc = final_state.alloc(n)))
+            return rc;
+        for (int m = 0; m < K; m++)
+            if ((rc = seg[m].alloc(n)) || (rc = fw[m].alloc(n, g->nonosc)))
+                return rc == MPD_ERR_NOMEM
+                           ? fail(rc, "mpd_adjoint_run: device memory for %d stored steps of "
+                                      "%zu cells exhausted; use a shorter checkpoint interval",
+                                  K, n)
+                           : rc;
+    }
 
-    hipEvent_t e0, e1, e2;
+    hipEvent_t e0, e1, e1b, e2;
     MPD_HIP(hipEventCreate(&e0));
     MPD_HIP(hipEventCreate(&e1));
+    MPD_HIP(hipEventCreate(&e1b));
     MPD_HIP(hipEventCreate(&e2));
     MPD_HIP(hipEventRecord(e0, s));
     // forward sweep
@@ -1316,11 +1346,8 @@ extern "C" int mpd_adjoint_run(mpd_grid g, int nsteps, int checkpoint_every, int
         }
     }
     MPD_HIP(hipEventRecord(e1, s));
-    DevBuf final_state;
-    if (want_grad) {  // c^nsteps: the last residual and the field handed back
-        if ((rc = final_state.alloc(n))) return rc;
+    if (want_grad)  // c^nsteps: the last residual and the field handed back
         MPD_HIP(hipMemcpyAsync(final_state.p, g->c[g->cur].p, n * 8, hipMemcpyDeviceToDevice, s));
-    }
     std::vector<double> res((size_t)std::max(nsamp, 1) * nrec, 0.0);
     if (nsamp) MPD_HIP(hipMemcpyAsync(res.data(), dres.p, (size_t)nsamp * nrec * 8, hipMemcpyDeviceToHost, s));
     MPD_HIP(hipStreamSynchronize(s));
@@ -1328,22 +1355,8 @@ extern "C" int mpd_adjoint_run(mpd_grid g, int nsteps, int checkpoint_every, int
     for (size_t q = 0; q < (size_t)nsamp * nrec; q++) jj += 0.5 * res[q] * res[q];
     *J = jj;
 
+    MPD_HIP(hipEventRecord(e1b, s));
     if (want_grad) {
-        Work w;
-        if ((rc = w.alloc(n))) return rc;
-        DevBuf lam, lnext, gq, tmp;
-        if ((rc = lam.alloc(n)) || (rc = lnext.alloc(n)) || (rc = gq.alloc(n)) || (rc = tmp.alloc(n)))
-            return rc;
-        // a segment's states and the intermediates of its steps (about 10 arrays per step)
-        std::vector<DevBuf> seg(K);
-        std::vector<Fwd> fw(K);
-        for (int m = 0; m < K; m++)
-            if ((rc = seg[m].alloc(n)) || (rc = fw[m].alloc(n, g->nonosc)))
-                return rc == MPD_ERR_NOMEM
-                           ? fail(rc, "mpd_adjoint_run: device memory for %d stored steps of "
-                                      "%zu cells exhausted; use a shorter checkpoint interval",
-                                  K, n)
-                           : rc;
         MPD_HIP(hipMemsetAsync(lam.p, 0, n * 8, s));
         MPD_HIP(hipMemsetAsync(gq.p, 0, n * 8, s));
         // a scratch grid view that steps the segment states: reuse g's buffers
@@ -1388,9 +1401,10 @@ extern "C" int mpd_adjoint_run(mpd_grid g, int nsteps, int checkpoint_every, int
     }
     float f1 = 0.f, f2 = 0.f;
     (void)hipEventElapsedTime(&f1, e0, e1);
-    (void)hipEventElapsedTime(&f2, e1, e2);
+    (void)hipEventElapsedTime(&f2, e1b, e2);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e1b);
     (void)hipEventDestroy(e2);
     if (times) {
         times[0] = f1;
