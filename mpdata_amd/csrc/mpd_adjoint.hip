@@ -390,7 +390,8 @@ __global__ void k_adj_anti_atomic(Geo g, AdjBufs b, const double* __restrict__ l
 // laout) -> barrier -> stage 2 on the tile of plane k = p - 1 (the bound gather over the
 // 7-point neighbourhood and the flux-sum adjoints of the six faces).  LDS slots: lout,
 // bup, bdn one plane (read in stage 1 only), c1, A0, A1 three (stage 1 of plane p, stage
-// 2 of plane p - 1), the stage-1 results and the codes two.
+// 2 of plane p - 1), the stage-1 results and the codes two (both written after the
+// first barrier: stage 2 of the previous iteration reads the other slot).
 struct LimArgs {
     int nx, ny, nz;
     long long np;
@@ -415,9 +416,10 @@ __device__ __forceinline__ int ld_code(const int* plane, unsigned colb)
 }
 
 template <int TX, int TY>
-__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(2))) k_adj_lim_t(LimArgs a)
+__global__ void __launch_bounds__((tile::GeoB<TX, TY>::NTH)) __attribute__((amdgpu_waves_per_eu(2)))
+k_adj_lim_t(LimArgs a)
 {
-    using G = tile::Geo<TX, TY>;
+    using G = tile::GeoB<TX, TY>;
     using tile::ld;
     using tile::st;
     constexpr int RX = G::RX, R = G::R;
@@ -429,15 +431,15 @@ __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(2)
     const int item = blockIdx.x;
     const int bx = item % a.tiles_x, by = (item / a.tiles_x) % a.tiles_y;
     const int bz = item / (a.tiles_x * a.tiles_y);
-    const int rx = 2 + t % TX, ry = 2 + t / TX;
-    const int L = ry * RX + rx;
-    const unsigned colb = tile::column<TX, TY>(a.nx, a.ny, bx, by, rx, ry);
-    const bool store = bx * TX + rx - 2 < a.nx && by * TY + ry - 2 < a.ny;
-    const bool hl = t < G::NH, h1 = t < G::H1;
-    int hx = 0, hy = 0;
-    if (hl) G::halo(t, hx, hy);
-    const int HL = hy * RX + hx;
-    const unsigned hcolb = tile::column<TX, TY>(a.nx, a.ny, bx, by, hx, hy);
+    // own cell (tile: stages 1 and 2; ring 1: stage 1) and ring-2 cell (loads)
+    int ox, oy, sx, sy;
+    const int role = G::own(t, ox, oy);
+    const bool comp = role >= 0, tcell = role == 0;
+    const bool sec = G::second(t, sx, sy);
+    const int L = oy * RX + ox, S = sy * RX + sx;
+    const unsigned colb = tile::column<TX, TY>(a.nx, a.ny, bx, by, ox, oy);
+    const unsigned scolb = tile::column<TX, TY>(a.nx, a.ny, bx, by, sx, sy);
+    const bool store = tcell && bx * TX + ox - 2 < a.nx && by * TY + oy - 2 < a.ny;
     const int z0 = bz * a.zc;
     const int z1 = min(z0 + a.zc, a.nz);
     // planes p + 2, p + 1, p, p - 1, p - 2 of iteration p
@@ -510,12 +512,12 @@ __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(2)
         q.cd = q.ncd;
     };
     LimCol P, H;
-    init(P, colb, true);
-    if (hl) init(H, hcolb, h1);
+    if (comp) init(P, colb, true);
+    if (sec) init(H, scolb, false);
     // c1 of plane z0 - 2: stage 1 of the first iteration reads it from LDS (the loop
     // publishes planes from z0 - 1 on)
-    s_c[(z0 + 1) % 3][L] = P.c1[0];
-    if (hl) s_c[(z0 + 1) % 3][HL] = H.c1[0];
+    if (comp) s_c[(z0 + 1) % 3][L] = P.c1[0];
+    if (sec) s_c[(z0 + 1) % 3][S] = H.c1[0];
 
     // tile cell: stage-1 results of planes p-2, p-1 (index 0, 1; 2 = plane p), the codes,
     // the carried donor sum and lam_Al * m of plane p-1, c1 and A2 of plane p-2
@@ -527,21 +529,21 @@ __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(2)
         const int p = __builtin_amdgcn_readfirstlane(pl);
         const int s3 = (p + 3) % 3, k3 = (p + 2) % 3;  // slots of planes p, p-1 (c1, A0, A1)
         const int s2 = (p + 2) % 2, k2 = (p + 1) % 2;  // slots of planes p, p-1 (stage 1)
-        s_lo[L] = P.lo[1];
-        s_bu[L] = P.bu[1];
-        s_bd[L] = P.bd[1];
-        s_c[s3][L] = P.c1[1];
-        s_a0[s3][L] = P.a0;
-        s_a1[s3][L] = P.a1;
-        s_cd[s2][L] = P.cd;
-        if (hl) {
-            s_lo[HL] = H.lo[1];
-            s_bu[HL] = H.bu[1];
-            s_bd[HL] = H.bd[1];
-            s_c[s3][HL] = H.c1[1];
-            s_a0[s3][HL] = H.a0;
-            s_a1[s3][HL] = H.a1;
-            s_cd[s2][HL] = H.cd;
+        if (comp) {
+            s_lo[L] = P.lo[1];
+            s_bu[L] = P.bu[1];
+            s_bd[L] = P.bd[1];
+            s_c[s3][L] = P.c1[1];
+            s_a0[s3][L] = P.a0;
+            s_a1[s3][L] = P.a1;
+        }
+        if (sec) {
+            s_lo[S] = H.lo[1];
+            s_bu[S] = H.bu[1];
+            s_bd[S] = H.bd[1];
+            s_c[s3][S] = H.c1[1];
+            s_a0[s3][S] = H.a0;
+            s_a1[s3][S] = H.a1;
         }
         tile::lds_barrier();
         // stage 1 on cell M of plane p (the merged k_adj_donor_limit, same operation order)
@@ -610,28 +612,25 @@ __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(2)
                 lx = -id * bd_t;
             }
         };
-        double dacc, lam[3];
-        {
+        double dacc = 0.0, lam[3] = {0.0, 0.0, 0.0};
+        if (comp) {
             double iu, id, li, lx;
             stage1(L, P, dacc, lam, iu, id, li, lx);
+            // the codes of plane p go to LDS here, after the first barrier, like the
+            // stage-1 results: their slot held plane p - 2, which stage 2 of the previous
+            // iteration may still be reading until every wave has reached that barrier
+            s_cd[s2][L] = P.cd;
             s_iu[s2][L] = iu;
             s_id[s2][L] = id;
             s_li[s2][L] = li;
             s_lx[s2][L] = lx;
         }
-        if (store && p >= z0 && p < z1 && a.lq) st(a.lq + zp0.off, colb, ld(a.lq + zp0.off, colb) + P.lo[1]);
-        if (h1) {
-            double hd, hl3[3], hiu, hid, hli, hlx;
-            stage1(HL, H, hd, hl3, hiu, hid, hli, hlx);
-            s_iu[s2][HL] = hiu;
-            s_id[s2][HL] = hid;
-            s_li[s2][HL] = hli;
-            s_lx[s2][HL] = hlx;
-        }
+        if (store && p >= z0 && p < z1 && a.lq)
+            st(a.lq + zp0.off, colb, ld(a.lq + zp0.off, colb) + P.lo[1]);
         tile::lds_barrier();
         // stage 2 on the tile cell of plane k = p - 1 (k_adj_limit3, same operation order)
         const int k = p - 1;
-        if (k >= z0 && k < z1) {
+        if (tcell && k >= z0 && k < z1) {
             const double* C = s_c[k3];
             const double *IU = s_iu[k2], *ID = s_id[k2], *LI = s_li[k2], *LX = s_lx[k2];
             const int* CD = s_cd[k2];
@@ -727,16 +726,16 @@ __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(2)
         lam_m[1] = lam[1];
         lam_m[2] = lam[2];
         a2_m2 = P.a2m;
-        rotate(P);
-        if (hl) rotate(H);
+        if (comp) rotate(P);
+        if (sec) rotate(H);
         zp2.next();
         zp1.next();
         zp0.next();
         zm1.next();
         if (p + 2 <= z1) {  // for the rotation after the next iteration: fields at p + 3,
                             // faces at p + 2 (the cursors now stand one plane higher)
-            load_next(P, colb, true, zp2, zp1);
-            if (hl) load_next(H, hcolb, h1, zp2, zp1);
+            if (comp) load_next(P, colb, true, zp2, zp1);
+            if (sec) load_next(H, scolb, false, zp2, zp1);
         }
     }
 }
@@ -745,7 +744,7 @@ int lim_tiled(const mpd_grid_s* g, const double* const in[9], const int* code,
               const double* lout, double* l1, double* const lA[3], double* lin, double* lq,
               hipStream_t s)
 {
-    constexpr int TX = 32, TY = 8;
+    constexpr int TX = 16, TY = 12;
     LimArgs a{};
     a.nx = g->nx;
     a.ny = g->ny;
@@ -774,8 +773,8 @@ int lim_tiled(const mpd_grid_s* g, const double* const in[9], const int* code,
     a.lA2 = lA[2];
     a.lin = lin;
     a.lq = lq;
-    hipLaunchKernelGGL((k_adj_lim_t<TX, TY>), dim3((unsigned)(cols * nchunks)), dim3(TX * TY), 0,
-                       s, a);
+    hipLaunchKernelGGL((k_adj_lim_t<TX, TY>), dim3((unsigned)(cols * nchunks)),
+                       dim3(tile::GeoB<TX, TY>::NTH), 0, s, a);
     MPD_HIP(hipGetLastError());
     return MPD_OK;
 }
@@ -833,9 +832,9 @@ struct AntiArgs {
 // coefficients one (written after the first barrier of an iteration, read before the
 // first barrier of the next).
 template <int TX, int TY>
-__global__ void __launch_bounds__(TX * TY) k_adj_anti_t(AntiArgs a)
+__global__ void __launch_bounds__((tile::GeoB<TX, TY>::NTH)) k_adj_anti_t(AntiArgs a)
 {
-    using G = tile::Geo<TX, TY>;
+    using G = tile::GeoB<TX, TY>;
     using tile::ld;
     using tile::st;
     constexpr int RX = G::RX, R = G::R;
@@ -846,15 +845,15 @@ __global__ void __launch_bounds__(TX * TY) k_adj_anti_t(AntiArgs a)
     const int item = blockIdx.x;
     const int bx = item % a.tiles_x, by = (item / a.tiles_x) % a.tiles_y;
     const int bz = item / (a.tiles_x * a.tiles_y);
-    const int rx = 2 + t % TX, ry = 2 + t / TX;
-    const int L = ry * RX + rx;
-    const unsigned colb = tile::column<TX, TY>(a.nx, a.ny, bx, by, rx, ry);
-    const bool store = bx * TX + rx - 2 < a.nx && by * TY + ry - 2 < a.ny;
-    const bool hl = t < G::NH, h1 = t < G::H1;
-    int hx = 0, hy = 0;
-    if (hl) G::halo(t, hx, hy);
-    const int HL = hy * RX + hx;
-    const unsigned hcolb = tile::column<TX, TY>(a.nx, a.ny, bx, by, hx, hy);
+    // own cell (tile: coefficients + gather; ring 1: coefficients) and ring-2 cell (loads)
+    int ox, oy, sx, sy;
+    const int role = G::own(t, ox, oy);
+    const bool comp = role >= 0, tcell = role == 0;
+    const bool sec = G::second(t, sx, sy);
+    const int L = oy * RX + ox, S = sy * RX + sx;
+    const unsigned colb = tile::column<TX, TY>(a.nx, a.ny, bx, by, ox, oy);
+    const unsigned scolb = tile::column<TX, TY>(a.nx, a.ny, bx, by, sx, sy);
+    const bool store = tcell && bx * TX + ox - 2 < a.nx && by * TY + oy - 2 < a.ny;
     const int z0 = bz * a.zc;
     const int z1 = min(z0 + a.zc, a.nz);
     // planes p + 2, p + 1, p, p - 1 of iteration p
@@ -864,36 +863,40 @@ __global__ void __launch_bounds__(TX * TY) k_adj_anti_t(AntiArgs a)
     // planes before the first iteration: c1 (z0-2, z0-1), U, V (z0-1), W (z0-2)
     {
         const int cs0 = (z0 + 1) % 3, cs1 = (z0 + 2) % 3, us = (z0 + 1) % 2, ws = z0 % 2;
-        s_c[cs0][L] = ld(a.c1 + zm1.off, colb);
-        s_c[cs1][L] = ld(a.c1 + zp0.off, colb);
-        s_u[us][L] = ld(a.U + zp0.off, colb);
-        s_v[us][L] = ld(a.V + zp0.off, colb);
-        s_w[ws][L] = ld(a.W + zm1.off, colb);
-        if (hl) {
-            s_c[cs0][HL] = ld(a.c1 + zm1.off, hcolb);
-            s_c[cs1][HL] = ld(a.c1 + zp0.off, hcolb);
-            s_u[us][HL] = ld(a.U + zp0.off, hcolb);
-            s_v[us][HL] = ld(a.V + zp0.off, hcolb);
-            s_w[ws][HL] = ld(a.W + zm1.off, hcolb);
+        if (comp) {
+            s_c[cs0][L] = ld(a.c1 + zm1.off, colb);
+            s_c[cs1][L] = ld(a.c1 + zp0.off, colb);
+            s_u[us][L] = ld(a.U + zp0.off, colb);
+            s_v[us][L] = ld(a.V + zp0.off, colb);
+            s_w[ws][L] = ld(a.W + zm1.off, colb);
+        }
+        if (sec) {
+            s_c[cs0][S] = ld(a.c1 + zm1.off, scolb);
+            s_c[cs1][S] = ld(a.c1 + zp0.off, scolb);
+            s_u[us][S] = ld(a.U + zp0.off, scolb);
+            s_v[us][S] = ld(a.V + zp0.off, scolb);
+            s_w[ws][S] = ld(a.W + zm1.off, scolb);
         }
     }
     // values published / consumed by the next iteration: c1, U, V at p+1, W and the face
-    // adjoints at p (tile and halo column), l1 at p (tile column)
-    double nc = ld(a.c1 + zp1.off, colb), nu = ld(a.U + zp1.off, colb);
-    double nv = ld(a.V + zp1.off, colb), nw = ld(a.W + zp0.off, colb);
-    double nl0 = ld(a.lA0 + zp0.off, colb), nl1 = ld(a.lA1 + zp0.off, colb);
-    double nl2 = ld(a.lA2 + zp0.off, colb), nl1c = ld(a.l1 + zp0.off, colb);
-    double hnc = 0, hnu = 0, hnv = 0, hnw = 0, hnl0 = 0, hnl1 = 0, hnl2 = 0;
-    if (hl) {
-        hnc = ld(a.c1 + zp1.off, hcolb);
-        hnu = ld(a.U + zp1.off, hcolb);
-        hnv = ld(a.V + zp1.off, hcolb);
-        hnw = ld(a.W + zp0.off, hcolb);
-        if (h1) {
-            hnl0 = ld(a.lA0 + zp0.off, hcolb);
-            hnl1 = ld(a.lA1 + zp0.off, hcolb);
-            hnl2 = ld(a.lA2 + zp0.off, hcolb);
-        }
+    // adjoints at p (own and ring-2 column), l1 at p (tile column)
+    double nc = 0, nu = 0, nv = 0, nw = 0, nl0 = 0, nl1 = 0, nl2 = 0, nl1c = 0;
+    double snc = 0, snu = 0, snv = 0, snw = 0;
+    if (comp) {
+        nc = ld(a.c1 + zp1.off, colb);
+        nu = ld(a.U + zp1.off, colb);
+        nv = ld(a.V + zp1.off, colb);
+        nw = ld(a.W + zp0.off, colb);
+        nl0 = ld(a.lA0 + zp0.off, colb);
+        nl1 = ld(a.lA1 + zp0.off, colb);
+        nl2 = ld(a.lA2 + zp0.off, colb);
+        if (tcell) nl1c = ld(a.l1 + zp0.off, colb);
+    }
+    if (sec) {
+        snc = ld(a.c1 + zp1.off, scolb);
+        snu = ld(a.U + zp1.off, scolb);
+        snv = ld(a.V + zp1.off, scolb);
+        snw = ld(a.W + zp0.off, scolb);
     }
     double accm = 0.0, accc = 0.0, accn = 0.0, l1m = 0.0;
     for (int pl = z0 - 1; pl <= z1; pl++) {
@@ -901,67 +904,64 @@ __global__ void __launch_bounds__(TX * TY) k_adj_anti_t(AntiArgs a)
         const int cm = (p + 2) % 3, cc = (p + 3) % 3, cq = (p + 4) % 3;  // c1 p-1, p, p+1
         const int up = (p + 2) % 2, uq = (p + 3) % 2;                     // U, V p, p+1
         const int wm = (p + 1) % 2, wc = (p + 2) % 2;                     // W p-1, p
-        const double la0 = nl0, la1 = nl1, la2 = nl2, hla0 = hnl0, hla1 = hnl1, hla2 = hnl2;
-        const double l1c = nl1c;
-        s_c[cq][L] = nc;
-        s_u[uq][L] = nu;
-        s_v[uq][L] = nv;
-        s_w[wc][L] = nw;
-        if (hl) {
-            s_c[cq][HL] = hnc;
-            s_u[uq][HL] = hnu;
-            s_v[uq][HL] = hnv;
-            s_w[wc][HL] = hnw;
+        const double la0 = nl0, la1 = nl1, la2 = nl2, l1c = nl1c;
+        if (comp) {
+            s_c[cq][L] = nc;
+            s_u[uq][L] = nu;
+            s_v[uq][L] = nv;
+            s_w[wc][L] = nw;
+        }
+        if (sec) {
+            s_c[cq][S] = snc;
+            s_u[uq][S] = snu;
+            s_v[uq][S] = snv;
+            s_w[wc][S] = snw;
         }
         if (p < z1) {
-            nc = ld(a.c1 + zp2.off, colb);
-            nu = ld(a.U + zp2.off, colb);
-            nv = ld(a.V + zp2.off, colb);
-            nw = ld(a.W + zp1.off, colb);
-            nl0 = ld(a.lA0 + zp1.off, colb);
-            nl1 = ld(a.lA1 + zp1.off, colb);
-            nl2 = ld(a.lA2 + zp1.off, colb);
-            nl1c = ld(a.l1 + zp1.off, colb);
-            if (hl) {
-                hnc = ld(a.c1 + zp2.off, hcolb);
-                hnu = ld(a.U + zp2.off, hcolb);
-                hnv = ld(a.V + zp2.off, hcolb);
-                hnw = ld(a.W + zp1.off, hcolb);
-                if (h1) {
-                    hnl0 = ld(a.lA0 + zp1.off, hcolb);
-                    hnl1 = ld(a.lA1 + zp1.off, hcolb);
-                    hnl2 = ld(a.lA2 + zp1.off, hcolb);
-                }
+            if (comp) {
+                nc = ld(a.c1 + zp2.off, colb);
+                nu = ld(a.U + zp2.off, colb);
+                nv = ld(a.V + zp2.off, colb);
+                nw = ld(a.W + zp1.off, colb);
+                nl0 = ld(a.lA0 + zp1.off, colb);
+                nl1 = ld(a.lA1 + zp1.off, colb);
+                nl2 = ld(a.lA2 + zp1.off, colb);
+                if (tcell) nl1c = ld(a.l1 + zp1.off, colb);
+            }
+            if (sec) {
+                snc = ld(a.c1 + zp2.off, scolb);
+                snu = ld(a.U + zp2.off, scolb);
+                snv = ld(a.V + zp2.off, scolb);
+                snw = ld(a.W + zp1.off, scolb);
             }
         }
         tile::lds_barrier();
-        // coefficients of the three faces above/right of cell M in plane p
-        auto faces = [&](int M, double lx, double ly, double lz, FaceK& kx, FaceK& ky,
-                         FaceK& kz) {
+        // coefficients of the three faces above/right of the own cell in plane p
+        FaceK kx{}, ky{}, kz{};
+        if (comp) {
             const double *C = s_c[cc], *Cm = s_c[cm], *Cq = s_c[cq];
             const double *Up = s_u[up], *Uq = s_u[uq], *Vp = s_v[up], *Vq = s_v[uq];
             const double *Wm = s_w[wm], *Wc = s_w[wc];
+            const int M = L;
             const double m = C[M];
             {  // x-face (uu.c:81-87)
                 const double s1 = 0.25 * (Vp[M + 1 - RX] + Vp[M + 1] + Vp[M] + Vp[M - RX]);
                 const double s2 = 0.25 * (Wm[M + 1] + Wc[M + 1] + Wc[M] + Wm[M]);
-                kx = face_k(lx, Up[M], C[M + 1], m, s1, C[M + 1 + RX], C[M + RX], C[M - RX],
+                kx = face_k(la0, Up[M], C[M + 1], m, s1, C[M + 1 + RX], C[M + RX], C[M - RX],
                             C[M + 1 - RX], s2, Cq[M + 1], Cq[M], Cm[M], Cm[M + 1]);
             }
             {  // y-face (vv.c:85-92)
                 const double s1 = 0.25 * (Up[M - 1 + RX] + Up[M + RX] + Up[M] + Up[M - 1]);
                 const double s2 = 0.25 * (Wm[M + RX] + Wc[M + RX] + Wc[M] + Wm[M]);
-                ky = face_k(ly, Vp[M], C[M + RX], m, s1, C[M + 1 + RX], C[M + 1], C[M - 1],
+                ky = face_k(la1, Vp[M], C[M + RX], m, s1, C[M + 1 + RX], C[M + 1], C[M - 1],
                             C[M - 1 + RX], s2, Cq[M + RX], Cq[M], Cm[M], Cm[M + RX]);
             }
             {  // z-face (ww.c:248-255)
                 const double s1 = 0.25 * (Up[M] + Uq[M] + Uq[M - 1] + Up[M - 1]);
                 const double s2 = 0.25 * (Vq[M - RX] + Vq[M] + Vp[M] + Vp[M - RX]);
-                kz = face_k(lz, Wc[M], Cq[M], m, s1, Cq[M + 1], C[M + 1], C[M - 1], Cq[M - 1],
+                kz = face_k(la2, Wc[M], Cq[M], m, s1, Cq[M + 1], C[M + 1], C[M - 1], Cq[M - 1],
                             s2, Cq[M + RX], C[M + RX], C[M - RX], Cq[M - RX]);
             }
-        };
-        auto publish = [&](int M, const FaceK& kx, const FaceK& ky, const FaceK& kz) {
             s_k[0][M] = kx.kp;
             s_k[1][M] = kx.ku1;
             s_k[2][M] = kx.kd1;
@@ -976,32 +976,27 @@ __global__ void __launch_bounds__(TX * TY) k_adj_anti_t(AntiArgs a)
             s_k[11][M] = kz.kd1;
             s_k[12][M] = kz.ku2;
             s_k[13][M] = kz.kd2;
-        };
-        FaceK kx, ky, kz;
-        faces(L, la0, la1, la2, kx, ky, kz);
-        publish(L, kx, ky, kz);
-        if (h1) {
-            FaceK hx_, hy_, hz_;
-            faces(HL, hla0, hla1, hla2, hx_, hy_, hz_);
-            publish(HL, hx_, hy_, hz_);
         }
         tile::lds_barrier();
-        // terms of the faces of plane p, by target plane (own-column terms from registers)
-        accm += (kx.kd2 + s_k[4][L - 1]) + (ky.kd2 + s_k[9][L - RX]);
-        accc += kx.km + s_k[0][L - 1] + s_k[1][L - 1 - RX] + s_k[1][L - RX] + s_k[2][L + RX] +
-                s_k[2][L - 1 + RX];
-        accc += ky.km + s_k[5][L - RX] + s_k[6][L - 1 - RX] + s_k[6][L - 1] + s_k[7][L + 1] +
-                s_k[7][L + 1 - RX];
-        const double zu1 = s_k[10][L - 1], zd1 = s_k[11][L + 1], zu2 = s_k[12][L - RX];
-        const double zd2 = s_k[13][L + RX];
-        accc += kz.km + zu1 + zd1 + zu2 + zd2;
-        accn += (kx.ku2 + s_k[3][L - 1]) + (ky.ku2 + s_k[8][L - RX]) + kz.kp + zu1 + zd1 + zu2 +
-                zd2;
-        if (store && p - 1 >= z0 && p - 1 < z1) st(a.l1 + zm1.off, colb, l1m + accm);
-        accm = accc;
-        accc = accn;
-        accn = 0.0;
-        l1m = l1c;
+        if (tcell) {
+            // terms of the faces of plane p, by target plane (own-column terms from
+            // registers)
+            accm += (kx.kd2 + s_k[4][L - 1]) + (ky.kd2 + s_k[9][L - RX]);
+            accc += kx.km + s_k[0][L - 1] + s_k[1][L - 1 - RX] + s_k[1][L - RX] +
+                    s_k[2][L + RX] + s_k[2][L - 1 + RX];
+            accc += ky.km + s_k[5][L - RX] + s_k[6][L - 1 - RX] + s_k[6][L - 1] + s_k[7][L + 1] +
+                    s_k[7][L + 1 - RX];
+            const double zu1 = s_k[10][L - 1], zd1 = s_k[11][L + 1], zu2 = s_k[12][L - RX];
+            const double zd2 = s_k[13][L + RX];
+            accc += kz.km + zu1 + zd1 + zu2 + zd2;
+            accn += (kx.ku2 + s_k[3][L - 1]) + (ky.ku2 + s_k[8][L - RX]) + kz.kp + zu1 + zd1 +
+                    zu2 + zd2;
+            if (store && p - 1 >= z0 && p - 1 < z1) st(a.l1 + zm1.off, colb, l1m + accm);
+            accm = accc;
+            accc = accn;
+            accn = 0.0;
+            l1m = l1c;
+        }
         zp2.next();
         zp1.next();
         zp0.next();
@@ -1012,7 +1007,7 @@ __global__ void __launch_bounds__(TX * TY) k_adj_anti_t(AntiArgs a)
 int anti_tiled(const mpd_grid_s* g, const double* c1, double* const lA[3], double* l1,
                hipStream_t s)
 {
-    constexpr int TX = 32, TY = 8;
+    constexpr int TX = 16, TY = 12;
     AntiArgs a{};
     a.nx = g->nx;
     a.ny = g->ny;
@@ -1034,8 +1029,8 @@ int anti_tiled(const mpd_grid_s* g, const double* c1, double* const lA[3], doubl
     a.lA1 = lA[1];
     a.lA2 = lA[2];
     a.l1 = l1;
-    hipLaunchKernelGGL((k_adj_anti_t<TX, TY>), dim3((unsigned)(cols * nchunks)), dim3(TX * TY), 0,
-                       s, a);
+    hipLaunchKernelGGL((k_adj_anti_t<TX, TY>), dim3((unsigned)(cols * nchunks)),
+                       dim3(tile::GeoB<TX, TY>::NTH), 0, s, a);
     MPD_HIP(hipGetLastError());
     return MPD_OK;
 }

@@ -44,6 +44,47 @@ struct Geo {
     }
 };
 
+// Geometry with at most one computing cell per thread, for kernels whose ring-1 work
+// is as expensive as their tile work: threads [0, NT) own the tile cells, threads
+// [NT, NT + H1) the ring-1 cells, so no wave evaluates the expensive stage twice.  The
+// ring-2 cells (loads only) are secondary cells of the spare lanes of the last wave and
+// then of the first tile threads.
+template <int TX, int TY>
+struct GeoB {
+    using G = Geo<TX, TY>;
+    static constexpr int RX = G::RX, RY = G::RY, R = G::R, NT = G::NT, H1 = G::H1;
+    static constexpr int NTH = ((NT + H1 + 63) / 64) * 64;  // threads of a workgroup
+    static constexpr int H2 = R - NT - H1;                  // ring-2 cells
+    static constexpr int SPARE = NTH - NT - H1;
+    static_assert(H2 <= SPARE + NT, "every ring-2 cell has a thread");
+    // own cell of thread t: 0 = tile cell, 1 = ring-1 cell, -1 = none
+    __device__ static int own(int t, int& rx, int& ry)
+    {
+        if (t < NT) {
+            rx = 2 + t % TX;
+            ry = 2 + t / TX;
+            return 0;
+        }
+        if (t < NT + H1) {
+            G::halo(t - NT, rx, ry);
+            return 1;
+        }
+        rx = ry = 0;
+        return -1;
+    }
+    // ring-2 cell of thread t, if any
+    __device__ static bool second(int t, int& rx, int& ry)
+    {
+        const int j = t >= NT + H1 ? t - NT - H1 : SPARE + t;
+        if ((t >= NT + H1 || t < H2 - SPARE) && j < H2) {
+            G::halo(H1 + j, rx, ry);
+            return true;
+        }
+        rx = ry = 0;
+        return false;
+    }
+};
+
 // Workgroup barrier ordering LDS only.  A full __syncthreads() also fences global
 // memory, which makes every wave wait for its outstanding global loads at the barrier
 // and would defeat the one-plane-ahead prefetch below; these kernels exchange data

@@ -148,7 +148,10 @@ def test_atomic_scatter_matches_gather():
         assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
 
 
-@pytest.mark.parametrize("dims", [(12, 10, 8), (1, 8, 6), (2, 6, 4), (40, 9, 5)])
+# shapes whose tiles are partly or fully inside the grid in x and y, with several
+# z-chunks: the tiled kernels' edge rows read the halo waves' results
+@pytest.mark.parametrize("dims", [(12, 10, 8), (1, 8, 6), (2, 6, 4), (40, 9, 5), (14, 12, 10),
+                                  (16, 12, 8), (33, 25, 9)])
 def test_tiled_limiter_adjoint_matches_per_cell_passes(dims):
     nx, ny, nz = dims
     rng, U, V, W, c = case(nx, ny, nz, 77 + nx)
