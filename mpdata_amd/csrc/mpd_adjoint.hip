@@ -564,23 +564,20 @@ k_adj_lim_t(LimArgs a)
                     } else {
                         a_ = q.a2c; lo_r = q.lo[2]; c_r = q.c1[2]; bu_r = q.bu[2]; bd_r = q.bd[2];
                     }
-                    double m = 1.0;
-                    if (a_ > 0) {
-                        const int w = pick(bd_t, bu_r);
-                        m = w == 0 ? 1.0 : (w == 1 ? bd_t : bu_r);
-                    } else if (a_ < 0) {
-                        const int w = pick(bu_t, bd_r);
-                        m = w == 0 ? 1.0 : (w == 1 ? bu_t : bd_r);
-                    } else {
-                        m = 0.0;
-                    }
-                    const double au = limited(a_, bu_t, bd_t, bu_r, bd_r);
+                    // limiter multiplier without branches: the ratio pair of the
+                    // flow direction, the selection of mn2(1, mn2(x, y))
+                    const bool pf = a_ > 0, nf = a_ < 0;
+                    const double x = pf ? bd_t : bu_t, y = pf ? bu_r : bd_r;
+                    const int w = pick(x, y);
+                    const double m = (pf || nf) ? (w == 0 ? 1.0 : (w == 1 ? x : y)) : 0.0;
+                    const double au = (pf || nf) ? m * a_ : 0.0;
                     fpart(ct, c_r, au, gl, gr, gv);
                     const double lf = lo_r - lt;
                     acc += gl * lf;
                     const double Lf = gv * lf;
-                    if (a_ > 0 && pick(bd_t, bu_r) == 1) sd += a_ * Lf;
-                    if (a_ < 0 && pick(bu_t, bd_r) == 1) su += a_ * Lf;
+                    const double sel = (w == 1 && (pf || nf)) ? a_ * Lf : 0.0;
+                    sd += pf ? sel : 0.0;
+                    su += nf ? sel : 0.0;
                     lam[d] = m * Lf;
                 }
                 {  // lower face (l, M)
@@ -592,13 +589,18 @@ k_adj_lim_t(LimArgs a)
                     } else {
                         a_ = q.a2m; lo_l = q.lo[0]; c_l = s_c[k3][M]; bu_l = q.bu[0]; bd_l = q.bd[0];
                     }
-                    const double al = limited(a_, bu_l, bd_l, bu_t, bd_t);
+                    const bool pf = a_ > 0, nf = a_ < 0;
+                    const double x = pf ? bd_l : bu_l, y = pf ? bu_t : bd_t;
+                    const int w = pick(x, y);
+                    const double al =
+                        (pf || nf) ? (w == 0 ? 1.0 : (w == 1 ? x : y)) * a_ : 0.0;
                     fpart(c_l, ct, al, gl, gr, gv);
                     const double lf = lt - lo_l;
                     acc += gr * lf;
                     const double Lf = gv * lf;
-                    if (a_ > 0 && pick(bd_l, bu_t) == 2) su += a_ * Lf;
-                    if (a_ < 0 && pick(bu_l, bd_t) == 2) sd += a_ * Lf;
+                    const double sel = (w == 2 && (pf || nf)) ? a_ * Lf : 0.0;
+                    su += pf ? sel : 0.0;
+                    sd += nf ? sel : 0.0;
                 }
             }
             dacc = acc;
@@ -793,6 +795,17 @@ struct FaceK {
     double kp, km, ku1, kd1, ku2, kd2;
 };
 
+// 1/d for d >= DBL_MIN: hardware reciprocal estimate (relative error ~2^-24) and two
+// Newton-Raphson steps r <- r + r(1 - d r): about one ulp.
+__device__ __forceinline__ double recip(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    double e = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-d, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
 __device__ __forceinline__ FaceK face_k(double L, double a, double p, double m, double s1,
                                         double u1a, double u1b, double d1a, double d1b,
                                         double s2, double u2a, double u2b, double d2a,
@@ -800,16 +813,16 @@ __device__ __forceinline__ FaceK face_k(double L, double a, double p, double m, 
 {
     FaceK k;
     const double K = ab(a) * (1.0 - ab(a));
-    const double r0 = 1.0 / (p + m + MINV);  // (x r0) r0: no underflow of den^2
+    const double r0 = recip(p + m + MINV);  // (x r0) r0: no underflow of den^2
     k.kp = L * (K * (((2.0 * m + MINV) * r0) * r0));
     k.km = L * (-K * (((2.0 * p + MINV) * r0) * r0));
     const double gb = -0.5 * a;
     const double n1 = u1a + u1b - d1a - d1b, D1 = u1a + u1b + d1a + d1b + MINV;
-    const double r1 = 1.0 / D1;
+    const double r1 = recip(D1);
     k.ku1 = L * (gb * s1 * (((D1 - n1) * r1) * r1));
     k.kd1 = L * (gb * s1 * (((-D1 - n1) * r1) * r1));
     const double n2 = u2a + u2b - d2a - d2b, D2 = u2a + u2b + d2a + d2b + MINV;
-    const double r2 = 1.0 / D2;
+    const double r2 = recip(D2);
     k.ku2 = L * (gb * s2 * (((D2 - n2) * r2) * r2));
     k.kd2 = L * (gb * s2 * (((-D2 - n2) * r2) * r2));
     return k;
