@@ -94,6 +94,11 @@ int mpd_set_arith(mpd_grid g, int arith);
 /* Tile shape of the fused kernels (0,0 = the default 32x8); one of the compiled shapes
  * 32x8, 16x16, 64x8, 16x8, 32x16. */
 int mpd_set_tile(mpd_grid g, int tile_x, int tile_y);
+/* Whether mpd_step replays blocks of 8 steps as captured HIP graphs (PERIODIC grids on
+ * the fused path; default off: on MI355X the replays measure no faster than launching
+ * the kernels, whose gaps are the GPU's own dispatch latency).  The results are
+ * identical either way. */
+int mpd_set_graph(mpd_grid g, int on);
 /* Planes per workgroup of the fused kernels (0 = chosen from the grid size). */
 int mpd_set_zchunk(mpd_grid g, int zc);
 /* Times every tile shape x z-chunk length on the grid itself (one warm-up step and nsteps

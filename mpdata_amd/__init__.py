@@ -79,6 +79,7 @@ SIGNATURES = {
     "mpd_set_path": (C.c_int, [C.c_void_p, C.c_int]),
     "mpd_set_tile": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "mpd_set_zchunk": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpd_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "mpd_autotune": (C.c_int, [C.c_void_p, C.c_int, PI, PI, PI]),
     "mpd_set_arith": (C.c_int, [C.c_void_p, C.c_int]),
     "mpd_step_kernel_times": (C.c_int, [C.c_void_p, C.c_int, PD, C.c_int, PI]),
@@ -274,6 +275,9 @@ class Grid:
 
     def set_zchunk(self, zc):
         _check(lib().mpd_set_zchunk(self.h, zc))
+
+    def set_graph(self, on):
+        _check(lib().mpd_set_graph(self.h, int(on)))
 
     def autotune(self, nsteps=2):
         """Picks the fastest tile shape and z-chunk length for this grid (state kept);

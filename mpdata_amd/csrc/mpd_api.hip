@@ -368,16 +368,74 @@ extern "C" int mpd_set_emission(mpd_grid g, const double* q)
     return sync();
 }
 
+// Steps per captured graph: even, so a replay leaves the field parity unchanged.
+constexpr int kGraphSteps = 8;
+
+static bool graphable(const mpd_grid_s* g)
+{
+    return g->use_graph && g->boundary == MPD_PERIODIC && g->iord >= 2 && g->path != 1 &&
+           fused_plane_fits(g->nx, g->ny) && !g->trace;
+}
+
+// kGraphSteps steps as one graph launch.  The graph of the current parity is captured on
+// first use and again whenever the launch configuration has changed since (tile shape,
+// z-chunk, arithmetic, path, emission array); it references the grid's buffers, whose
+// contents may change freely between replays.  The steps launch kernels only (no memset
+// or copy nodes).
+static int graph_steps(mpd_grid_s* g)
+{
+    const mpd_grid_s::GraphKey key{g->tile_x, g->tile_y, g->zc, g->arith, g->path,
+                                   g->has_q ? g->q.p : nullptr};
+    const int par = g->cur;
+    hipStream_t s = stream();
+    if (!g->gexec[par] || !(g->gkey[par] == key)) {
+        if (g->gexec[par]) {
+            (void)hipGraphExecDestroy(g->gexec[par]);
+            g->gexec[par] = nullptr;
+        }
+        int rc = ensure_fused_scratch(g);  // allocations stay outside the capture
+        if (rc) return rc;
+        MPD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+        for (int i = 0; i < kGraphSteps && rc == MPD_OK; i++) rc = periodic_step(g);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &graph);
+        g->cur = par;  // the capture ran the host side of an even number of steps
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+        e = hipGraphInstantiate(&g->gexec[par], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            g->gexec[par] = nullptr;
+            return hip_fail(e, "hipGraphInstantiate");
+        }
+        g->gkey[par] = key;
+    }
+    MPD_HIP(hipGraphLaunch(g->gexec[par], s));
+    return MPD_OK;
+}
+
 extern "C" int mpd_step(mpd_grid g, int nsteps)
 {
     GRID_OR_FAIL(g);
     if (nsteps < 0) return fail(MPD_ERR_ARG, "mpd_step: nsteps %d < 0", nsteps);
     if (!g->has_winds || !g->has_field)
         return fail(MPD_ERR_ARG, "mpd_step: set the winds (or Courant numbers) and the field first");
-    for (int s = 0; s < nsteps; s++) {
-        int rc = step_once(g);
-        if (rc) return rc;
-    }
+    int done = 0, rc;
+    if (graphable(g))
+        for (; done + kGraphSteps <= nsteps; done += kGraphSteps)
+            if ((rc = graph_steps(g))) return rc;
+    for (; done < nsteps; done++)
+        if ((rc = step_once(g))) return rc;
+    return MPD_OK;
+}
+
+extern "C" int mpd_set_graph(mpd_grid g, int on)
+{
+    GRID_OR_FAIL(g);
+    g->use_graph = on != 0;
     return MPD_OK;
 }
 

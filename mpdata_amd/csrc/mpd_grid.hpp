@@ -35,6 +35,30 @@ struct mpd_grid_s {
     // trace_pairs set); the other paths record one event after each launch
     std::vector<hipEvent_t>* trace = nullptr;
     bool trace_pairs = false;
+
+    // time loop captured as a HIP graph (PERIODIC fused path): a block of steps from each
+    // field parity, with the launch configuration it was captured with
+    struct GraphKey {
+        int tx, ty, zc, arith, path;
+        const double* q;
+        bool operator==(const GraphKey& o) const
+        {
+            return tx == o.tx && ty == o.ty && zc == o.zc && arith == o.arith &&
+                   path == o.path && q == o.q;
+        }
+    };
+    bool use_graph = false;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    GraphKey gkey[2] = {};
+
+    mpd_grid_s() = default;
+    mpd_grid_s(const mpd_grid_s&) = delete;
+    mpd_grid_s& operator=(const mpd_grid_s&) = delete;
+    ~mpd_grid_s()
+    {
+        for (hipGraphExec_t& e : gexec)
+            if (e) (void)hipGraphExecDestroy(e);
+    }
 };
 
 namespace mpd {
