@@ -94,6 +94,8 @@ def main():
                     help="edge of the large single-grid roofline run (0: skip)")
     ap.add_argument("--adjoint", type=int, default=256,
                     help="edge of the source-inversion adjoint leg (0: skip)")
+    ap.add_argument("--adjoint-steps", type=int, default=512,
+                    help="time steps of the adjoint leg's loss window")
     ap.add_argument("--ring2d", type=int, default=128,
                     help="edge of the periodic 2-D ring leg, 1024 steps (0: skip)")
     ap.add_argument("--deform", type=int, default=1,
@@ -232,7 +234,7 @@ def main():
     if args.deform:
         line["deform"] = deform_leg(M)
     if args.adjoint:
-        line["adjoint"] = adjoint_leg(M, args.adjoint)
+        line["adjoint"] = adjoint_leg(M, args.adjoint, nsteps=args.adjoint_steps)
     print(json.dumps(line))
 
 
