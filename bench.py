@@ -375,7 +375,7 @@ def adjoint_leg(M, n, nsteps=512, every=16, K=32, nrec=64):
     twin.close()
     g = fresh()
     c0 = g.field()
-    J, g0, gq, (tf, tb) = g.receptor_gradient(nsteps, K, rec, every, obs)
+    J, g0, gq, (tf, tb, tr) = g.receptor_gradient(nsteps, K, rec, every, obs)
     g.close()
     d = np.random.default_rng(5).standard_normal(c0.size) * c0
 
@@ -428,6 +428,8 @@ def adjoint_leg(M, n, nsteps=512, every=16, K=32, nrec=64):
                         f"every {K}, iord 2 nonoscillatory",
             "J": J, "forward_ms_per_step": tf / nsteps, "backward_ms_per_step": tb / nsteps,
             "adjoint_over_forward": tb / tf,
+            "recompute_ms_per_step": tr / nsteps,
+            "adjoint_steps_only_over_forward": (tb - tr) / tf,
             "fd_limiter_on": fd_lim, "fd_limiter_off": fd_unlim, "cpu_reference": cpu,
             "grad_norms": {"c0": float(np.linalg.norm(g0)), "q": float(np.linalg.norm(gq))}}
 

@@ -157,7 +157,8 @@ int mpd_sample_receptors(mpd_grid g, int nsteps, int nrec, const int* receptors,
  * (rec: distinct cell indices).  The forward keeps c^n every checkpoint_every steps; the
  * backward sweep recomputes each segment from its checkpoint and applies the adjoint
  * steps in reverse.  grad_c0 = dJ/dc^0 and grad_q = dJ/dq (host arrays, may be NULL;
- * both NULL: loss only).  times (may be NULL): forward and backward milliseconds.  The
+ * both NULL: loss only).  times (may be NULL, 3 doubles): forward and backward
+ * milliseconds, and the part of the backward spent recomputing checkpoint segments.  The
  * forward runs in exact arithmetic; on return the grid holds c^nsteps. */
 int mpd_adjoint_run(mpd_grid g, int nsteps, int checkpoint_every, int nrec,
                     const int* receptors, int sample_every, const double* obs, double* J,

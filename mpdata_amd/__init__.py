@@ -319,18 +319,19 @@ class Grid:
 
     def receptor_gradient(self, nsteps, checkpoint_every, receptors, sample_every, obs,
                           grads=True):
-        """(J, dJ/dc0, dJ/dq, (forward_ms, backward_ms)) of the receptor-misfit loss from
-        the current field; the grid ends holding c^nsteps."""
+        """(J, dJ/dc0, dJ/dq, (forward_ms, backward_ms, recompute_ms)) of the
+        receptor-misfit loss from the current field (recompute_ms: the part of the backward
+        spent recomputing checkpoint segments); the grid ends holding c^nsteps."""
         rec = np.ascontiguousarray(receptors, dtype=np.int32)
         obs = np.ascontiguousarray(obs, dtype=np.float64).ravel()
         J = C.c_double(0)
         g0 = np.empty(self.n) if grads else None
         gq = np.empty(self.n) if grads else None
-        t = (C.c_double * 2)()
+        t = (C.c_double * 3)()
         _check(lib().mpd_adjoint_run(self.h, nsteps, checkpoint_every, len(rec),
                                      rec.ctypes.data_as(PI), sample_every, _p(obs),
                                      C.byref(J), _p(g0), _p(gq), t))
-        return J.value, g0, gq, (t[0], t[1])
+        return J.value, g0, gq, (t[0], t[1], t[2])
 
     def close(self):
         if self.h and self.owned:

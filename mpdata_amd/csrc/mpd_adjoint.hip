@@ -1265,8 +1265,9 @@ extern "C" int mpd_sample_receptors(mpd_grid g, int nsteps, int nrec, const int*
 // The forward keeps c^n every `checkpoint_every` steps; the backward sweep recomputes
 // each segment from its checkpoint and runs the adjoint steps in reverse.  grad_c0 /
 // grad_q (host, n cells each) may be NULL; with both NULL only J is computed.  On
-// return the grid holds c^nsteps.  times (optional, 2 doubles): forward and backward
-// milliseconds (HIP events).
+// return the grid holds c^nsteps.  times (optional, 3 doubles): forward and backward
+// milliseconds, and the part of the backward spent recomputing checkpoint segments
+// (HIP events).
 extern "C" int mpd_adjoint_run(mpd_grid g, int nsteps, int checkpoint_every, int nrec,
                                const int* receptors, int sample_every, const double* obs,
                                double* J, double* grad_c0, double* grad_q, double* times)
@@ -1363,6 +1364,17 @@ extern "C" int mpd_adjoint_run(mpd_grid g, int nsteps, int checkpoint_every, int
     for (size_t q = 0; q < (size_t)nsamp * nrec; q++) jj += 0.5 * res[q] * res[q];
     *J = jj;
 
+    // start / end of each segment's recompute
+    std::vector<hipEvent_t> rec_ev(want_grad ? 2 * nck : 0, nullptr);
+    struct FreeEv {
+        std::vector<hipEvent_t>& v;
+        ~FreeEv()
+        {
+            for (hipEvent_t e : v)
+                if (e) (void)hipEventDestroy(e);
+        }
+    } free_ev{rec_ev};
+    for (hipEvent_t& e : rec_ev) MPD_HIP(hipEventCreate(&e));
     MPD_HIP(hipEventRecord(e1b, s));
     if (want_grad) {
         MPD_HIP(hipMemsetAsync(lam.p, 0, n * 8, s));
@@ -1373,10 +1385,12 @@ extern "C" int mpd_adjoint_run(mpd_grid g, int nsteps, int checkpoint_every, int
             const int s0 = c * K, s1 = std::min(nsteps, s0 + K);
             // recompute c^{s0} .. c^{s1-1} into seg[0 .. s1-s0-1] with each step's
             // intermediates (the last step's result, c^{s1}, is not needed again)
+            MPD_HIP(hipEventRecord(rec_ev[2 * c], s));
             MPD_HIP(hipMemcpyAsync(seg[0].p, ck[c].p, n * 8, hipMemcpyDeviceToDevice, s));
             for (int m = 0; m < s1 - s0; m++)
                 if ((rc = fw[m].step(g, seg[m].p, m + 1 < s1 - s0 ? seg[m + 1].p : tmp.p)))
                     return rc;
+            MPD_HIP(hipEventRecord(rec_ev[2 * c + 1], s));
             for (int step = s1; step > s0; step--) {  // lam holds lam^{step}
                 if (step % sample_every == 0) {
                     // the residual needs c^{step}: c^{s1} is the next segment's first state
@@ -1417,6 +1431,13 @@ extern "C" int mpd_adjoint_run(mpd_grid g, int nsteps, int checkpoint_every, int
     if (times) {
         times[0] = f1;
         times[1] = f2;
+        double fr = 0.0;
+        for (size_t q = 0; q + 1 < rec_ev.size(); q += 2) {
+            float f = 0.f;
+            (void)hipEventElapsedTime(&f, rec_ev[q], rec_ev[q + 1]);
+            fr += f;
+        }
+        times[2] = fr;
     }
     return MPD_OK;
 }
