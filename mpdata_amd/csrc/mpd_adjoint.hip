@@ -428,7 +428,7 @@ k_adj_lim_t(LimArgs a)
     __shared__ int s_cd[2][R];
 
     const int t = threadIdx.x;
-    const int item = blockIdx.x;
+    const int item = tile::xcd_item(blockIdx.x, gridDim.x);
     const int bx = item % a.tiles_x, by = (item / a.tiles_x) % a.tiles_y;
     const int bz = item / (a.tiles_x * a.tiles_y);
     // own cell (tile: stages 1 and 2; ring 1: stage 1) and ring-2 cell (loads)
@@ -524,6 +524,8 @@ k_adj_lim_t(LimArgs a)
     double siu0 = 0, sid0 = 0, sli0 = 0, slx0 = 0, dacc_m = 0.0, lam_m[3] = {0, 0, 0};
     double a2_m2 = 0.0;
     int scd0 = 0;
+    // the emission adjoint of plane p, loaded one iteration ahead
+    double lq_p = 0.0;
 
     for (int pl = z0 - 1; pl <= z1; pl++) {
         const int p = __builtin_amdgcn_readfirstlane(pl);
@@ -627,8 +629,7 @@ k_adj_lim_t(LimArgs a)
             s_li[s2][L] = li;
             s_lx[s2][L] = lx;
         }
-        if (store && p >= z0 && p < z1 && a.lq)
-            st(a.lq + zp0.off, colb, ld(a.lq + zp0.off, colb) + P.lo[1]);
+        if (store && p >= z0 && p < z1 && a.lq) st(a.lq + zp0.off, colb, lq_p + P.lo[1]);
         tile::lds_barrier();
         // stage 2 on the tile cell of plane k = p - 1 (k_adj_limit3, same operation order)
         const int k = p - 1;
@@ -645,14 +646,16 @@ k_adj_lim_t(LimArgs a)
             // of 1 wrap a neighbour onto the cell)
             double s0 = 0.0, s1 = 0.0;
             const bool fx = a.nx == 1, fy = a.ny == 1, fz = a.nz == 1;
+            // (selects rather than branches: a skipped term adds zero, which changes
+            // nothing but the sign of a zero sum)
             auto bound = [&](int cq, double iuy, double idy, int q, bool folds) {
                 const int hc = cq & 0xff, lc = (cq >> 8) & 0xff;
-                if (hc >> 1 == q && !(folds && (hc & 1))) {
-                    if (hc & 1) s1 += iuy; else s0 += iuy;
-                }
-                if (lc >> 1 == q && !(folds && (lc & 1))) {
-                    if (lc & 1) s1 -= idy; else s0 -= idy;
-                }
+                const bool hm = hc >> 1 == q && !(folds && (hc & 1));
+                const bool lm = lc >> 1 == q && !(folds && (lc & 1));
+                s0 += (hm && !(hc & 1)) ? iuy : 0.0;
+                s1 += (hm && (hc & 1)) ? iuy : 0.0;
+                s0 -= (lm && !(lc & 1)) ? idy : 0.0;
+                s1 -= (lm && (lc & 1)) ? idy : 0.0;
             };
             bound(cd_k, iu_k, id_k, 0, true);
             bound(CD[L + 1], IU[L + 1], ID[L + 1], 1, fx);
@@ -739,6 +742,7 @@ k_adj_lim_t(LimArgs a)
             if (comp) load_next(P, colb, true, zp2, zp1);
             if (sec) load_next(H, scolb, false, zp2, zp1);
         }
+        if (store && p + 1 >= z0 && p + 1 < z1 && a.lq) lq_p = ld(a.lq + zp0.off, colb);
     }
 }
 
@@ -855,7 +859,7 @@ __global__ void __launch_bounds__((tile::GeoB<TX, TY>::NTH)) k_adj_anti_t(AntiAr
     __shared__ double s_k[14][R];
 
     const int t = threadIdx.x;
-    const int item = blockIdx.x;
+    const int item = tile::xcd_item(blockIdx.x, gridDim.x);
     const int bx = item % a.tiles_x, by = (item / a.tiles_x) % a.tiles_y;
     const int bz = item / (a.tiles_x * a.tiles_y);
     // own cell (tile: coefficients + gather; ring 1: coefficients) and ring-2 cell (loads)

@@ -102,7 +102,7 @@ __device__ __forceinline__ void f1_tile(const FusedArgs& a, int item)
     constexpr int RX = G::RX, R = G::R;
     __shared__ double s_c0[R], s_c1[3][R], s_u[3][R], s_v[3][R], s_w[R];
 
-    const int t = threadIdx.x;
+    const int t = G::role(item);
     int bx, by, bz;
     tile_of(a, item, bx, by, bz);
     // own tile cell
@@ -294,15 +294,17 @@ __device__ __forceinline__ void f1_tile(const FusedArgs& a, int item)
 // and ring 1 -> barrier -> limited donor pass of plane k = p - 1 on the tile.
 // LDS slots: c^n, Ax and Ay one plane each (read only between the two barriers that
 // follow the write; the two face values the donor pass needs a plane later are carried
-// in registers), cp three planes, the ratios two.
-template <int TX, int TY>
+// in registers), cp three planes, the ratios two.  SAVE also stores the ratios, the
+// flux sums and the bound codes of every plane (the adjoint's forward record); it is a
+// separate instantiation so the plain step does not carry its registers.
+template <int TX, int TY, bool SAVE>
 __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
 {
     using G = Geo<TX, TY>;
     constexpr int RX = G::RX, R = G::R;
     __shared__ double s_cn[R], s_cp[3][R], s_ax[R], s_ay[R], s_bu[2][R], s_bd[2][R];
 
-    const int t = threadIdx.x;
+    const int t = G::role(item);
     int bx, by, bz;
     tile_of(a, item, bx, by, bz);
     const int rx = 2 + t % TX, ry = 2 + t / TX;
@@ -328,6 +330,7 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
     // halo cell
     double hcnm = 0, hcnc = 0, hcpm = 0, hcpc = 0, hazm = 0;
     double hncn = 0, hncp = 0, hnax = 0, hnay = 0, hnaz = 0;
+    double nq = 0;  // emission of plane p, loaded in iteration p for plane k = p of p + 1
     cnm = ld(a.cn + zm1.off, colb);
     cnc = ld(a.cn + zp0.off, colb);
     cpm = ld(a.cp + zm1.off, colb);
@@ -357,6 +360,8 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
         const int s2 = (p + 2) % 2, s3 = (p + 3) % 3;
         const double cnn = ncn, cpn = ncp, axn = nax, ayn = nay, azc = naz;
         const double hcnn = hncn, hcpn = hncp, haxn = hnax, hayn = hnay, hazc = hnaz;
+        const double qk = nq;
+        if (a.q && p >= z0 && p < z1) nq = ld(a.q + zp0.off, colb);
         if (p < z1) {  // the next iteration's values: fields at p + 2, faces at p + 1
             ncn = ld(a.cn + zp2.off, colb);
             ncp = ld(a.cp + zp2.off, colb);
@@ -398,35 +403,31 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
                 hi = mx2(hi, mx2(nbn[m], nbp[m]));
                 lo = mn2(lo, mn2(nbn[m], nbp[m]));
             }
+            if (code) {
+                // which element holds each bound: element 2m + s is nbn[m] (s = 0) or
+                // nbp[m] (s = 1), and of the elements equal to the bound the last one
+                // counts (the convention of a running maximum that replaces on ties)
+                int hc = 0, lc = 0;
+#pragma unroll
+                for (int m = 0; m < 7; m++) {
+                    hc = nbn[m] == hi ? 2 * m : hc;
+                    lc = nbn[m] == lo ? 2 * m : lc;
+                    hc = nbp[m] == hi ? 2 * m + 1 : hc;
+                    lc = nbp[m] == lo ? 2 * m + 1 : lc;
+                }
+                *code = hc | (lc << 8);
+            }
             const double fxl = 0.5 * F2(nbp[1], c, axl), fxr = 0.5 * F2(c, nbp[2], axr);
             const double fyl = 0.5 * F2(nbp[3], c, ayl), fyr = 0.5 * F2(c, nbp[4], ayr);
             const double fzl = 0.5 * F2(cplo, c, azlo), fzr = 0.5 * F2(c, cphi, azhi);
             beta(hi, lo, c, fxl, fxr, fyl, fyr, fzl, fzr, bu, bd, ain, aout);
-            if (code) {  // which element holds each bound (later elements win ties)
-                double h = cn0, l = cn0;
-                int hc = 0, lc = 0;
-#pragma unroll
-                for (int m = 0; m < 7; m++) {
-                    const double x = nbn[m], y = nbp[m];
-                    const double mh = x > y ? x : y, ml = x < y ? x : y;
-                    if (!(h > mh)) {
-                        h = mh;
-                        hc = 2 * m + (x > y ? 0 : 1);
-                    }
-                    if (!(l < ml)) {
-                        l = ml;
-                        lc = 2 * m + (x < y ? 0 : 1);
-                    }
-                }
-                *code = hc | (lc << 8);
-            }
         };
         const double axl = s_ax[L - 1], ayd = s_ay[L - RX];
         double buc, bdc, ain, aout;
         int code = 0;
         ratios(L, cnc, cnm, cnn, cpc, cpm, cpn, axn, ayn, azm, azc, axl, ayd, buc, bdc, ain, aout,
-               a.code ? &code : nullptr);
-        if (a.bu && store && p >= z0 && p < z1) {
+               SAVE ? &code : nullptr);
+        if (SAVE && store && p >= z0 && p < z1) {
             st(a.bu + zp0.off, colb, buc);
             st(a.bd + zp0.off, colb, bdc);
             st(a.ain + zp0.off, colb, ain);
@@ -459,7 +460,7 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
             double r = donor(cpm, cq[L - 1], cq[L + 1], cq[L - RX], cq[L + RX], cpm2, cpc, uxl,
                              uxr, vyl, vyr, wzl, wzr);
             if (store) {
-                if (a.q) r = r + ld(a.q + zm1.off, colb);
+                if (a.q) r = r + qk;
                 st(a.cout + zm1.off, colb, r);
                 if (a.Lx) {
                     st(a.Lx + zm1.off, colb, uxr);
@@ -516,8 +517,9 @@ k_f1(FusedArgs a, Work w)
     }
 }
 
-template <int TX, int TY>
-__global__ void __launch_bounds__((Geo<TX, TY>::NT)) k_f2(FusedArgs a, Work w)
+template <int TX, int TY, bool SAVE>
+__global__ void __launch_bounds__((Geo<TX, TY>::NT)) __attribute__((amdgpu_waves_per_eu(SAVE ? 3 : 1)))
+k_f2(FusedArgs a, Work w)
 {
     for (int it = blockIdx.x; it < w.nitems; it += gridDim.x) {
         int item = item_of(w, it);
@@ -528,7 +530,7 @@ __global__ void __launch_bounds__((Geo<TX, TY>::NT)) k_f2(FusedArgs a, Work w)
             __builtin_memcpy(&x, (const FusedArgs*)((ConstArgs*)w.table + i), sizeof x);
             item -= item0[i];
         }
-        f2_tile<TX, TY>(x, item);
+        f2_tile<TX, TY, SAVE>(x, item);
         lds_barrier();
     }
 }
@@ -579,8 +581,10 @@ int launch_pair(const FusedArgs& a, const Work& w, int grid, bool f1, bool donor
             launch_k(k_f1<TX, TY, true>, dim3(grid), dim3(G::NT), s, ev, a, w);
         else
             launch_k(k_f1<TX, TY, false>, dim3(grid), dim3(G::NT), s, ev, a, w);
+    } else if (a.bu) {
+        launch_k(k_f2<TX, TY, true>, dim3(grid), dim3(G::NT), s, ev, a, w);
     } else {
-        launch_k(k_f2<TX, TY>, dim3(grid), dim3(G::NT), s, ev, a, w);
+        launch_k(k_f2<TX, TY, false>, dim3(grid), dim3(G::NT), s, ev, a, w);
     }
     MPD_HIP(hipGetLastError());
     return MPD_OK;
@@ -648,7 +652,8 @@ int MPD_CAT(fused_resident_blocks, MPD_ARITH)(int tx, int ty, bool f1)
     if (tx == X && ty == Y)                                                                 \
         e = f1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_f1<X, Y, true>,         \
                                                              Geo<X, Y>::NT, 0)              \
-               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_f2<X, Y>, Geo<X, Y>::NT, 0);
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_f2<X, Y, false>,        \
+                                                             Geo<X, Y>::NT, 0);
     MPD_FUSED_TILES(MPD_TILE)
 #undef MPD_TILE
     return e == hipSuccess ? n : 0;

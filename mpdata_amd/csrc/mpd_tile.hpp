@@ -16,6 +16,15 @@ struct Geo {
     static constexpr int H1 = 2 * (TX + 2) + 2 * TY;    // ring-1 cells
     static constexpr int NH = R - NT;                   // halo cells, rings 1 and 2
     static_assert(NH <= NT, "every thread owns at most one halo cell");
+    static_assert(NT % 64 == 0, "whole waves");
+    // Role of this thread in work item `item`: the thread numbering rotated by whole
+    // waves from one item to the next.  The ring-1 work falls on the lowest-numbered
+    // roles, so without the rotation the same waves of every workgroup -- and the same
+    // SIMDs of a CU -- would carry it.
+    __device__ static int role(int item)
+    {
+        return (int)((threadIdx.x + 64u * (unsigned)(item % (NT / 64))) % (unsigned)NT);
+    }
     // Region cell (rx, ry) of halo cell h: ring 1 first (the rows above and below the
     // tile, then the two side columns), then ring 2 in the same order, so that the
     // threads with ring-1 work are the lowest-numbered and fill as few waves as possible.
@@ -94,6 +103,16 @@ __device__ __forceinline__ void lds_barrier()
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
     __builtin_amdgcn_s_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Work item of workgroup b of a launch of n workgroups, one item each.  Workgroups go
+// to the 8 XCDs round robin (b % 8); each XCD is dealt one contiguous range of the item
+// list (the first n % 8 ranges one item longer), so tiles that share halo rows run in
+// the same L2.  A bijection of [0, n) for every n.
+__device__ __forceinline__ int xcd_item(int b, int n)
+{
+    const int x = b % 8, j = b / 8, q = n / 8, r = n % 8;
+    return x * q + min(x, r) + j;
 }
 
 __device__ __forceinline__ int wrapi(int a, int n)
