@@ -162,7 +162,7 @@ __device__ __forceinline__ double limited(double a, double bu_l, double bd_l, do
 // ------------------------------------------------------------------ adjoint parts ---
 
 struct AdjBufs {
-    const double *U, *V, *W, *c0, *c1, *A0, *A1, *A2, *bup, *bdn, *ain, *aout;
+    const double *U, *V, *W, *c0, *c1, *A0, *A1, *A2, *bup, *bdn;
     const int* code;
     int nonosc;
 };
@@ -207,11 +207,13 @@ __global__ void k_adj_donor_limit(Geo g, AdjBufs b, const double* __restrict__ l
         bd_t = b.bdn[t];
     }
     double acc = lt, su = 0.0, sd = 0.0;
+    double fl[3], fu[3];  // the unlimited donor fluxes of the lower and upper faces
     for (int d = 0; d < 3; d++) {
         double gl, gr, gv;
         {  // upper face (t, r): t is the lower cell
             const size_t r = up[d];
             const double a = fam_a(b, d, t);
+            fu[d] = 0.5 * F2(ct, b.c1[r], a);
             double au = a, m = 1.0;
             if (b.nonosc) {
                 const double bu_r = b.bup[r], bd_r = b.bdn[r];
@@ -239,6 +241,7 @@ __global__ void k_adj_donor_limit(Geo g, AdjBufs b, const double* __restrict__ l
         {  // lower face (l, t): t is the upper cell
             const size_t l = lo[d];
             const double a = fam_a(b, d, l);
+            fl[d] = 0.5 * F2(b.c1[l], ct, a);
             double al = a;
             double bu_l = 0.0, bd_l = 0.0;
             if (b.nonosc) {
@@ -260,14 +263,17 @@ __global__ void k_adj_donor_limit(Geo g, AdjBufs b, const double* __restrict__ l
     if (lq) lq[t] += lt;
     if (!b.nonosc) return;
     // ratio adjoints -> bound / flux-sum adjoints (pointwise).  A ratio that was not
-    // selected carries no adjoint and may be inf: skipped.
+    // selected carries no adjoint and may be inf: skipped.  The flux sums are those the
+    // forward formed from the same six fluxes.
+    double ain, aout;
+    flux_sums(fl[0], fu[0], fl[1], fu[1], fl[2], fu[2], ain, aout);
     double u = 0.0, dn = 0.0, li = 0.0, lo2 = 0.0;
     if (su != 0.0) {
-        u = su / (b.ain[t] + MINV);
+        u = su / (ain + MINV);
         li = -u * bu_t;
     }
     if (sd != 0.0) {
-        dn = sd / (b.aout[t] + MINV);
+        dn = sd / (aout + MINV);
         lo2 = -dn * bd_t;
     }
     iu[t] = u;
@@ -396,7 +402,7 @@ struct LimArgs {
     int nx, ny, nz;
     long long np;
     int tiles_x, tiles_y, zc;
-    const double *lout, *c1, *A0, *A1, *A2, *bup, *bdn, *ain, *aout;
+    const double *lout, *c1, *A0, *A1, *A2, *bup, *bdn;
     const int* code;
     double *l1, *lA0, *lA1, *lA2, *lin, *lq;
 };
@@ -404,9 +410,9 @@ struct LimArgs {
 // own-column values of one region cell: planes p-1, p, p+1 of lout, c1, the ratios; the
 // z-face velocity below (p-1) and above (p); plane p of the rest; n*: the next plane
 struct LimCol {
-    double lo[3], c1[3], bu[3], bd[3], a2m, a2c, a0, a1, ain, aout;
+    double lo[3], c1[3], bu[3], bd[3], a2m, a2c, a0, a1;
     int cd;
-    double nlo, nc1, nbu, nbd, na0, na1, na2, nain, naout;
+    double nlo, nc1, nbu, nbd, na0, na1, na2;
     int ncd;
 };
 
@@ -427,8 +433,8 @@ k_adj_lim_t(LimArgs a)
     __shared__ double s_iu[2][R], s_id[2][R], s_li[2][R], s_lx[2][R];
     __shared__ int s_cd[2][R];
 
-    const int t = threadIdx.x;
     const int item = tile::xcd_item(blockIdx.x, gridDim.x);
+    const int t = G::role(item);
     const int bx = item % a.tiles_x, by = (item / a.tiles_x) % a.tiles_y;
     const int bz = item / (a.tiles_x * a.tiles_y);
     // own cell (tile: stages 1 and 2; ring 1: stage 1) and ring-2 cell (loads)
@@ -458,8 +464,6 @@ k_adj_lim_t(LimArgs a)
         q.na1 = ld(a.A1 + zc1.off, cb);
         if (full) {
             q.na2 = ld(a.A2 + zc1.off, cb);
-            q.nain = ld(a.ain + zc1.off, cb);
-            q.naout = ld(a.aout + zc1.off, cb);
             q.ncd = ld_code(a.code + zc1.off, cb);
         }
     };
@@ -474,13 +478,11 @@ k_adj_lim_t(LimArgs a)
         q.bd[1] = ld(a.bdn + zp0.off, cb);
         q.a0 = ld(a.A0 + zp0.off, cb);
         q.a1 = ld(a.A1 + zp0.off, cb);
-        q.a2m = q.a2c = q.ain = q.aout = 0.0;
+        q.a2m = q.a2c = 0.0;
         q.cd = 0;
         if (full) {
             q.a2m = ld(a.A2 + zm1.off, cb);
             q.a2c = ld(a.A2 + zp0.off, cb);
-            q.ain = ld(a.ain + zp0.off, cb);
-            q.aout = ld(a.aout + zp0.off, cb);
             q.cd = ld_code(a.code + zp0.off, cb);
         }
         // plane p + 1 of the first iteration, then the prefetch of the second
@@ -507,8 +509,6 @@ k_adj_lim_t(LimArgs a)
         q.a2c = q.na2;
         q.a0 = q.na0;
         q.a1 = q.na1;
-        q.ain = q.nain;
-        q.aout = q.naout;
         q.cd = q.ncd;
     };
     LimCol P, H;
@@ -554,6 +554,7 @@ k_adj_lim_t(LimArgs a)
             const double* C = s_c[s3];
             const double lt = q.lo[1], ct = q.c1[1], bu_t = q.bu[1], bd_t = q.bd[1];
             double acc = lt, su = 0.0, sd = 0.0;
+            double fl[3], fu[3];  // the unlimited donor fluxes of the lower / upper faces
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 double gl, gr, gv;
@@ -568,6 +569,7 @@ k_adj_lim_t(LimArgs a)
                     }
                     // limiter multiplier without branches: the ratio pair of the
                     // flow direction, the selection of mn2(1, mn2(x, y))
+                    fu[d] = 0.5 * F2(ct, c_r, a_);
                     const bool pf = a_ > 0, nf = a_ < 0;
                     const double x = pf ? bd_t : bu_t, y = pf ? bu_r : bd_r;
                     const int w = pick(x, y);
@@ -591,6 +593,7 @@ k_adj_lim_t(LimArgs a)
                     } else {
                         a_ = q.a2m; lo_l = q.lo[0]; c_l = s_c[k3][M]; bu_l = q.bu[0]; bd_l = q.bd[0];
                     }
+                    fl[d] = 0.5 * F2(c_l, ct, a_);
                     const bool pf = a_ > 0, nf = a_ < 0;
                     const double x = pf ? bd_l : bu_l, y = pf ? bu_t : bd_t;
                     const int w = pick(x, y);
@@ -606,13 +609,16 @@ k_adj_lim_t(LimArgs a)
                 }
             }
             dacc = acc;
+            // the flux sums of the forward, formed again from the same six fluxes
+            double ain, aout;
+            flux_sums(fl[0], fu[0], fl[1], fu[1], fl[2], fu[2], ain, aout);
             iu = id = li = lx = 0.0;
             if (su != 0.0) {
-                iu = su / (q.ain + MINV);
+                iu = su / (ain + MINV);
                 li = -iu * bu_t;
             }
             if (sd != 0.0) {
-                id = sd / (q.aout + MINV);
+                id = sd / (aout + MINV);
                 lx = -id * bd_t;
             }
         };
@@ -688,9 +694,10 @@ k_adj_lim_t(LimArgs a)
                     } else {
                         a_ = P.a2m; c_r = s_c[s3][L]; li_r = s_li[s2][L]; lx_r = s_lx[s2][L];
                     }
+                    // (lx_t [f>0] - li_t [f<0]) + (li_r [f>0] - lx_r [f<0]), the zero
+                    // terms dropped (the same value up to the sign of a zero)
                     const double f = 0.5 * F2(ct, c_r, a_);
-                    const double lf = (f > 0 ? lx_t : 0.0) - (f < 0 ? li_t : 0.0) +
-                                      (f > 0 ? li_r : 0.0) - (f < 0 ? lx_r : 0.0);
+                    const double lf = f > 0 ? lx_t + li_r : (f < 0 ? -(li_t + lx_r) : 0.0);
                     fpart(ct, c_r, a_, gl, gr, gv);
                     acc += gl * lf;
                     lAf[d] = lam_m[d] + gv * lf;
@@ -705,8 +712,7 @@ k_adj_lim_t(LimArgs a)
                         a_ = a2_m2; c_l = s_c[(p + 1) % 3][L]; li_l = sli0; lx_l = slx0;
                     }
                     const double f = 0.5 * F2(c_l, ct, a_);
-                    const double lf = (f > 0 ? lx_l : 0.0) - (f < 0 ? li_l : 0.0) +
-                                      (f > 0 ? li_t : 0.0) - (f < 0 ? lx_t : 0.0);
+                    const double lf = f > 0 ? lx_l + li_t : (f < 0 ? -(li_l + lx_t) : 0.0);
                     fpart(c_l, ct, a_, gl, gr, gv);
                     acc += gr * lf;
                 }
@@ -746,7 +752,7 @@ k_adj_lim_t(LimArgs a)
     }
 }
 
-int lim_tiled(const mpd_grid_s* g, const double* const in[9], const int* code,
+int lim_tiled(const mpd_grid_s* g, const double* const in[6], const int* code,
               const double* lout, double* l1, double* const lA[3], double* lin, double* lq,
               hipStream_t s)
 {
@@ -770,8 +776,6 @@ int lim_tiled(const mpd_grid_s* g, const double* const in[9], const int* code,
     a.A2 = in[3];
     a.bup = in[4];
     a.bdn = in[5];
-    a.ain = in[6];
-    a.aout = in[7];
     a.code = code;
     a.l1 = l1;
     a.lA0 = lA[0];
@@ -858,8 +862,8 @@ __global__ void __launch_bounds__((tile::GeoB<TX, TY>::NTH)) k_adj_anti_t(AntiAr
     __shared__ double s_c[3][R], s_u[2][R], s_v[2][R], s_w[2][R];
     __shared__ double s_k[14][R];
 
-    const int t = threadIdx.x;
     const int item = tile::xcd_item(blockIdx.x, gridDim.x);
+    const int t = G::role(item);
     const int bx = item % a.tiles_x, by = (item / a.tiles_x) % a.tiles_y;
     const int bz = item / (a.tiles_x * a.tiles_y);
     // own cell (tile: coefficients + gather; ring 1: coefficients) and ring-2 cell (loads)
@@ -1024,7 +1028,7 @@ __global__ void __launch_bounds__((tile::GeoB<TX, TY>::NTH)) k_adj_anti_t(AntiAr
 int anti_tiled(const mpd_grid_s* g, const double* c1, double* const lA[3], double* l1,
                hipStream_t s)
 {
-    constexpr int TX = 16, TY = 12;
+    constexpr int TX = 16, TY = 10;
     AntiArgs a{};
     a.nx = g->nx;
     a.ny = g->ny;
@@ -1085,7 +1089,7 @@ __global__ void k_receptors(const double* __restrict__ c, const int* __restrict_
 
 // Forward intermediates of one step, stored by the fused forward kernels.
 struct Fwd {
-    DevBuf c1, A[3], bup, bdn, ain, aout;
+    DevBuf c1, A[3], bup, bdn;
     int* code = nullptr;
     Fwd() = default;
     Fwd(const Fwd&) = delete;
@@ -1097,8 +1101,8 @@ struct Fwd {
     int alloc(size_t cells, bool nonosc)
     {
         int rc;
-        DevBuf* all[] = {&c1, &A[0], &A[1], &A[2], &bup, &bdn, &ain, &aout};
-        for (int i = 0; i < (nonosc ? 8 : 4); i++)
+        DevBuf* all[] = {&c1, &A[0], &A[1], &A[2], &bup, &bdn};
+        for (int i = 0; i < (nonosc ? 6 : 4); i++)
             if ((rc = all[i]->alloc(cells))) return rc;
         if (nonosc) MPD_HIP(hipMalloc((void**)&code, cells * sizeof(int)));
         return MPD_OK;
@@ -1107,7 +1111,7 @@ struct Fwd {
     int step(mpd_grid_s* g, const double* cin, double* cout)
     {
         double* const A3[3] = {A[0].p, A[1].p, A[2].p};
-        return fused_store_step_exact(g, cin, cout, c1.p, A3, bup.p, bdn.p, ain.p, aout.p, code);
+        return fused_store_step_exact(g, cin, cout, c1.p, A3, bup.p, bdn.p, code);
     }
 };
 
@@ -1133,10 +1137,9 @@ int adjoint_step(mpd_grid_s* g, Work& w, const Fwd& f, const double* cin, const 
     const unsigned nb = nblk(g->ncell);
     const double *U = g->face[0].p, *V = g->face[1].p, *W = g->face[2].p;
     AdjBufs b{U, V, W, cin, f.c1.p, f.A[0].p, f.A[1].p, f.A[2].p, f.bup.p, f.bdn.p,
-              f.ain.p, f.aout.p, f.code, g->nonosc};
+              f.code, g->nonosc};
     if (g->nonosc && g->adj_tiled) {
-        const double* in[9] = {f.c1.p, f.A[0].p, f.A[1].p, f.A[2].p, f.bup.p, f.bdn.p, f.ain.p,
-                               f.aout.p, nullptr};
+        const double* in[6] = {f.c1.p, f.A[0].p, f.A[1].p, f.A[2].p, f.bup.p, f.bdn.p};
         double* const lA[3] = {w.lA[0].p, w.lA[1].p, w.lA[2].p};
         int rc;
         if ((rc = lim_tiled(g, in, f.code, lout, w.l1.p, lA, lin, lq, s))) return rc;

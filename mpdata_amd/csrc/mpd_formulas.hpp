@@ -92,23 +92,26 @@ __device__ __forceinline__ double limit_face(double u, double bup_l, double bdn_
     return mn2(1.0, mn2(bdn_l, bup_r)) * pos(u) + mn2(1.0, mn2(bup_l, bdn_r)) * neg(u);
 }
 
-// Limiter ratios of one cell from the bounds hi/lo, its value c and the six donor
-// fluxes (0.5 F2) of the pass: bup = (hi - c)/(Ain + MINV), bdn = (c - lo)/(Aout + MINV).
-__device__ __forceinline__ void beta(double hi, double lo, double c, double fxl, double fxr,
-                                     double fyl, double fyr, double fzl, double fzr,
-                                     double& bup, double& bdn, double& ain, double& aout)
+// Inflow and outflow sums of one cell from its six donor fluxes (0.5 F2; lower and upper
+// face of x, y, z).  The adjoint recomputes them with this same function, so its values
+// are those of the forward bit for bit.
+__device__ __forceinline__ void flux_sums(double fxl, double fxr, double fyl, double fyr,
+                                          double fzl, double fzr, double& ain, double& aout)
 {
     ain = ((pos(fxl) - neg(fxr)) + (pos(fyl) - neg(fyr))) + (pos(fzl) - neg(fzr));
     aout = ((pos(fxr) - neg(fxl)) + (pos(fyr) - neg(fyl))) + (pos(fzr) - neg(fzl));
-    bup = dv(hi - c, ain + MINV);
-    bdn = dv(c - lo, aout + MINV);
 }
+
+// Limiter ratios of one cell from the bounds hi/lo, its value c and the six donor
+// fluxes of the pass: bup = (hi - c)/(Ain + MINV), bdn = (c - lo)/(Aout + MINV).
 __device__ __forceinline__ void beta(double hi, double lo, double c, double fxl, double fxr,
                                      double fyl, double fyr, double fzl, double fzr,
                                      double& bup, double& bdn)
 {
     double ain, aout;
-    beta(hi, lo, c, fxl, fxr, fyl, fyr, fzl, fzr, bup, bdn, ain, aout);
+    flux_sums(fxl, fxr, fyl, fyr, fzl, fzr, ain, aout);
+    bup = dv(hi - c, ain + MINV);
+    bdn = dv(c - lo, aout + MINV);
 }
 
 }  // namespace MPD_ARITH

@@ -391,8 +391,7 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
         const double* cpp = s_cp[s3];
         auto ratios = [&](int M, double cn0, double cnlo, double cnhi, double c, double cplo,
                           double cphi, double axr, double ayr, double azlo, double azhi,
-                          double axl, double ayl, double& bu, double& bd, double& ain,
-                          double& aout, int* code) {
+                          double axl, double ayl, double& bu, double& bd, int* code) {
             double hi = cn0, lo = cn0;
             const double nbn[7] = {cn0, s_cn[M - 1], s_cn[M + 1], s_cn[M - RX], s_cn[M + RX],
                                    cnlo, cnhi};
@@ -420,26 +419,24 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
             const double fxl = 0.5 * F2(nbp[1], c, axl), fxr = 0.5 * F2(c, nbp[2], axr);
             const double fyl = 0.5 * F2(nbp[3], c, ayl), fyr = 0.5 * F2(c, nbp[4], ayr);
             const double fzl = 0.5 * F2(cplo, c, azlo), fzr = 0.5 * F2(c, cphi, azhi);
-            beta(hi, lo, c, fxl, fxr, fyl, fyr, fzl, fzr, bu, bd, ain, aout);
+            beta(hi, lo, c, fxl, fxr, fyl, fyr, fzl, fzr, bu, bd);
         };
         const double axl = s_ax[L - 1], ayd = s_ay[L - RX];
-        double buc, bdc, ain, aout;
+        double buc, bdc;
         int code = 0;
-        ratios(L, cnc, cnm, cnn, cpc, cpm, cpn, axn, ayn, azm, azc, axl, ayd, buc, bdc, ain, aout,
+        ratios(L, cnc, cnm, cnn, cpc, cpm, cpn, axn, ayn, azm, azc, axl, ayd, buc, bdc,
                SAVE ? &code : nullptr);
         if (SAVE && store && p >= z0 && p < z1) {
             st(a.bu + zp0.off, colb, buc);
             st(a.bd + zp0.off, colb, bdc);
-            st(a.ain + zp0.off, colb, ain);
-            st(a.aout + zp0.off, colb, aout);
             *(int*)((char*)(a.code + zp0.off) + colb / 2) = code;
         }
         s_bu[s2][L] = buc;
         s_bd[s2][L] = bdc;
         if (h1) {
-            double hbu, hbd, hain, haout;
+            double hbu, hbd;
             ratios(HL, hcnc, hcnm, hcnn, hcpc, hcpm, hcpn, haxn, hayn, hazm, hazc,
-                   s_ax[HL - 1], s_ay[HL - RX], hbu, hbd, hain, haout, nullptr);
+                   s_ax[HL - 1], s_ay[HL - RX], hbu, hbd, nullptr);
             s_bu[s2][HL] = hbu;
             s_bd[s2][HL] = hbd;
         }
@@ -661,7 +658,7 @@ int MPD_CAT(fused_resident_blocks, MPD_ARITH)(int tx, int ty, bool f1)
 
 int MPD_CAT(fused_store_step, MPD_ARITH)(mpd_grid_s* g, const double* cin, double* cout,
                                          double* c1, double* const A[3], double* bu,
-                                         double* bd, double* ain, double* aout, int* code)
+                                         double* bd, int* code)
 {
     hipStream_t s = stream();
     int rc;
@@ -689,8 +686,6 @@ int MPD_CAT(fused_store_step, MPD_ARITH)(mpd_grid_s* g, const double* cin, doubl
         a.cout = cout;
         a.bu = bu;
         a.bd = bd;
-        a.ain = ain;
-        a.aout = aout;
         a.code = code;
         return launch(a, tx, ty, false, false, s, nullptr);
     }

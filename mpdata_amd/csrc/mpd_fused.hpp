@@ -27,10 +27,10 @@ struct FusedArgs {
     double* cout;             // pass result
     double *Lx, *Ly, *Lz;     // limited faces or null
     // F2, optional (the adjoint's stored forward): limiter data of the tile cells -- the
-    // ratios, the flux sums and which neighbourhood element holds the bounds
+    // ratios and which neighbourhood element holds the bounds
     // (code = hc | lc << 8, hc / lc = 2*q + field, q over {self, x-, x+, y-, y+, z-, z+},
     // field 0 = c^n, 1 = pass input; ties go to the later element)
-    double *bu, *bd, *ain, *aout;
+    double *bu, *bd;
     int* code;
 };
 
@@ -58,8 +58,7 @@ int fused_batch_launch_fast(int tx, int ty, bool f1, bool donor, const FusedArgs
 // forward's intermediates for the adjoint: the donor field c1, the antidiffusive faces
 // A[3] and, with the limiter, its data (see FusedArgs; null pointers when unlimited).
 int fused_store_step_exact(mpd_grid_s* g, const double* cin, double* cout, double* c1,
-                           double* const A[3], double* bu, double* bd, double* ain,
-                           double* aout, int* code);
+                           double* const A[3], double* bu, double* bd, int* code);
 // Resident workgroups per CU of the F1 (donor) or F2 kernel of a tile shape (0: unknown
 // shape).
 int fused_resident_blocks_exact(int tx, int ty, bool f1);

@@ -16,13 +16,13 @@ struct Geo {
     static constexpr int H1 = 2 * (TX + 2) + 2 * TY;    // ring-1 cells
     static constexpr int NH = R - NT;                   // halo cells, rings 1 and 2
     static_assert(NH <= NT, "every thread owns at most one halo cell");
-    static_assert(NT % 64 == 0, "whole waves");
     // Role of this thread in work item `item`: the thread numbering rotated by whole
     // waves from one item to the next.  The ring-1 work falls on the lowest-numbered
     // roles, so without the rotation the same waves of every workgroup -- and the same
     // SIMDs of a CU -- would carry it.
     __device__ static int role(int item)
     {
+        static_assert(NT % 64 == 0, "whole waves");
         return (int)((threadIdx.x + 64u * (unsigned)(item % (NT / 64))) % (unsigned)NT);
     }
     // Region cell (rx, ry) of halo cell h: ring 1 first (the rows above and below the
@@ -66,6 +66,13 @@ struct GeoB {
     static constexpr int H2 = R - NT - H1;                  // ring-2 cells
     static constexpr int SPARE = NTH - NT - H1;
     static_assert(H2 <= SPARE + NT, "every ring-2 cell has a thread");
+    // Role of this thread in work item `item`: the numbering rotated by whole waves
+    // from item to item, so that the wave left without a tile cell -- idle in the
+    // tile-only stages -- is a different one in the workgroups that share a CU.
+    __device__ static int role(int item)
+    {
+        return (int)((threadIdx.x + 64u * (unsigned)(item % (NTH / 64))) % (unsigned)NTH);
+    }
     // own cell of thread t: 0 = tile cell, 1 = ring-1 cell, -1 = none
     __device__ static int own(int t, int& rx, int& ry)
     {
