@@ -1056,23 +1056,50 @@ int anti_tiled(const mpd_grid_s* g, const double* c1, double* const lA[3], doubl
     return MPD_OK;
 }
 
-// donor pass transposed: lin += l1 + gather over the Courant faces
-__global__ void k_adj_donor1(Geo g, AdjBufs b, const double* __restrict__ l1,
-                             double* __restrict__ lin)
+// donor pass transposed: lin += l1 + gather over the Courant faces.  One thread per
+// (x, y) column and z-chunk (workgroups of 64 x 4 columns, coalesced along x): the own
+// column's l1 below / above and W below are carried from plane to plane, the in-plane
+// neighbours are cached loads.  Per face (x, y, z): the upper face's outflow term, then
+// the lower face's, in that order.
+constexpr int D1X = 64, D1Y = 4;
+__global__ void __launch_bounds__(D1X * D1Y)
+k_adj_donor1(int nx, int ny, int nz, int zc, const double* __restrict__ U,
+             const double* __restrict__ V, const double* __restrict__ W,
+             const double* __restrict__ l1, double* __restrict__ lin)
 {
-    size_t t = (size_t)blockIdx.x * TPB + threadIdx.x;
-    if (t >= g.n) return;
-    size_t lo[3], up[3];
-    nbrs(g, t, lo, up);
-    const double* F[3] = {b.U, b.V, b.W};
-    double acc = l1[t];
-    for (int d = 0; d < 3; d++) {
-        double v = F[d][t];
-        acc += 0.5 * (v + ab(v)) * (l1[up[d]] - l1[t]);
-        v = F[d][lo[d]];
-        acc += 0.5 * (v - ab(v)) * (l1[t] - l1[lo[d]]);
+    const int i = blockIdx.x * D1X + (int)threadIdx.x % D1X;
+    const int j = blockIdx.y * D1Y + (int)threadIdx.x / D1X;
+    if (i >= nx || j >= ny) return;
+    const int z0 = blockIdx.z * zc, z1 = min(z0 + zc, nz);
+    const size_t np = (size_t)nx * ny;
+    const int il = i == 0 ? nx - 1 : i - 1, ir = i + 1 == nx ? 0 : i + 1;
+    const int jd = j == 0 ? ny - 1 : j - 1, ju = j + 1 == ny ? 0 : j + 1;
+    const size_t c = (size_t)j * nx + i;
+    const size_t cxl = (size_t)j * nx + il, cxr = (size_t)j * nx + ir;
+    const size_t cyd = (size_t)jd * nx + i, cyu = (size_t)ju * nx + i;
+    const size_t below = (size_t)(z0 == 0 ? nz - 1 : z0 - 1) * np;
+    double lm = l1[below + c], wm = W[below + c];
+    size_t o = (size_t)z0 * np;
+    double lc = l1[o + c];
+    for (int k = z0; k < z1; k++, o += np) {
+        const double lp = l1[(k + 1 == nz ? 0 : o + np) + c];
+        double acc = lc;
+        double v = U[o + c];
+        acc += 0.5 * (v + ab(v)) * (l1[o + cxr] - lc);
+        v = U[o + cxl];
+        acc += 0.5 * (v - ab(v)) * (lc - l1[o + cxl]);
+        v = V[o + c];
+        acc += 0.5 * (v + ab(v)) * (l1[o + cyu] - lc);
+        v = V[o + cyd];
+        acc += 0.5 * (v - ab(v)) * (lc - l1[o + cyd]);
+        const double wc = W[o + c];
+        acc += 0.5 * (wc + ab(wc)) * (lp - lc);
+        acc += 0.5 * (wm - ab(wm)) * (lc - lm);
+        lin[o + c] += acc;
+        lm = lc;
+        lc = lp;
+        wm = wc;
     }
-    lin[t] += acc;
 }
 
 // receptor residuals: lam[rec] += c[rec] - obs (and the loss terms), one thread each
@@ -1162,7 +1189,14 @@ int adjoint_step(mpd_grid_s* g, Work& w, const Fwd& f, const double* cin, const 
         int rc;
         if ((rc = anti_tiled(g, f.c1.p, lA, w.l1.p, s))) return rc;
     }
-    hipLaunchKernelGGL(k_adj_donor1, dim3(nb), dim3(TPB), 0, s, geo, b, w.l1.p, lin);
+    {
+        const unsigned gx = (g->nx + D1X - 1) / D1X, gy = (g->ny + D1Y - 1) / D1Y;
+        int zc = g->nz;
+        while (zc > 8 && (long long)gx * gy * ((g->nz + zc - 1) / zc) < 2048) zc = (zc + 1) / 2;
+        const unsigned gz = (g->nz + zc - 1) / zc;
+        hipLaunchKernelGGL(k_adj_donor1, dim3(gx, gy, gz), dim3(D1X * D1Y), 0, s, g->nx, g->ny,
+                           g->nz, zc, U, V, W, (const double*)w.l1.p, lin);
+    }
     MPD_HIP(hipGetLastError());
     return MPD_OK;
 }
@@ -1174,6 +1208,8 @@ int adjoint_supported(const mpd_grid_s* g)
         return fail(MPD_ERR_UNSUPPORTED, "the adjoint covers PERIODIC grids with iord = 2");
     if (!fused_plane_fits(g->nx, g->ny))
         return fail(MPD_ERR_UNSUPPORTED, "the adjoint needs z-planes of at most 2^29 cells");
+    if ((g->ny + D1Y - 1) / D1Y > 65535)
+        return fail(MPD_ERR_UNSUPPORTED, "the adjoint needs ny <= %d", 65535 * D1Y);
     return MPD_OK;
 }
 
