@@ -289,20 +289,24 @@ __device__ __forceinline__ void f1_tile(const FusedArgs& a, int item)
     }
 }
 
-// F2 over one work item.  Iteration p: publish c^n(p), the pass input cp(p) and the
-// x/y antidiffusive faces of plane p -> barrier -> limiter ratios of plane p on the tile
-// and ring 1 -> barrier -> limited donor pass of plane k = p - 1 on the tile.
-// LDS slots: c^n, Ax and Ay one plane each (read only between the two barriers that
-// follow the write; the two face values the donor pass needs a plane later are carried
-// in registers), cp three planes, the ratios two.  SAVE also stores the ratios, the
-// flux sums and the bound codes of every plane (the adjoint's forward record); it is a
-// separate instantiation so the plain step does not carry its registers.
+// F2 over one work item, one barrier per plane.  Iteration p: publish c^n(p), the pass
+// input cp(p), the x/y antidiffusive faces of plane p and the limiter ratios of plane
+// p - 1 (computed, on the tile and ring 1, in the previous iteration) -> barrier ->
+// limiter ratios of plane p on the tile and ring 1 (held in registers until the next
+// iteration publishes them) -> limited donor pass of plane k = p - 1 on the tile.
+// LDS slots: c^n, Ax, Ay and the ratios two planes each (a fast wave publishes the next
+// plane while slower waves still read this one; it cannot run further ahead than that
+// before every wave has passed the next barrier), cp three (planes p and p - 1 are read
+// while the next iteration writes p + 1).  The face values the donor pass needs a plane
+// later are carried in registers.  SAVE also stores the ratios and the bound codes of
+// every plane (the adjoint's forward record); it is a separate instantiation so the
+// plain step does not carry its registers.
 template <int TX, int TY, bool SAVE>
 __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
 {
     using G = Geo<TX, TY>;
     constexpr int RX = G::RX, R = G::R;
-    __shared__ double s_cn[R], s_cp[3][R], s_ax[R], s_ay[R], s_bu[2][R], s_bd[2][R];
+    __shared__ double s_cn[2][R], s_cp[3][R], s_ax[2][R], s_ay[2][R], s_bu[2][R], s_bd[2][R];
 
     const int t = G::role(item);
     int bx, by, bz;
@@ -326,6 +330,7 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
     double cnm = 0, cnc = 0, cpm2 = 0, cpm = 0, cpc = 0, azm2 = 0, azm = 0;
     double axm = 0, aym = 0, axlm = 0, aydm = 0;
     double bum2 = 0, bdm2 = 0, bum = 0, bdm = 0;
+    double hbum = 0, hbdm = 0;  // halo cell: ratios of plane p - 1 (published in iteration p)
     double ncn = 0, ncp = 0, nax = 0, nay = 0, naz = 0;
     // halo cell
     double hcnm = 0, hcnc = 0, hcpm = 0, hcpc = 0, hazm = 0;
@@ -376,15 +381,22 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
                 if (h1) hnaz = ld(a.Az + zp1.off, hcolb);
             }
         }
-        s_cn[L] = cnc;
+        const int k2 = (p + 1) % 2;  // slot of plane p - 1
+        s_cn[s2][L] = cnc;
         s_cp[s3][L] = cpc;
-        s_ax[L] = axn;
-        s_ay[L] = ayn;
+        s_ax[s2][L] = axn;
+        s_ay[s2][L] = ayn;
+        s_bu[k2][L] = bum;
+        s_bd[k2][L] = bdm;
         if (hl) {
-            s_cn[HL] = hcnc;
+            s_cn[s2][HL] = hcnc;
             s_cp[s3][HL] = hcpc;
-            s_ax[HL] = haxn;
-            s_ay[HL] = hayn;
+            s_ax[s2][HL] = haxn;
+            s_ay[s2][HL] = hayn;
+        }
+        if (h1) {
+            s_bu[k2][HL] = hbum;
+            s_bd[k2][HL] = hbdm;
         }
         lds_barrier();
         // limiter ratios of plane p (see cpu_ref.c)
@@ -393,8 +405,9 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
                           double cphi, double axr, double ayr, double azlo, double azhi,
                           double axl, double ayl, double& bu, double& bd, int* code) {
             double hi = cn0, lo = cn0;
-            const double nbn[7] = {cn0, s_cn[M - 1], s_cn[M + 1], s_cn[M - RX], s_cn[M + RX],
-                                   cnlo, cnhi};
+            const double* cnp = s_cn[s2];
+            const double nbn[7] = {cn0, cnp[M - 1], cnp[M + 1], cnp[M - RX], cnp[M + RX], cnlo,
+                                   cnhi};
             const double nbp[7] = {c, cpp[M - 1], cpp[M + 1], cpp[M - RX], cpp[M + RX], cplo,
                                    cphi};
 #pragma unroll
@@ -421,7 +434,7 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
             const double fzl = 0.5 * F2(cplo, c, azlo), fzr = 0.5 * F2(c, cphi, azhi);
             beta(hi, lo, c, fxl, fxr, fyl, fyr, fzl, fzr, bu, bd);
         };
-        const double axl = s_ax[L - 1], ayd = s_ay[L - RX];
+        const double axl = s_ax[s2][L - 1], ayd = s_ay[s2][L - RX];
         double buc, bdc;
         int code = 0;
         ratios(L, cnc, cnm, cnn, cpc, cpm, cpn, axn, ayn, azm, azc, axl, ayd, buc, bdc,
@@ -431,19 +444,12 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
             st(a.bd + zp0.off, colb, bdc);
             *(int*)((char*)(a.code + zp0.off) + colb / 2) = code;
         }
-        s_bu[s2][L] = buc;
-        s_bd[s2][L] = bdc;
-        if (h1) {
-            double hbu, hbd;
+        if (h1)  // (the plane p - 1 values are dead: published above)
             ratios(HL, hcnc, hcnm, hcnn, hcpc, hcpm, hcpn, haxn, hayn, hazm, hazc,
-                   s_ax[HL - 1], s_ay[HL - RX], hbu, hbd, nullptr);
-            s_bu[s2][HL] = hbu;
-            s_bd[s2][HL] = hbd;
-        }
-        lds_barrier();
+                   s_ax[s2][HL - 1], s_ay[s2][HL - RX], hbum, hbdm, nullptr);
         const int k = p - 1;
         if (k >= z0 && k < z1) {
-            const int k2 = (p + 1) % 2, k3 = (p + 2) % 3;  // slots of plane k
+            const int k3 = (p + 2) % 3;  // slot of plane k (c_p; the ratios: k2)
             const double* bu = s_bu[k2];
             const double* bd = s_bd[k2];
             const double* cq = s_cp[k3];
@@ -515,7 +521,7 @@ k_f1(FusedArgs a, Work w)
 }
 
 template <int TX, int TY, bool SAVE>
-__global__ void __launch_bounds__((Geo<TX, TY>::NT)) __attribute__((amdgpu_waves_per_eu(SAVE ? 3 : 1)))
+__global__ void __launch_bounds__((Geo<TX, TY>::NT)) __attribute__((amdgpu_waves_per_eu(SAVE && Geo<TX, TY>::NT <= 256 ? 3 : 1)))
 k_f2(FusedArgs a, Work w)
 {
     for (int it = blockIdx.x; it < w.nitems; it += gridDim.x) {
