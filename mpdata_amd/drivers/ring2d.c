@@ -1,7 +1,8 @@
 /* ring2d <n> <nt>: the reference's 2-D ring test (tests/ring2d/test.c:10-71) on the
  * HIP-backed smolar().  Writes ring2d.bin (per step: the u, v and c planes of layer 1,
- * n*n doubles each; first the initial state) and ring2d.ctl, and prints the probe cell
- * each step. */
+ * n*n doubles each; first the initial state) with its GrADS descriptor, script and perl
+ * launcher (ring2d.ctl, ring2d.gs, ring2d.pl, the ring2d_frames directory), and prints
+ * the probe cell each step. */
 #include <stdio.h>
 #include <stdlib.h>
 
@@ -27,7 +28,9 @@ int main(int argc, char* argv[])
         fprintf(stderr, "out of memory\n");
         return 1;
     }
-    ring_write_grads_ctl(n, name);
+    ring_write_grads_ctl(n, name); /* ring2d/test.c:26-33 */
+    ring_write_grads_gs(n, nt, name);
+    ring_write_grads_pl(name);
     char fn[64];
     snprintf(fn, sizeof fn, "%s.bin", name);
     FILE* fp = fopen(fn, "wb");

@@ -7,6 +7,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/stat.h>
 
 /* Ring wind (genring2d.c:29-44 and genringf2d.c:31-46, the non-uniform variants):
  * solid-body rotation scaled by 1/rmin inside rmin <= r <= rmax, faded by exp(rmax - r)
@@ -127,5 +128,47 @@ void ring_write_grads_ctl(int n, const char* name)
     fprintf(fp, "v 1 -1 vwind\n");
     fprintf(fp, "c 1 -1 source\n");
     fprintf(fp, "ENDVARS\n");
+    fclose(fp);
+}
+
+/* GrADS script drawing each of the nt stored steps into <name>_frames/stepNNNN.png:
+ * shaded c with contour levels 0, 0.1, .., 1 and the (u, v) vectors.  The text is the
+ * reference's, byte for byte, including the missing blank between the 0.4 and 0.5
+ * levels of its `set clevs` line. */
+void ring_write_grads_gs(int n, int nt, const char* name)
+{
+    char fn[256];
+    snprintf(fn, sizeof fn, "%s.gs", name);
+    FILE* fp = fopen(fn, "w");
+    if (!fp) return;
+    fprintf(fp, "'open '\"%s.ctl\"\n", name);
+    for (int it = 0; it < nt; it++) {
+        fputs("'set parea '0.5' '10.5' '1.5' '7.5\n", fp);
+        for (int axis = 0; axis < 2; axis++)
+            fprintf(fp, "'set %caxis '0' '%d' '16\n", "xy"[axis], n);
+        fputs("'set mproj 'off\n'set mpdraw 'off\n'set gxout 'shaded\n", fp);
+        fputs("'set clevs ", fp);
+        for (int l = 0; l <= 10; l++) /* levels 0.0 .. 1.0; none after the last */
+            fprintf(fp, "'%f%s", 0.1 * l, l == 4 ? "'" : (l < 10 ? "' " : "\n"));
+        fprintf(fp, "'set t '%d\n'display 'c\n", it + 1);
+        fprintf(fp, "'draw title '\"Source in %s flow - step %04d\"\n", name, it);
+        fputs("'run 'cbarm.gs\n'set gxout 'vector\n", fp);
+        fputs("'display '\"skip(u,4); skip(v,4); mag(u,v)\"\n", fp);
+        fprintf(fp, "'printim '\"%s_frames/step%04d.png\"' 'png' 'x800' 'y1024\n", name, it);
+        fputs("'clear 'graphics\n", fp);
+    }
+    fclose(fp);
+}
+
+/* Perl launcher of the GrADS script, and the <name>_frames directory it draws into. */
+void ring_write_grads_pl(const char* name)
+{
+    char fn[256];
+    snprintf(fn, sizeof fn, "%s_frames", name);
+    mkdir(fn, 0777);
+    snprintf(fn, sizeof fn, "%s.pl", name);
+    FILE* fp = fopen(fn, "w");
+    if (!fp) return;
+    fprintf(fp, "#!/usr/bin/perl -w\nsystem(\"grads -lbxc \\\"run %s.gs\\\"\");\n", name);
     fclose(fp);
 }

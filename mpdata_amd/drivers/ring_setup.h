@@ -18,3 +18,7 @@ struct ring_config* ring_init(int n, int m);
 void ring_free(struct ring_config* t);
 /* tests/init.c:124-145 (GrADS descriptor of <name>.bin) */
 void ring_write_grads_ctl(int n, const char* name);
+/* tests/init.c:148-183 (GrADS script drawing the nt steps of <name>.bin) */
+void ring_write_grads_gs(int n, int nt, const char* name);
+/* tests/init.c:186-203 (perl launcher of the script; creates <name>_frames) */
+void ring_write_grads_pl(const char* name);

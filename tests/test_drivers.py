@@ -38,6 +38,35 @@ def test_ring2d_initial_state_matches_reference_generators(tmp_path):
     assert (tmp_path / "ring2d.ctl").read_text().startswith("DSET ^ring2d.bin")
 
 
+def grads_gs_frame(n, it, name="ring2d"):
+    """One frame of the GrADS script as tests/init.c:154-179 prints it (its format
+    strings applied in the same order; the `set clevs` line is two printf calls)."""
+    clevs = ("'set clevs '%f' '%f' '%f' '%f' '%f'" % (0.0, 0.1, 0.2, 0.3, 0.4)
+             + "'%f' '%f' '%f' '%f' '%f' '%f\n" % (0.5, 0.6, 0.7, 0.8, 0.9, 1.0))
+    return ("'set parea '0.5' '10.5' '1.5' '7.5\n"
+            + "'set xaxis '0' '%d' '16\n" % n + "'set yaxis '0' '%d' '16\n" % n
+            + "'set mproj 'off\n'set mpdraw 'off\n'set gxout 'shaded\n" + clevs
+            + "'set t '%d\n'display 'c\n" % (it + 1)
+            + "'draw title '\"Source in %s flow - step %04d\"\n" % (name, it)
+            + "'run 'cbarm.gs\n'set gxout 'vector\n"
+            + "'display '\"skip(u,4); skip(v,4); mag(u,v)\"\n"
+            + "'printim '\"%s_frames/step%04d.png\"' 'png' 'x800' 'y1024\n" % (name, it)
+            + "'clear 'graphics\n")
+
+
+def test_ring2d_grads_files(tmp_path):
+    # tests/init.c:124-203: descriptor, script (no frames for nt = 0), perl launcher
+    r = run("ring2d", 32, 0, tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert (tmp_path / "ring2d.gs").read_text() == "'open '\"ring2d.ctl\"\n"
+    assert (tmp_path / "ring2d.pl").read_text() == (
+        "#!/usr/bin/perl -w\nsystem(\"grads -lbxc \\\"run ring2d.gs\\\"\");\n")
+    assert (tmp_path / "ring2d_frames").is_dir()
+    ctl = (tmp_path / "ring2d.ctl").read_text().splitlines()
+    assert ctl[:4] == ["DSET ^ring2d.bin", "XDEF 32 LINEAR 0 1", "YDEF 32 LINEAR 0 1",
+                       "ZDEF 1 LINEAR 0 1"] and ctl[-1] == "ENDVARS"
+
+
 def test_driver_usage_message(tmp_path):
     r = subprocess.run([os.path.join(BIN, "ring3d")], cwd=tmp_path, capture_output=True,
                        text=True, timeout=60)
@@ -57,6 +86,8 @@ def test_ring2d_trajectory_matches_reference(tmp_path):
         np.testing.assert_array_equal(raw[s, 2], d[f"out_s{s}"][n2:2 * n2])
     lines = r.stdout.splitlines()
     assert lines[0] == "step 0" and lines[-1].startswith("step 50 ")
+    gs = (tmp_path / "ring2d.gs").read_text()
+    assert gs == "'open '\"ring2d.ctl\"\n" + "".join(grads_gs_frame(32, it) for it in range(50))
 
 
 @pytest.mark.gpu
