@@ -501,7 +501,9 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
 
 // F1 is held to 128 registers (4 waves per SIMD; a 16-byte spill in the donor variant),
 // which measures faster than the 3 waves its natural 144 registers allow.  F2 would spill
-// 96 bytes at that bound and keeps its natural allocation.
+// at that bound and keeps its natural allocation (3 waves per SIMD); its SAVE variant is
+// held to the same 3 waves (168 registers), below which its natural allocation would
+// fall, for 4-wave tiles (8-wave tiles are held to one workgroup per CU by LDS anyway).
 template <int TX, int TY, bool DONOR>
 __global__ void __launch_bounds__((Geo<TX, TY>::NT)) __attribute__((amdgpu_waves_per_eu(4)))
 k_f1(FusedArgs a, Work w)
