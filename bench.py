@@ -96,6 +96,8 @@ def main():
                     help="edge of the source-inversion adjoint leg (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=512,
                     help="time steps of the adjoint leg's loss window")
+    ap.add_argument("--dropin", type=int, default=128,
+                    help="edge of the drop-in (FRAME, smolar semantics) leg (0: skip)")
     ap.add_argument("--ring2d", type=int, default=128,
                     help="edge of the periodic 2-D ring leg, 1024 steps (0: skip)")
     ap.add_argument("--deform", type=int, default=1,
@@ -231,6 +233,8 @@ def main():
         line["large"] = large_leg(M, args.large, args.iord, args.arith)
     if args.ring2d:
         line["ring2d"] = ring2d_leg(M, args.ring2d)
+    if args.dropin:
+        line["dropin"] = dropin_leg(M, args.dropin)
     if args.deform:
         line["deform"] = deform_leg(M)
     if args.adjoint:
@@ -259,6 +263,34 @@ def ring2d_leg(M, n, nsteps=1024):
             "ms_per_step": ms, "cell_updates_per_s": n * n / (ms * 1e-3),
             "rel_err_vs_oracle": relerr(got, want),
             "mass_rel_drift": abs(mass - c.sum()) / c.sum(), "min": cmin}
+
+
+def dropin_leg(M, n, nsteps=64, order=2):
+    """The drop-in path: smolar() semantics (FRAME mode: the reference's face layouts,
+    frame held, its reduced patterns; no limiter) on a device-resident grid, which is
+    what msmolar() steps.  n^3 with the rotation case's velocities as cell-centred winds
+    (dx = dt = 1, max Courant ~0.4), order 2.  One step against the oracle's smolar()
+    restatement (bitwise expected), then nsteps timed steps; cell-updates count the
+    (n-2)^3 interior cells the reference updates."""
+    import oracle_lib as O
+
+    case = rotation_case(n)
+    u, v, w, c = case["U"], case["V"], case["W"], case["c"]
+    one = np.ones(n)
+    g = M.Grid(n, n, n, M.FRAME, iord=order, nonosc=False)
+    g.set_winds(one, one, one, 1.0, u, u, v, v, w, w)
+    g.set_field(c)
+    g.step(1)
+    got = g.field()
+    _, _, want = O.call_smolar(O.oracle().orc_smolar, n, n, n, one, one, one, 1.0, u, u, v, v,
+                               w, w, c.copy(), np.zeros_like(c), order)
+    g.step(3)
+    ms = g.step_timed(nsteps) / nsteps
+    g.close()
+    return {"workload": f"FRAME (smolar semantics) {n}^3, rotation winds as cell-centred "
+                        f"velocities, order {order}, device-resident (the msmolar path)",
+            "ms_per_step": ms, "cell_updates_per_s": (n - 2) ** 3 / (ms * 1e-3),
+            "rel_err_1_step_vs_oracle": relerr(got, want)}
 
 
 def deform_leg(M, nsteps=200):

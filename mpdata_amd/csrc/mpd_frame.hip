@@ -4,6 +4,8 @@
 // vv.c:133, ww.c:299) and every formula comes from mpd_formulas.hpp, so the device
 // result equals the reference's FP64 result bit for bit.  One thread per face or cell;
 // this mode is the drop-in path, the periodic mode carries the tuned kernels.
+#include <algorithm>
+
 #include "mpd_formulas.hpp"
 #include "mpd_grid.hpp"
 
@@ -74,17 +76,23 @@ struct FrameIdx {
     }
 };
 
+// The per-cell and per-face kernels run on 3-D grids of 64 x 4 (x, y) blocks, one z
+// plane (or one z plane of one face family) per blockIdx.z: indices come from block
+// coordinates, with no 64-bit division (which costs more than the stencil itself).
+constexpr int BX = 64, BY = 4;  // a wave per row segment, four rows per workgroup
+inline dim3 grid3(int nx, int ny, int nz) { return dim3((nx + BX - 1) / BX, (ny + BY - 1) / BY, nz); }
+
 // smolar.c:51-53 fused: frame cells copy c0 (bndcpy, smolar.c:5-31), interior cells
 // take the donor update (smolar_pass.c:30-58).
-__global__ void k_frame_donor(FrameIdx x, const double* __restrict__ U,
-                              const double* __restrict__ V, const double* __restrict__ W,
-                              const double* __restrict__ c0, double* __restrict__ c1)
+__global__ void __launch_bounds__(BX * BY)
+k_frame_donor(FrameIdx x, int k0, const double* __restrict__ U, const double* __restrict__ V,
+              const double* __restrict__ W, const double* __restrict__ c0,
+              double* __restrict__ c1)
 {
-    size_t t = (size_t)blockIdx.x * TPB + threadIdx.x;
-    if (t >= x.np * x.ns) return;
-    int i = (int)(t % x.nx);
-    int j = (int)((t / x.nx) % x.ny);
-    int k = (int)(t / x.np);
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const int k = k0 + (int)blockIdx.z;
+    if (i >= x.nx || j >= x.ny) return;
+    const size_t t = x.C(i, j, k);
     if (i == 0 || j == 0 || k == 0 || i == x.nx - 1 || j == x.ny - 1 || k == x.ns - 1) {
         c1[t] = c0[t];
         return;
@@ -98,19 +106,20 @@ __global__ void k_frame_donor(FrameIdx x, const double* __restrict__ U,
 // pass from the previous pass's U0,V0,W0 (all three, pvelocity.c:24-26).  blockIdx.y
 // selects the family.  Faces next to the frame take the reduced pattern; the y-faces
 // of column i=1 between interior rows are not written (vv.c:81-82).
-__global__ void k_frame_antidiff(FrameIdx x, const double* __restrict__ c,
-                                 const double* __restrict__ U0, const double* __restrict__ V0,
-                                 const double* __restrict__ W0, double* __restrict__ U1,
-                                 double* __restrict__ V1, double* __restrict__ W1)
+// Grid: (i, j) over [0, nx-1) x [0, ny-1), blockIdx.z = 3 (k - k0) + family, k in
+// [0, ns-1); each family takes the faces of its own index ranges.
+__global__ void __launch_bounds__(BX * BY)
+k_frame_antidiff(FrameIdx x, int k0, const double* __restrict__ c, const double* __restrict__ U0,
+                 const double* __restrict__ V0, const double* __restrict__ W0,
+                 double* __restrict__ U1, double* __restrict__ V1, double* __restrict__ W1)
 {
     const int nx = x.nx, ny = x.ny, ns = x.ns;
-    size_t t = (size_t)blockIdx.x * TPB + threadIdx.x;
-    if (blockIdx.y == 0) {
-        size_t n = (size_t)(nx - 1) * (ny - 2) * (ns - 2);
-        if (t >= n) return;
-        int i = (int)(t % (nx - 1));
-        int j = (int)((t / (nx - 1)) % (ny - 2)) + 1;
-        int k = (int)(t / ((size_t)(nx - 1) * (ny - 2))) + 1;
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const int fam = blockIdx.z % 3, k = k0 + (int)(blockIdx.z / 3);
+    if (fam == 0) {
+        // faces i + 1/2, i in [0, nx-2], of interior rows j, k
+        if (i > nx - 2 || j < 1 || j > ny - 2 || k < 1 || k > ns - 2) return;
+        const size_t t = x.U(i, j, k);
         double a = U0[t], div, sv, sw;
         if (i == 0 || i == nx - 2) {
             int ii = (i == 0) ? 1 : nx - 2;
@@ -134,12 +143,10 @@ __global__ void k_frame_antidiff(FrameIdx x, const double* __restrict__ c,
         double crf = c[o + np + 1], cf = c[o + np], cb = c[o - np], crb = c[o - np + 1];
         U1[t] = antidiff(div, a, cr, cc, sv, cru + cu - cd - crd, cru + cu + cd + crd, sw,
                          crf + cf - cb - crb, crf + cf + cb + crb);
-    } else if (blockIdx.y == 1) {
-        size_t n = (size_t)(nx - 2) * (ny - 1) * (ns - 2);
-        if (t >= n) return;
-        int i = (int)(t % (nx - 2)) + 1;
-        int j = (int)((t / (nx - 2)) % (ny - 1));
-        int k = (int)(t / ((size_t)(nx - 2) * (ny - 1))) + 1;
+    } else if (fam == 1) {
+        // faces j + 1/2, j in [0, ny-2], of interior columns i, k
+        if (i < 1 || i > nx - 2 || j > ny - 2 || k < 1 || k > ns - 2) return;
+        const size_t t = x.V(i, j, k);
         bool edge = (j == 0 || j == ny - 2);
         if (!edge && i == 1) return;  // vv.c:81-82
         double a = V0[t], div, su, sw;
@@ -167,11 +174,9 @@ __global__ void k_frame_antidiff(FrameIdx x, const double* __restrict__ c,
         V1[t] = antidiff(div, a, cu, cc, su, cru + cr - cl - clu, cru + cr + cl + clu, sw,
                          cuf + cf - cb - cub, cuf + cf + cb + cub);
     } else {
-        size_t n = (size_t)(nx - 2) * (ny - 2) * (ns - 1);
-        if (t >= n) return;
-        int i = (int)(t % (nx - 2)) + 1;
-        int j = (int)((t / (nx - 2)) % (ny - 2)) + 1;
-        int k = (int)(t / ((size_t)(nx - 2) * (ny - 2)));
+        // faces k + 1/2, k in [0, ns-2], of interior columns i, j
+        if (i < 1 || i > nx - 2 || j < 1 || j > ny - 2 || k > ns - 2) return;
+        const size_t t = x.W(i, j, k);
         double a = W0[t], div, su, sv;
         if (k == 0 || k == ns - 2) {
             int kk = (k == 0) ? 1 : ns - 2;
@@ -252,6 +257,11 @@ int frame_step(mpd_grid_s* g)
 {
     hipStream_t s = stream();
     int rc;
+    // planes per launch (blockIdx.z <= 65535 with three face families per plane) and
+    // rows (blockIdx.y <= 65535)
+    constexpr int kMaxPlanes = 65535 / 3;
+    if ((g->ny + BY - 1) / BY > 65535)
+        return fail(MPD_ERR_UNSUPPORTED, "FRAME grids need ny <= %d", 65535 * BY);
     FrameIdx x{g->nx, g->ny, g->nz, (size_t)g->nx * g->ny};
     double* set[2][3];
     if (g->iord > 1) {
@@ -271,20 +281,21 @@ int frame_step(mpd_grid_s* g)
     } else {
         for (int d = 0; d < 3; d++) set[0][d] = g->face[d].p;
     }
-    size_t nmax = g->nface[0];
-    if (g->nface[1] > nmax) nmax = g->nface[1];
-    if (g->nface[2] > nmax) nmax = g->nface[2];
     int cold = g->cur;
     for (int o = 0; o < g->iord; o++) {
         double** vel = set[o & 1];
         if (o) {
             double** prev = set[(o - 1) & 1];
-            hipLaunchKernelGGL(k_frame_antidiff, dim3(nblocks(nmax), 3), dim3(TPB), 0, s, x,
-                               g->c[cold].p, prev[0], prev[1], prev[2], vel[0], vel[1],
-                               vel[2]);
+            for (int k0 = 0; k0 < g->nz - 1; k0 += kMaxPlanes)
+                hipLaunchKernelGGL(k_frame_antidiff,
+                                   grid3(g->nx - 1, g->ny - 1, 3 * std::min(kMaxPlanes, g->nz - 1 - k0)),
+                                   dim3(BX, BY), 0, s, x, k0, g->c[cold].p, prev[0], prev[1],
+                                   prev[2], vel[0], vel[1], vel[2]);
         }
-        hipLaunchKernelGGL(k_frame_donor, dim3(nblocks(g->ncell)), dim3(TPB), 0, s, x, vel[0],
-                           vel[1], vel[2], g->c[cold].p, g->c[1 - cold].p);
+        for (int k0 = 0; k0 < g->nz; k0 += kMaxPlanes)
+            hipLaunchKernelGGL(k_frame_donor, grid3(g->nx, g->ny, std::min(kMaxPlanes, g->nz - k0)),
+                               dim3(BX, BY), 0, s, x, k0, vel[0], vel[1], vel[2], g->c[cold].p,
+                               g->c[1 - cold].p);
         cold = 1 - cold;
     }
     MPD_HIP(hipGetLastError());

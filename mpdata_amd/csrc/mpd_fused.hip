@@ -25,6 +25,8 @@
 // division; <= 1e-12 normwise per step against the oracle).
 #include <hip/hip_ext.h>
 
+#include <algorithm>
+
 #include "mpd_formulas.hpp"
 #include "mpd_fused.hpp"
 #include "mpd_grid.hpp"
@@ -540,17 +542,21 @@ k_f2(FusedArgs a, Work w)
     }
 }
 
-// The staged donor (mpd_periodic.hip) serves the unlimited corrective pass.
-constexpr int TPB = 256;
-__global__ void k_donor_faces(int nx, int ny, int nz, const double* __restrict__ U,
-                              const double* __restrict__ V, const double* __restrict__ W,
-                              const double* __restrict__ cin, const double* __restrict__ q,
-                              double* __restrict__ cout)
+// The donor pass of the unlimited corrective pass, one thread per cell on a 3-D grid of
+// 64 x 4 (x, y) blocks and one z plane per blockIdx.z (plane k0 + blockIdx.z): indices
+// from block coordinates, no 64-bit division.
+constexpr int DBX = 64, DBY = 4, DMAXZ = 65535;
+__global__ void __launch_bounds__(DBX * DBY)
+k_donor_faces(int nx, int ny, int nz, int k0, const double* __restrict__ U,
+              const double* __restrict__ V, const double* __restrict__ W,
+              const double* __restrict__ cin, const double* __restrict__ q,
+              double* __restrict__ cout)
 {
-    const size_t np = (size_t)nx * ny, n = np * nz;
-    size_t t = (size_t)blockIdx.x * TPB + threadIdx.x;
-    if (t >= n) return;
-    int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / np);
+    const size_t np = (size_t)nx * ny;
+    const int i = blockIdx.x * DBX + threadIdx.x, j = blockIdx.y * DBY + threadIdx.y;
+    const int k = k0 + (int)blockIdx.z;
+    if (i >= nx || j >= ny) return;
+    const size_t t = (size_t)k * np + (size_t)j * nx + i;
     size_t xl = t - i + (i == 0 ? nx - 1 : i - 1), xr = t - i + (i + 1 == nx ? 0 : i + 1);
     size_t yd = t + ((size_t)(j == 0 ? ny - 1 : j - 1) - j) * nx;
     size_t yu = t + ((size_t)(j + 1 == ny ? 0 : j + 1) - j) * nx;
@@ -628,6 +634,29 @@ int span(mpd_grid_s* g, hipEvent_t ev[2], const hipEvent_t** use)
     return MPD_OK;
 }
 
+// The unlimited corrective donor pass cout = donor(cin; U, V, W) (+ q), in launches of
+// at most DMAXZ planes (the timing events, when given, bracket a single launch).
+int donor_faces(const mpd_grid_s* g, const double* U, const double* V, const double* W,
+                const double* cin, const double* q, double* cout, hipStream_t s,
+                const hipEvent_t* ev)
+{
+    if ((g->ny + DBY - 1) / DBY > 65535)
+        return fail(MPD_ERR_UNSUPPORTED, "unlimited periodic passes need ny <= %d", 65535 * DBY);
+    const dim3 grid((g->nx + DBX - 1) / DBX, (g->ny + DBY - 1) / DBY, 1);
+    if (g->nz <= DMAXZ) {
+        launch_k(k_donor_faces, dim3(grid.x, grid.y, g->nz), dim3(DBX, DBY), s, ev, g->nx,
+                 g->ny, g->nz, 0, U, V, W, cin, q, cout);
+    } else {
+        if (ev) MPD_HIP(hipEventRecord(ev[0], s));
+        for (int k0 = 0; k0 < g->nz; k0 += DMAXZ)
+            launch_k(k_donor_faces, dim3(grid.x, grid.y, std::min(DMAXZ, g->nz - k0)),
+                     dim3(DBX, DBY), s, nullptr, g->nx, g->ny, g->nz, k0, U, V, W, cin, q, cout);
+        if (ev) MPD_HIP(hipEventRecord(ev[1], s));
+    }
+    MPD_HIP(hipGetLastError());
+    return MPD_OK;
+}
+
 // z-chunk length: enough workgroups to fill 256 CUs several times over, long enough
 // that the 2-3 extra planes below each chunk stay a small fraction.
 int choose_zc(const mpd_grid_s* g, int tx, int ty)
@@ -697,19 +726,13 @@ int MPD_CAT(fused_store_step, MPD_ARITH)(mpd_grid_s* g, const double* cin, doubl
         a.code = code;
         return launch(a, tx, ty, false, false, s, nullptr);
     }
-    const size_t n = g->ncell;
-    launch_k(k_donor_faces, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), s, nullptr, g->nx,
-             g->ny, g->nz, (const double*)A[0], (const double*)A[1], (const double*)A[2],
-             (const double*)c1, q, cout);
-    MPD_HIP(hipGetLastError());
-    return MPD_OK;
+    return donor_faces(g, A[0], A[1], A[2], c1, q, cout, s, nullptr);
 }
 
 int MPD_CAT(periodic_fused_step, MPD_ARITH)(mpd_grid_s* g)
 {
     hipStream_t s = stream();
     int rc;
-    const size_t n = g->ncell;
     if ((rc = ensure_fused_scratch(g))) return rc;
     int tx = g->tile_x ? g->tile_x : 32, ty = g->tile_y ? g->tile_y : 8;
     FusedArgs a{};
@@ -753,10 +776,9 @@ int MPD_CAT(periodic_fused_step, MPD_ARITH)(mpd_grid_s* g)
             if ((rc = launch(a, tx, ty, false, false, s, use))) return rc;
             for (int d = 0; d < 3; d++) prev[d] = g->fb[d].p;
         } else {
-            launch_k(k_donor_faces, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), s, use,
-                     g->nx, g->ny, g->nz, (const double*)tgt[0].p, (const double*)tgt[1].p,
-                     (const double*)tgt[2].p, pass_in, last ? q : (const double*)nullptr, dst);
-            MPD_HIP(hipGetLastError());
+            if ((rc = donor_faces(g, tgt[0].p, tgt[1].p, tgt[2].p, pass_in,
+                                  last ? q : nullptr, dst, s, use)))
+                return rc;
             for (int d = 0; d < 3; d++) prev[d] = tgt[d].p;
         }
         field = dst;
