@@ -8,6 +8,7 @@
 
 #include "mpd_formulas.hpp"
 #include "mpd_grid.hpp"
+#include "mpd_tile.hpp"
 
 namespace mpd {
 using namespace exact;
@@ -249,6 +250,315 @@ int courant_from_winds(mpd_grid_s* g, const double* dx, const double* dy, const 
     return MPD_OK;
 }
 
+// ------------------------------------------- donor + antidiffusive velocities, tiled ---
+//
+// One corrective pass's antidiffusive velocities -- and on the first pass the donor
+// field they are formed from -- in one LDS-tiled 2.5-D streaming kernel: the frame-mode
+// counterpart of the periodic fused F1 (mpd_fused.hip), on the reference's layouts and
+// with its frame rules.  Same values as k_frame_donor followed by k_frame_antidiff, bit
+// for bit (the same formulas in the same order from mpd_formulas.hpp).
+//
+// A workgroup owns a TX x TY column tile of the grid (frame included) and a chunk of
+// z-planes; each thread owns a tile column and at most one column of the 2-cell halo
+// (tile::Geo).  Iteration p: publish c^n(p), U0(p), V0(p) -> barrier -> donor field
+// c1(p) on the tile and ring 1 (frame cells: c^n), publish it with W0(p-1) -> barrier ->
+// the velocities of the faces above / right of the tile cells of plane k = p - 1.
+// Cells and faces outside the grid, or absent from the reference's layouts, read as 0
+// and are never used by a face or cell that exists.
+struct FrameF1Args {
+    int nx, ny, ns, tiles_x, tiles_y, zc;
+    const double* cin;            // first pass: c^n; later passes: the pass input
+    const double *U0, *V0, *W0;   // velocities of the previous pass (reference layouts)
+    double* c1out;                // first pass: the donor field, every cell
+    double *U1, *V1, *W1;         // this pass's velocities
+};
+
+// Column offsets of a cell (gx, gy) in each layout and whether the column exists there.
+struct FrameCol {
+    bool c, u, v, w;
+    size_t oc, ou, ov, ow;
+    __device__ FrameCol(const FrameF1Args& a, int gx, int gy)
+    {
+        const int nx = a.nx, ny = a.ny;
+        c = gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+        u = gx >= 0 && gx <= nx - 2 && gy >= 1 && gy <= ny - 2;
+        v = gx >= 1 && gx <= nx - 2 && gy >= 0 && gy <= ny - 2;
+        w = gx >= 1 && gx <= nx - 2 && gy >= 1 && gy <= ny - 2;
+        oc = c ? (size_t)gy * nx + gx : 0;
+        ou = u ? (size_t)(gy - 1) * (nx - 1) + gx : 0;
+        ov = v ? (size_t)gy * (nx - 2) + gx - 1 : 0;
+        ow = w ? (size_t)(gy - 1) * (nx - 2) + gx - 1 : 0;
+    }
+};
+
+template <int TX, int TY, bool DONOR>
+__global__ void __launch_bounds__((tile::Geo<TX, TY>::NT)) k_frame_f1(FrameF1Args a)
+{
+    using G = tile::Geo<TX, TY>;
+    constexpr int RX = G::RX, R = G::R;
+    __shared__ double s_c0[R], s_c1[3][R], s_u[3][R], s_v[3][R], s_w[R];
+    const int nx = a.nx, ny = a.ny, ns = a.ns;
+    const size_t np = (size_t)nx * ny;
+    const size_t upl = (size_t)(nx - 1) * (ny - 2), vpl = (size_t)(nx - 2) * (ny - 1);
+    const size_t wpl = (size_t)(nx - 2) * (ny - 2);  // elements per plane of U, V, W
+
+    const int t = threadIdx.x;
+    const int item = tile::xcd_item(blockIdx.x, gridDim.x);
+    const int bx = item % a.tiles_x, by = (item / a.tiles_x) % a.tiles_y;
+    const int bz = item / (a.tiles_x * a.tiles_y);
+    const int rx = 2 + t % TX, ry = 2 + t / TX;
+    const int L = ry * RX + rx;
+    const int gx = bx * TX + rx - 2, gy = by * TY + ry - 2;
+    const FrameCol P(a, gx, gy);
+    const bool hl = t < G::NH, h1 = t < G::H1;
+    int hx = 0, hy = 0;
+    if (hl) G::halo(t, hx, hy);
+    const int HL = hy * RX + hx;
+    const int hgx = bx * TX + hx - 2, hgy = by * TY + hy - 2;
+    const FrameCol H(a, hl ? hgx : -1, hl ? hgy : -1);
+    const bool frame_xy = gx == 0 || gx == nx - 1 || gy == 0 || gy == ny - 1;
+    const bool hframe_xy = hgx == 0 || hgx == nx - 1 || hgy == 0 || hgy == ny - 1;
+
+    // plane loads: zero outside the grid or the layout
+    auto ldc = [&](const FrameCol& q, int k) {
+        return (q.c && k >= 0 && k < ns) ? a.cin[(size_t)k * np + q.oc] : 0.0;
+    };
+    auto ldu = [&](const FrameCol& q, int k) {
+        return (q.u && k >= 1 && k <= ns - 2) ? a.U0[(size_t)(k - 1) * upl + q.ou] : 0.0;
+    };
+    auto ldv = [&](const FrameCol& q, int k) {
+        return (q.v && k >= 1 && k <= ns - 2) ? a.V0[(size_t)(k - 1) * vpl + q.ov] : 0.0;
+    };
+    auto ldw = [&](const FrameCol& q, int k) {
+        return (q.w && k >= 0 && k <= ns - 2) ? a.W0[(size_t)k * wpl + q.ow] : 0.0;
+    };
+
+    const int z0 = bz * a.zc, z1 = min(z0 + a.zc, ns);
+    const int p0 = max(z0 - 1, 0), p1 = min(z1, ns - 1);
+    // own column at the top of iteration p: c^n (p-1, p), c1 (p-2, p-1), U, V (p-1),
+    // W (p-2, p-1), the W neighbours (L+1, L+RX) of plane p-2; n*: c^n at p+1 and the
+    // faces at p, loaded one iteration ahead (first pass; later passes read the pass
+    // input field as nf)
+    double c0m = 0, c0c = 0, nc0 = 0, nf = 0, c1m = 0, c1c = 0, uc = 0, vc = 0;
+    double wmm = 0, wm1 = ldw(P, p0 - 1), wmr = 0, wmu = 0;
+    double nu = ldu(P, p0), nv = ldv(P, p0), nw = ldw(P, p0);
+    double hc0m = 0, hc0c = 0, hnc0 = 0, hnf = 0, hwm1 = 0;
+    double hnu = 0, hnv = 0, hnw = 0;
+    if (DONOR) {
+        c0m = ldc(P, p0 - 1);
+        c0c = ldc(P, p0);
+        nc0 = ldc(P, p0 + 1);
+    } else {
+        nf = ldc(P, p0);
+    }
+    if (hl) {
+        if (DONOR) {
+            hc0m = ldc(H, p0 - 1);
+            hc0c = ldc(H, p0);
+            hnc0 = ldc(H, p0 + 1);
+        } else if (h1) {
+            hnf = ldc(H, p0);
+        }
+        hnu = ldu(H, p0);
+        hnv = ldv(H, p0);
+        if (h1) {
+            hwm1 = ldw(H, p0 - 1);
+            hnw = ldw(H, p0);
+        }
+    }
+    for (int pl = p0; pl <= p1; pl++) {
+        const int p = __builtin_amdgcn_readfirstlane(pl);
+        const int s3 = (p + 3) % 3;
+        const double c0n = nc0, un = nu, vn = nv, wn = nw, fld = nf;
+        const double hc0n = hnc0, hun = hnu, hvn = hnv, hwn = hnw, hfld = hnf;
+        if (p < p1) {  // the next iteration's values: c^n at p + 2, faces at p + 1
+            if (DONOR)
+                nc0 = ldc(P, p + 2);
+            else
+                nf = ldc(P, p + 1);
+            nu = ldu(P, p + 1);
+            nv = ldv(P, p + 1);
+            nw = ldw(P, p + 1);
+            if (hl) {
+                if (DONOR)
+                    hnc0 = ldc(H, p + 2);
+                else if (h1)
+                    hnf = ldc(H, p + 1);
+                hnu = ldu(H, p + 1);
+                hnv = ldv(H, p + 1);
+                if (h1) hnw = ldw(H, p + 1);
+            }
+        }
+        if (DONOR) s_c0[L] = c0c;
+        s_u[s3][L] = un;
+        s_v[s3][L] = vn;
+        if (hl) {
+            if (DONOR) s_c0[HL] = hc0c;
+            s_u[s3][HL] = hun;
+            s_v[s3][HL] = hvn;
+        }
+        tile::lds_barrier();
+        // donor field of plane p (smolar.c:51-53: frame cells keep c^n)
+        double c1p;
+        if (DONOR) {
+            c1p = c0c;
+            if (P.c && !frame_xy && p != 0 && p != ns - 1)
+                c1p = donor(c0c, s_c0[L - 1], s_c0[L + 1], s_c0[L - RX], s_c0[L + RX], c0m,
+                            c0n, s_u[s3][L - 1], un, s_v[s3][L - RX], vn, wm1, wn);
+            if (P.c && p >= z0 && p < z1) a.c1out[(size_t)p * np + P.oc] = c1p;
+        } else {
+            c1p = fld;
+        }
+        s_c1[s3][L] = c1p;
+        s_w[L] = wm1;
+        if (h1) {
+            double hc1 = DONOR ? hc0c : hfld;
+            if (DONOR && H.c && !hframe_xy && p != 0 && p != ns - 1)
+                hc1 = donor(hc0c, s_c0[HL - 1], s_c0[HL + 1], s_c0[HL - RX], s_c0[HL + RX],
+                            hc0m, hc0n, s_u[s3][HL - 1], hun, s_v[s3][HL - RX], hvn, hwm1,
+                            hwn);
+            s_c1[s3][HL] = hc1;
+            s_w[HL] = hwm1;
+        }
+        tile::lds_barrier();
+        const int k = p - 1;  // plane whose faces are computed now
+        const double wkr = s_w[L + 1], wku = s_w[L + RX];
+        if (k >= z0 && k < z1 && k >= 0) {
+            const int m3 = (p + 1) % 3, c3 = (p + 2) % 3;  // slots of planes k-1, k
+            const double* cm = s_c1[m3];
+            const double* cc = s_c1[c3];
+            const double* cq = s_c1[s3];
+            const double* uk = s_u[c3];
+            const double* uq = s_u[s3];
+            const double* vk = s_v[c3];
+            const double* vq = s_v[s3];
+            const double c = c1c;
+            if (P.u && k >= 1 && k <= ns - 2) {  // x-face (uu.c:56-99)
+                const double av = uc;
+                double div, sv, sw;
+                if (gx == 0 || gx == nx - 2) {  // reduced pattern at column ii (uu.c:92-97)
+                    const int o = gx == 0 ? 1 : 0;  // ii - gx
+                    div = 0.0;
+                    sv = 0.5 * (vk[L + o - RX] + vk[L + o]);
+                    sw = 0.5 * ((o ? wmr : wmm) + (o ? wkr : wm1));
+                } else {
+                    double vr = vk[L + 1 - RX], vru = vk[L + 1], vu = vc, vcc = vk[L - RX];
+                    double wr = wmr, wrf = wkr, wf = wm1, wc = wmm;
+                    sv = 0.25 * (vr + vru + vu + vcc);
+                    sw = 0.25 * (wr + wrf + wf + wc);
+                    div = 0.25 * av *
+                          (uk[L + 1] - uk[L - 1] + vr - vru - vu + vcc + wr - wrf - wf + wc);
+                }
+                double cr = cc[L + 1], cru = cc[L + 1 + RX], cu = cc[L + RX];
+                double cd = cc[L - RX], crd = cc[L + 1 - RX];
+                double crf = cq[L + 1], cf = c1p, cb = c1m, crb = cm[L + 1];
+                a.U1[(size_t)(k - 1) * upl + P.ou] =
+                    antidiff(div, av, cr, c, sv, cru + cu - cd - crd, cru + cu + cd + crd, sw,
+                             crf + cf - cb - crb, crf + cf + cb + crb);
+            }
+            const bool yedge = gy == 0 || gy == ny - 2;
+            if (P.v && k >= 1 && k <= ns - 2 && (yedge || gx != 1)) {  // y-face (vv.c:58-104)
+                const double av = vc;
+                double div, su, sw;
+                if (yedge) {  // reduced pattern at row jj (vv.c:97-103)
+                    const int o = gy == 0 ? RX : 0;  // LDS offset of row jj
+                    div = 0.0;
+                    su = 0.5 * (uk[L - 1 + o] + (o ? uk[L + RX] : uc));
+                    sw = 0.5 * ((o ? wmu : wmm) + (o ? wku : wm1));
+                } else {
+                    double uu = uk[L - 1 + RX], uru = uk[L + RX], ur = uc, ucc = uk[L - 1];
+                    double ulu = uk[L - 2 + RX], ul = uk[L - 2];
+                    double wu = wmu, wuf = wku, wf = wm1, wc = wmm;
+                    su = 0.25 * (uu + uru + ur + ucc);
+                    sw = 0.25 * (wu + wuf + wf + wc);
+                    div = 0.25 * av *
+                          (vk[L + RX] - vk[L - RX] + wu - wuf - wf + wc + uu - ulu - ul + ucc);
+                }
+                double cu = cc[L + RX], cru = cc[L + 1 + RX], cr = cc[L + 1];
+                double cl = cc[L - 1], clu = cc[L - 1 + RX];
+                double cuf = cq[L + RX], cf = c1p, cb = c1m, cub = cm[L + RX];
+                a.V1[(size_t)(k - 1) * vpl + P.ov] =
+                    antidiff(div, av, cu, c, su, cru + cr - cl - clu, cru + cr + cl + clu, sw,
+                             cuf + cf - cb - cub, cuf + cf + cb + cub);
+            }
+            if (P.w && k <= ns - 2) {  // z-face (ww.c:222-270)
+                const double av = wm1;
+                double div, su, sv;
+                if (k == 0 || k == ns - 2) {  // reduced pattern at plane kk (ww.c:101-110)
+                    const bool up = k == 0;  // kk = k + 1
+                    div = 0.0;
+                    su = 0.5 * ((up ? uq : uk)[L - 1] + (up ? un : uc));
+                    sv = 0.5 * ((up ? vq : vk)[L - RX] + (up ? vn : vc));
+                } else {
+                    double ur = uc, urf = un, uf = uq[L - 1], ucc = uk[L - 1];
+                    double vf = vq[L - RX], vuf = vn, vu = vc, vcc = vk[L - RX];
+                    su = 0.25 * (ur + urf + uf + ucc);
+                    sv = 0.25 * (vf + vuf + vu + vcc);
+                    div = 0.25 * av * (wn - wmm + uf - urf - ur + ucc + vf - vuf - vu + vcc);
+                }
+                double cf = c1p, crf = cq[L + 1], cr = cc[L + 1], cl = cc[L - 1];
+                double clf = cq[L - 1], cuf = cq[L + RX], cu = cc[L + RX], cd = cc[L - RX];
+                double cdf = cq[L - RX];
+                a.W1[(size_t)k * wpl + P.ow] =
+                    antidiff(div, av, cf, c, su, crf + cr - cl - clf, crf + cr + cl + clf, sv,
+                             cuf + cu - cd - cdf, cuf + cu + cd + cdf);
+            }
+        }
+        if (DONOR) {
+            c0m = c0c;
+            c0c = c0n;
+            hc0m = hc0c;
+            hc0c = hc0n;
+        }
+        c1m = c1c;
+        c1c = c1p;
+        uc = un;
+        vc = vn;
+        wmm = wm1;
+        wm1 = wn;
+        wmr = wkr;
+        wmu = wku;
+        hwm1 = hwn;
+    }
+}
+
+// Launches k_frame_f1 over the whole grid: 32x8 tiles, z-chunks long enough to keep
+// the 2-3 planes a chunk reads below itself a small share, short enough for ~2048
+// workgroups.
+int frame_f1(const mpd_grid_s* g, bool donor1, const double* cin, double* const prev[3],
+             double* const out[3], double* c1out, hipStream_t s)
+{
+    constexpr int TX = 32, TY = 8;
+    FrameF1Args a{};
+    a.nx = g->nx;
+    a.ny = g->ny;
+    a.ns = g->nz;
+    a.tiles_x = (a.nx + TX - 1) / TX;
+    a.tiles_y = (a.ny + TY - 1) / TY;
+    const long long cols = (long long)a.tiles_x * a.tiles_y;
+    int zc = a.ns;
+    while (zc > 8 && cols * ((a.ns + zc - 1) / zc) < 2048) zc = (zc + 1) / 2;
+    a.zc = zc;
+    const long long items = cols * ((a.ns + zc - 1) / zc);
+    if (items > 0x7fffffff) return fail(MPD_ERR_UNSUPPORTED, "FRAME grid too large for one launch");
+    a.cin = cin;
+    a.U0 = prev[0];
+    a.V0 = prev[1];
+    a.W0 = prev[2];
+    a.c1out = c1out;
+    a.U1 = out[0];
+    a.V1 = out[1];
+    a.W1 = out[2];
+    if (donor1)
+        hipLaunchKernelGGL((k_frame_f1<TX, TY, true>), dim3((unsigned)items),
+                           dim3(tile::Geo<TX, TY>::NT), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_frame_f1<TX, TY, false>), dim3((unsigned)items),
+                           dim3(tile::Geo<TX, TY>::NT), 0, s, a);
+    MPD_HIP(hipGetLastError());
+    return MPD_OK;
+}
+
 // smolar.c:33-63.  The antidiffusive ping-pong of the reference (uu.c:43-44) is kept:
 // pass o reads buffer set (o-1)%2 and writes set o%2, where set 0 starts as the
 // Courant numbers and set 1 starts zero-filled (the zero-malloc contract; it decides
@@ -282,9 +592,19 @@ int frame_step(mpd_grid_s* g)
         for (int d = 0; d < 3; d++) set[0][d] = g->face[d].p;
     }
     int cold = g->cur;
+    // the tiled kernel forms the donor field together with the first pass's velocities;
+    // mpd_set_path(g, 1) keeps the per-face kernels
+    const bool tiled = g->path != 1 && g->iord > 1;
     for (int o = 0; o < g->iord; o++) {
         double** vel = set[o & 1];
-        if (o) {
+        if (tiled) {
+            if (o == 0) continue;  // formed in pass 1's kernel
+            const bool first = o == 1;
+            if ((rc = frame_f1(g, first, g->c[cold].p, set[(o - 1) & 1], vel,
+                               first ? g->c[1 - cold].p : nullptr, s)))
+                return rc;
+            if (first) cold = 1 - cold;  // c1 is now the newest field
+        } else if (o) {
             double** prev = set[(o - 1) & 1];
             for (int k0 = 0; k0 < g->nz - 1; k0 += kMaxPlanes)
                 hipLaunchKernelGGL(k_frame_antidiff,

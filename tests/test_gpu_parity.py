@@ -110,6 +110,40 @@ def test_frame_grid_handle_matches_oracle():
     np.testing.assert_array_equal(got, c0)
 
 
+FRAME_SHAPES = [(70, 37, 21), (33, 9, 40), (3, 3, 3), (35, 3, 5), (3, 20, 9), (66, 18, 3)]
+
+
+@pytest.mark.parametrize("dims", FRAME_SHAPES)
+def test_frame_tiled_kernel_matches_per_face_kernels_and_oracle(dims):
+    # the tiled donor + antidiffusive kernel (default) against the per-face kernels
+    # (mpd_set_path 1) and the oracle's smolar(), bitwise: shapes spanning several
+    # 32x8 tiles and z-chunks, and 3-cell extents (every face next to the frame)
+    nx, ny, ns = dims
+    rng = np.random.default_rng(nx * 1000 + ny * 10 + ns)
+    n = nx * ny * ns
+    sp = [rng.uniform(0.9, 1.1, k) for k in (nx, ny, ns)]
+    winds = [rng.uniform(-0.3, 0.3, n) for _ in range(6)]
+    c = rng.uniform(0, 1, n) ** 2
+    print(f"case frame {dims}", flush=True)
+    for order in (1, 2, 3, 4):
+        out = []
+        for path in (0, 1):
+            g = M.Grid(nx, ny, ns, M.FRAME, iord=order, nonosc=False)
+            g.set_path(path)
+            g.set_winds(*sp, 0.7, *winds)
+            g.set_field(c)
+            g.step(3)
+            out.append(g.field())
+            g.close()
+        np.testing.assert_array_equal(out[0], out[1], err_msg=f"order {order}")
+        c0, c1 = c.copy(), np.zeros(n)
+        for _ in range(3):
+            _, a0, a1 = O.call_smolar(O.oracle().orc_smolar, nx, ny, ns, *sp, 0.7, *winds, c0,
+                                      c1, order)
+            c0, c1 = a1, a0
+        np.testing.assert_array_equal(out[0], c0, err_msg=f"order {order} vs oracle")
+
+
 # --------------------------------------------------------------- periodic mode ---
 
 def per_case(nx, ny, nz, seed, cmax=0.2):
