@@ -84,9 +84,11 @@ int mpd_calibration_copy(size_t n, int reps, double* ms);
 /* Mass (sum of c), minimum and maximum of the field, reduced on the device: the
  * failure sentinel for long runs (MPD_ERR_NUMERIC if any value is non-finite). */
 int mpd_field_stats(mpd_grid g, double* mass, double* cmin, double* cmax);
-/* Kernel-path selection for PERIODIC grids: 0 = automatic (fused), 1 = staged
- * (one kernel per stage, intermediates in HBM), 2 = fused.  Grids with more than
- * 2^29 cells per z-plane always take the staged path. */
+/* Kernel-path selection.  PERIODIC grids: 0 = automatic (fused), 1 = staged (one
+ * kernel per stage, intermediates in HBM), 2 = fused; grids with more than 2^29 cells
+ * per z-plane always take the staged path.  FRAME grids: 1 = the per-face
+ * antidiffusive kernel and a separate donor pass, otherwise the tiled kernel that forms
+ * the donor field and the velocities together (results are identical). */
 int mpd_set_path(mpd_grid g, int path);
 /* Arithmetic of the PERIODIC fused kernels: MPD_ARITH_FAST (default) or
  * MPD_ARITH_EXACT.  FRAME grids and the staged path always compute exactly. */
