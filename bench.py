@@ -79,7 +79,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=128)
-    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=64)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--iord", type=int, default=2)
     ap.add_argument("--path", type=int, default=0, help="0 auto, 1 staged, 2 fused")
