@@ -271,6 +271,11 @@ struct FrameF1Args {
     const double *U0, *V0, *W0;   // velocities of the previous pass (reference layouts)
     double* c1out;                // first pass: the donor field, every cell
     double *U1, *V1, *W1;         // this pass's velocities
+    // the y-faces of column i = 1 between interior rows, which vv_pass never writes
+    // (vv.c:81-82), receive what the reference's buffer holds there: zero on odd passes
+    // (the zero-filled second buffer), the first-order V on even ones (the buffer that
+    // starts as the Courant numbers) -- q7 = null or the Courant V array
+    const double* q7;
 };
 
 // Column offsets of a cell (gx, gy) in each layout and whether the column exists there.
@@ -457,7 +462,10 @@ __global__ void __launch_bounds__((tile::Geo<TX, TY>::NT)) k_frame_f1(FrameF1Arg
                              crf + cf - cb - crb, crf + cf + cb + crb);
             }
             const bool yedge = gy == 0 || gy == ny - 2;
-            if (P.v && k >= 1 && k <= ns - 2 && (yedge || gx != 1)) {  // y-face (vv.c:58-104)
+            if (P.v && k >= 1 && k <= ns - 2 && !yedge && gx == 1) {
+                const size_t o = (size_t)(k - 1) * vpl + P.ov;
+                a.V1[o] = a.q7 ? a.q7[o] : 0.0;
+            } else if (P.v && k >= 1 && k <= ns - 2) {  // y-face (vv.c:58-104)
                 const double av = vc;
                 double div, su, sw;
                 if (yedge) {  // reduced pattern at row jj (vv.c:97-103)
@@ -526,7 +534,7 @@ __global__ void __launch_bounds__((tile::Geo<TX, TY>::NT)) k_frame_f1(FrameF1Arg
 // the 2-3 planes a chunk reads below itself a small share, short enough for ~2048
 // workgroups.
 int frame_f1(const mpd_grid_s* g, bool donor1, const double* cin, double* const prev[3],
-             double* const out[3], double* c1out, hipStream_t s)
+             double* const out[3], double* c1out, const double* q7, hipStream_t s)
 {
     constexpr int TX = 32, TY = 8;
     FrameF1Args a{};
@@ -549,6 +557,7 @@ int frame_f1(const mpd_grid_s* g, bool donor1, const double* cin, double* const 
     a.U1 = out[0];
     a.V1 = out[1];
     a.W1 = out[2];
+    a.q7 = q7;
     if (donor1)
         hipLaunchKernelGGL((k_frame_f1<TX, TY, true>), dim3((unsigned)items),
                            dim3(tile::Geo<TX, TY>::NT), 0, s, a);
@@ -573,35 +582,44 @@ int frame_step(mpd_grid_s* g)
     if ((g->ny + BY - 1) / BY > 65535)
         return fail(MPD_ERR_UNSUPPORTED, "FRAME grids need ny <= %d", 65535 * BY);
     FrameIdx x{g->nx, g->ny, g->nz, (size_t)g->nx * g->ny};
-    double* set[2][3];
-    if (g->iord > 1) {
-        for (int d = 0; d < 3; d++) {
-            if ((rc = g->fb[d].alloc(g->nface[d])) != MPD_OK) return rc;
-            MPD_HIP(hipMemsetAsync(g->fb[d].p, 0, g->nface[d] * sizeof(double), s));
-            set[1][d] = g->fb[d].p;
-            if (g->iord > 2) {  // set 0 is written from pass 2 on: work on a copy
-                if ((rc = g->fa[d].alloc(g->nface[d])) != MPD_OK) return rc;
-                MPD_HIP(hipMemcpyAsync(g->fa[d].p, g->face[d].p, g->nface[d] * sizeof(double),
-                                       hipMemcpyDeviceToDevice, s));
-                set[0][d] = g->fa[d].p;
-            } else {
-                set[0][d] = g->face[d].p;
-            }
-        }
-    } else {
-        for (int d = 0; d < 3; d++) set[0][d] = g->face[d].p;
-    }
-    int cold = g->cur;
     // the tiled kernel forms the donor field together with the first pass's velocities;
     // mpd_set_path(g, 1) keeps the per-face kernels
     const bool tiled = g->path != 1 && g->iord > 1;
+    double* set[2][3];
+    for (int d = 0; d < 3; d++) set[0][d] = set[1][d] = g->face[d].p;
+    if (g->iord > 1) {
+        for (int d = 0; d < 3; d++) {
+            if ((rc = g->fb[d].alloc(g->nface[d])) != MPD_OK) return rc;
+            set[1][d] = g->fb[d].p;
+            if (g->iord > 2) {  // set 0 is written from pass 2 on: a buffer of its own
+                if ((rc = g->fa[d].alloc(g->nface[d])) != MPD_OK) return rc;
+                set[0][d] = g->fa[d].p;
+            }
+            // The per-face kernels reproduce the reference's buffers: set 1 zero-filled,
+            // set 0 a copy of the Courant numbers.  The tiled kernel writes every entry
+            // of its output, the never-written column included, so needs neither.
+            if (!tiled) {
+                MPD_HIP(hipMemsetAsync(g->fb[d].p, 0, g->nface[d] * sizeof(double), s));
+                if (g->iord > 2)
+                    MPD_HIP(hipMemcpyAsync(g->fa[d].p, g->face[d].p,
+                                           g->nface[d] * sizeof(double),
+                                           hipMemcpyDeviceToDevice, s));
+            }
+        }
+    }
+    int cold = g->cur;
     for (int o = 0; o < g->iord; o++) {
+        // pass o's velocities: set 1 on odd passes, set 0 on even ones (pass 0: the
+        // Courant numbers or, with more than two passes on the per-face path, their copy)
         double** vel = set[o & 1];
         if (tiled) {
             if (o == 0) continue;  // formed in pass 1's kernel
             const bool first = o == 1;
-            if ((rc = frame_f1(g, first, g->c[cold].p, set[(o - 1) & 1], vel,
-                               first ? g->c[1 - cold].p : nullptr, s)))
+            double* const courant[3] = {g->face[0].p, g->face[1].p, g->face[2].p};
+            double* const* prev = first ? courant : set[(o - 1) & 1];
+            if ((rc = frame_f1(g, first, g->c[cold].p, prev, vel,
+                               first ? g->c[1 - cold].p : nullptr,
+                               (o & 1) ? nullptr : g->face[1].p, s)))
                 return rc;
             if (first) cold = 1 - cold;  // c1 is now the newest field
         } else if (o) {
