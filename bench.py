@@ -383,7 +383,12 @@ def adjoint_leg(M, n, nsteps=512, every=16, K=32, nrec=64):
     8 seeded positive Gaussians added every step; 64 seeded receptors sampled every 16
     steps against a twin run whose emission amplitudes are scaled by (1 + 0.2 xi);
     gradient of J = 1/2 sum (c - obs)^2 w.r.t. c^0 and the emission, checkpoints every 32
-    steps, iord 2 with the limiter.  Reports the backward/forward time ratio, central
+    steps, iord 2 with the limiter.  The gradient run keeps the records of as many final
+    segments as the device memory holds and recomputes the others; it is repeated with
+    only the last segment kept (pure checkpointing, same gradient bit for bit).  The
+    adjoint's cost is the gradient run's time beyond a plain exact forward of the same
+    steps, in forward steps: (forward sweep + backward - plain forward) / plain forward.
+    Also reported: central
     differences along a direction scaled by c^0 (limiter on, and the same with the limiter
     off, where the step is differentiable), and the gradient against the CPU oracle's at
     32^3 x 64 steps."""
@@ -407,7 +412,20 @@ def adjoint_leg(M, n, nsteps=512, every=16, K=32, nrec=64):
     twin.close()
     g = fresh()
     c0 = g.field()
+    # records of as many final segments as the device memory holds are kept from the
+    # forward sweep (the rest are recomputed from their checkpoints)
+    g.set_adjoint_tape(-1)
     J, g0, gq, (tf, tb, tr) = g.receptor_gradient(nsteps, K, rec, every, obs)
+    kept = g.adjoint_tape()[1]
+    # the same gradient with only the last segment's record kept (pure checkpointing)
+    g.set_field(c0)
+    g.set_adjoint_tape(0)
+    J_c, g0_c, gq_c, (tf_c, tb_c, tr_c) = g.receptor_gradient(nsteps, K, rec, every, obs)
+    same = bool(J_c == J and np.array_equal(g0_c, g0) and np.array_equal(gq_c, gq))
+    # a plain forward of the same steps in the driver's (exact) arithmetic
+    g.set_field(c0)
+    g.set_arith(M.ARITH_EXACT)
+    tp = g.step_timed(nsteps)
     g.close()
     d = np.random.default_rng(5).standard_normal(c0.size) * c0
 
@@ -458,10 +476,18 @@ def adjoint_leg(M, n, nsteps=512, every=16, K=32, nrec=64):
     return {"workload": f"periodic {n}^3 deformational wind (seed 5), 8-source emission, "
                         f"{nsteps} steps, {nrec} receptors every {every} steps, checkpoints "
                         f"every {K}, iord 2 nonoscillatory",
-            "J": J, "forward_ms_per_step": tf / nsteps, "backward_ms_per_step": tb / nsteps,
-            "adjoint_over_forward": tb / tf,
+            "J": J, "forward_ms_per_step": tp / nsteps,
+            "forward_sweep_ms_per_step": tf / nsteps, "backward_ms_per_step": tb / nsteps,
+            "adjoint_over_forward": (tf + tb - tp) / tp,
+            "kept_segments": kept, "segments": -(-nsteps // K),
             "recompute_ms_per_step": tr / nsteps,
-            "adjoint_steps_only_over_forward": (tb - tr) / tf,
+            "adjoint_steps_only_over_forward": (tb - tr) / tp,
+            "checkpointing_only": {
+                "forward_sweep_ms_per_step": tf_c / nsteps,
+                "backward_ms_per_step": tb_c / nsteps,
+                "recompute_ms_per_step": tr_c / nsteps,
+                "adjoint_over_forward": (tf_c + tb_c - tp) / tp,
+                "same_gradient_bitwise": same},
             "fd_limiter_on": fd_lim, "fd_limiter_off": fd_unlim, "cpu_reference": cpu,
             "grad_norms": {"c0": float(np.linalg.norm(g0)), "q": float(np.linalg.norm(gq))}}
 

@@ -149,6 +149,16 @@ int mpd_set_adjoint_scatter(mpd_grid g, int atomic);
 /* Kernels of the adjoint's limiter stages: 1 (default) one LDS-tiled kernel, 0 two
  * per-cell passes through device arrays (the same arithmetic, for cross-checks). */
 int mpd_set_adjoint_tiled(mpd_grid g, int tiled);
+/* Device memory mpd_adjoint_run may use for keeping forward records (each step's state
+ * and intermediates, 60 bytes per cell and step with the limiter, 40 without) of whole
+ * checkpoint segments from its forward sweep, beyond the one segment record it always
+ * holds: the segments kept this way are not recomputed in the backward sweep.  0
+ * (default): only the last segment is kept; -1: as many segments as the free device
+ * memory holds, less an 8 GiB reserve.  The gradient is the same bit for bit. */
+int mpd_set_adjoint_tape(mpd_grid g, long long max_bytes);
+/* The tape budget set above and the number of segment records the last mpd_adjoint_run
+ * with gradients kept from its forward sweep (either pointer may be NULL). */
+int mpd_get_adjoint_tape(mpd_grid g, long long* max_bytes, int* kept_segments);
 /* Forward run of nsteps steps from the grid's field recording the receptor values every
  * sample_every steps: out[(s-1)*nrec + r] = c^{s*sample_every}[rec_r]. */
 int mpd_sample_receptors(mpd_grid g, int nsteps, int nrec, const int* receptors,
@@ -156,9 +166,10 @@ int mpd_sample_receptors(mpd_grid g, int nsteps, int nrec, const int* receptors,
 /* Receptor-misfit loss and gradient.  From the grid's field c^0 and its emission q (zero
  * if unset), c^{n+1} = A(c^n) + q for n < nsteps, and
  *   J = 1/2 sum_{s=1}^{nsteps/sample_every} sum_r (c^{s*sample_every}[rec_r] - obs[(s-1)*nrec + r])^2
- * (rec: distinct cell indices).  The forward keeps c^n every checkpoint_every steps; the
- * backward sweep recomputes each segment from its checkpoint and applies the adjoint
- * steps in reverse.  grad_c0 = dJ/dc^0 and grad_q = dJ/dq (host arrays, may be NULL;
+ * (rec: distinct cell indices).  The forward keeps c^n every checkpoint_every steps and
+ * the full record of the last segments (mpd_set_adjoint_tape); the backward sweep
+ * recomputes each other segment from its checkpoint and applies the adjoint steps in
+ * reverse.  grad_c0 = dJ/dc^0 and grad_q = dJ/dq (host arrays, may be NULL;
  * both NULL: loss only).  times (may be NULL, 3 doubles): forward and backward
  * milliseconds, and the part of the backward spent recomputing checkpoint segments.  The
  * forward runs in exact arithmetic; on return the grid holds c^nsteps. */

@@ -88,6 +88,8 @@ SIGNATURES = {
     "mpd_generate_blobs": (C.c_int, [C.c_void_p, C.c_int, C.c_int, PD, PD, PD, C.c_int]),
     "mpd_set_adjoint_scatter": (C.c_int, [C.c_void_p, C.c_int]),
     "mpd_set_adjoint_tiled": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpd_set_adjoint_tape": (C.c_int, [C.c_void_p, C.c_longlong]),
+    "mpd_get_adjoint_tape": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), PI]),
     "mpd_sample_receptors": (C.c_int, [C.c_void_p, C.c_int, C.c_int, PI, C.c_int, PD]),
     "mpd_adjoint_step": (C.c_int, [C.c_void_p, PD, PD, PD, PD]),
     "mpd_adjoint_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, PI, C.c_int, PD, PD,
@@ -309,6 +311,18 @@ class Grid:
 
     def set_adjoint_tiled(self, tiled):
         _check(lib().mpd_set_adjoint_tiled(self.h, int(tiled)))
+
+    def set_adjoint_tape(self, max_bytes):
+        """Bytes receptor_gradient may use for keeping the forward records of whole
+        checkpoint segments beyond the last one (-1: the free device memory less a
+        reserve; 0: the last segment only)."""
+        _check(lib().mpd_set_adjoint_tape(self.h, int(max_bytes)))
+
+    def adjoint_tape(self):
+        """(tape budget in bytes, segment records the last gradient run kept)."""
+        b, k = C.c_longlong(0), C.c_int(0)
+        _check(lib().mpd_get_adjoint_tape(self.h, C.byref(b), C.byref(k)))
+        return b.value, k.value
 
     def sample_receptors(self, nsteps, receptors, sample_every):
         rec = np.ascontiguousarray(receptors, dtype=np.int32)

@@ -16,6 +16,7 @@
 // took, with the oracle's conventions (oracle/cpu_adj.c), so the GPU adjoint is the
 // transpose of the oracle's tangent-linear model.  Arithmetic is unfused IEEE FP64.
 #include <algorithm>
+#include <memory>
 #include <vector>
 
 #include "mpd_formulas.hpp"
@@ -1155,6 +1156,30 @@ struct Work {
     }
 };
 
+// Forward record of one checkpoint segment of K steps: the state entering each step and
+// the step's intermediates.
+struct Record {
+    std::vector<DevBuf> st;
+    std::vector<Fwd> fw;
+    static size_t step_bytes(size_t cells, bool nonosc)
+    {
+        return cells * (nonosc ? 8 * 7 + 4 : 8 * 5);
+    }
+    int alloc(int K, size_t cells, bool nonosc)
+    {
+        st = std::vector<DevBuf>(K);
+        fw = std::vector<Fwd>(K);
+        int rc;
+        for (int m = 0; m < K; m++)
+            if ((rc = st[m].alloc(cells)) || (rc = fw[m].alloc(cells, nonosc))) return rc;
+        return MPD_OK;
+    }
+};
+
+// Device memory the gradient driver leaves free when it sizes the kept records by the
+// free memory.
+constexpr size_t kTapeReserve = (size_t)8 << 30;
+
 // lin = A'(c_in)^T lout given the step's stored forward f; lq += lout (lq may be null)
 int adjoint_step(mpd_grid_s* g, Work& w, const Fwd& f, const double* cin, const double* lout,
                  double* lin, double* lq)
@@ -1229,6 +1254,23 @@ extern "C" int mpd_set_adjoint_tiled(mpd_grid g, int tiled)
     clear_error();
     if (!g) return fail(MPD_ERR_ARG, "mpd_set_adjoint_tiled: NULL grid");
     g->adj_tiled = tiled ? 1 : 0;
+    return MPD_OK;
+}
+
+extern "C" int mpd_set_adjoint_tape(mpd_grid g, long long max_bytes)
+{
+    clear_error();
+    if (!g) return fail(MPD_ERR_ARG, "mpd_set_adjoint_tape: NULL grid");
+    g->adj_tape = max_bytes;
+    return MPD_OK;
+}
+
+extern "C" int mpd_get_adjoint_tape(mpd_grid g, long long* max_bytes, int* kept_segments)
+{
+    clear_error();
+    if (!g) return fail(MPD_ERR_ARG, "mpd_get_adjoint_tape: NULL grid");
+    if (max_bytes) *max_bytes = g->adj_tape;
+    if (kept_segments) *kept_segments = g->adj_kept;
     return MPD_OK;
 }
 
@@ -1356,28 +1398,56 @@ extern "C" int mpd_adjoint_run(mpd_grid g, int nsteps, int checkpoint_every, int
     if ((rc = dobs.alloc((size_t)std::max(nsamp, 1) * nrec)) || (rc = dres.alloc((size_t)std::max(nsamp, 1) * nrec)))
         return rc;
     if (nsamp) MPD_HIP(hipMemcpy(dobs.p, obs, (size_t)nsamp * nrec * 8, hipMemcpyHostToDevice));
-    std::vector<DevBuf> ck(want_grad ? nck : 0);
-    for (auto& b : ck)
-        if ((rc = b.alloc(n))) return rc;
-    // storage of the backward sweep, allocated before the timed sweeps: the adjoint
-    // intermediates, a segment's states and the intermediates of its steps (about ten
-    // arrays per step)
+    // Storage of the backward sweep, allocated before the timed sweeps: the adjoint
+    // intermediates and the forward records of whole segments (each step's state and
+    // intermediates, about ten arrays per step).  Record 0 is the working record: it keeps
+    // the last segment from the forward sweep and then takes each recomputed segment.
+    // Records 1 .. keep-1 keep the segments before the last one from the forward sweep,
+    // as many as the tape budget and the free device memory allow; only the segments
+    // before those are recomputed from their checkpoints.
     Work w;
     DevBuf lam, lnext, gq, tmp, final_state;
-    std::vector<DevBuf> seg(want_grad ? K : 0);
-    std::vector<Fwd> fw(want_grad ? K : 0);
+    std::vector<std::unique_ptr<Record>> recs;
+    int keep = 0;
     if (want_grad) {
         if ((rc = w.alloc(n)) || (rc = lam.alloc(n)) || (rc = lnext.alloc(n)) ||
             (rc = gq.alloc(n)) || (rc = tmp.alloc(n)) || (rc = final_state.alloc(n)))
             return rc;
-        for (int m = 0; m < K; m++)
-            if ((rc = seg[m].alloc(n)) || (rc = fw[m].alloc(n, g->nonosc)))
+        const size_t rec_bytes = (size_t)K * Record::step_bytes(n, g->nonosc);
+        int extra = 0;
+        if (g->adj_tape != 0 && nck > 1) {
+            size_t freeb = 0, totalb = 0;
+            MPD_HIP(hipMemGetInfo(&freeb, &totalb));
+            // the working record and the checkpoints of the recomputed segments are still
+            // to be allocated; a reserve stays free
+            const size_t need = rec_bytes + (size_t)nck * n * 8 + kTapeReserve;
+            size_t fit = freeb > need ? (freeb - need) / rec_bytes : 0;
+            if (g->adj_tape > 0) fit = std::min(fit, (size_t)g->adj_tape / rec_bytes);
+            extra = (int)std::min(fit, (size_t)(nck - 1));
+        }
+        keep = 1 + extra;
+        for (int r = 0; r < keep; r++) {
+            recs.emplace_back(new Record);
+            if ((rc = recs.back()->alloc(K, n, g->nonosc)))
                 return rc == MPD_ERR_NOMEM
                            ? fail(rc, "mpd_adjoint_run: device memory for %d stored steps of "
-                                      "%zu cells exhausted; use a shorter checkpoint interval",
-                                  K, n)
+                                      "%zu cells exhausted; use a shorter checkpoint interval "
+                                      "or a smaller tape budget",
+                                  K * (r + 1), n)
                            : rc;
+        }
     }
+    // segments [first_kept, nck) keep their records from the forward sweep; segment c
+    // of those is held by record nck - 1 - c
+    const int first_kept = nck - keep;
+    g->adj_kept = keep;
+    std::vector<DevBuf> ck(want_grad ? first_kept : 0);
+    for (auto& b : ck)
+        if ((rc = b.alloc(n))) return rc;
+    // c^{cK}: a checkpoint, or the first state of a kept record
+    auto ckpt = [&](int c) -> const double* {
+        return c < first_kept ? ck[c].p : recs[nck - 1 - c]->st[0].p;
+    };
 
     hipEvent_t e0, e1, e1b, e2;
     MPD_HIP(hipEventCreate(&e0));
@@ -1385,21 +1455,34 @@ extern "C" int mpd_adjoint_run(mpd_grid g, int nsteps, int checkpoint_every, int
     MPD_HIP(hipEventCreate(&e1b));
     MPD_HIP(hipEventCreate(&e2));
     MPD_HIP(hipEventRecord(e0, s));
-    // forward sweep
+    // forward sweep: plain steps on the grid's buffers up to the first kept segment, then
+    // steps that store their record
     for (int step = 0; step < nsteps; step++) {
-        if (want_grad && step % K == 0)
-            MPD_HIP(hipMemcpyAsync(ck[step / K].p, g->c[g->cur].p, n * 8, hipMemcpyDeviceToDevice, s));
-        if ((rc = periodic_step(g))) return rc;
+        const int c = step / K, m = step - c * K;
+        const double* out;
+        if (c < first_kept) {
+            if (want_grad && m == 0)
+                MPD_HIP(hipMemcpyAsync(ck[c].p, g->c[g->cur].p, n * 8, hipMemcpyDeviceToDevice, s));
+            if ((rc = periodic_step(g))) return rc;
+            out = g->c[g->cur].p;
+        } else {
+            Record& R = *recs[nck - 1 - c];
+            if (step == first_kept * K)
+                MPD_HIP(hipMemcpyAsync(R.st[0].p, g->c[g->cur].p, n * 8, hipMemcpyDeviceToDevice, s));
+            const int len = std::min(nsteps, (c + 1) * K) - c * K;
+            double* dst = m + 1 < len ? R.st[m + 1].p
+                                      : (c + 1 < nck ? recs[nck - 2 - c]->st[0].p : final_state.p);
+            if ((rc = R.fw[m].step(g, R.st[m].p, dst))) return rc;
+            out = dst;
+        }
         if ((step + 1) % sample_every == 0) {
             int sm = (step + 1) / sample_every - 1;
-            hipLaunchKernelGGL(k_receptors, dim3(nblk(nrec)), dim3(TPB), 0, s, g->c[g->cur].p,
-                               drec, nrec, dobs.p + (size_t)sm * nrec, dres.p + (size_t)sm * nrec,
+            hipLaunchKernelGGL(k_receptors, dim3(nblk(nrec)), dim3(TPB), 0, s, out, drec, nrec,
+                               dobs.p + (size_t)sm * nrec, dres.p + (size_t)sm * nrec,
                                (double*)nullptr);
         }
     }
     MPD_HIP(hipEventRecord(e1, s));
-    if (want_grad)  // c^nsteps: the last residual and the field handed back
-        MPD_HIP(hipMemcpyAsync(final_state.p, g->c[g->cur].p, n * 8, hipMemcpyDeviceToDevice, s));
     std::vector<double> res((size_t)std::max(nsamp, 1) * nrec, 0.0);
     if (nsamp) MPD_HIP(hipMemcpyAsync(res.data(), dres.p, (size_t)nsamp * nrec * 8, hipMemcpyDeviceToHost, s));
     MPD_HIP(hipStreamSynchronize(s));
@@ -1422,39 +1505,43 @@ extern "C" int mpd_adjoint_run(mpd_grid g, int nsteps, int checkpoint_every, int
     if (want_grad) {
         MPD_HIP(hipMemsetAsync(lam.p, 0, n * 8, s));
         MPD_HIP(hipMemsetAsync(gq.p, 0, n * 8, s));
-        // a scratch grid view that steps the segment states: reuse g's buffers
-        const int cur0 = g->cur;
         for (int c = nck - 1; c >= 0; c--) {
             const int s0 = c * K, s1 = std::min(nsteps, s0 + K);
-            // recompute c^{s0} .. c^{s1-1} into seg[0 .. s1-s0-1] with each step's
-            // intermediates (the last step's result, c^{s1}, is not needed again)
-            MPD_HIP(hipEventRecord(rec_ev[2 * c], s));
-            MPD_HIP(hipMemcpyAsync(seg[0].p, ck[c].p, n * 8, hipMemcpyDeviceToDevice, s));
-            for (int m = 0; m < s1 - s0; m++)
-                if ((rc = fw[m].step(g, seg[m].p, m + 1 < s1 - s0 ? seg[m + 1].p : tmp.p)))
-                    return rc;
-            MPD_HIP(hipEventRecord(rec_ev[2 * c + 1], s));
+            // the residual of c^{s1}, the next segment's first state (or the final field),
+            // is added before the recompute below overwrites the working record
+            if (s1 % sample_every == 0) {
+                const double* cs = c + 1 < nck ? ckpt(c + 1) : final_state.p;
+                int sm = s1 / sample_every - 1;
+                hipLaunchKernelGGL(k_receptors, dim3(nblk(nrec)), dim3(TPB), 0, s, cs, drec,
+                                   nrec, dobs.p + (size_t)sm * nrec, (double*)nullptr, lam.p);
+            }
+            Record* R;
+            if (c >= first_kept) {
+                R = recs[nck - 1 - c].get();
+            } else {
+                // recompute c^{s0} .. c^{s1-1} into the working record with each step's
+                // intermediates (the last step's result, c^{s1}, is not needed again)
+                R = recs[0].get();
+                MPD_HIP(hipEventRecord(rec_ev[2 * c], s));
+                MPD_HIP(hipMemcpyAsync(R->st[0].p, ck[c].p, n * 8, hipMemcpyDeviceToDevice, s));
+                for (int m = 0; m < s1 - s0; m++)
+                    if ((rc = R->fw[m].step(g, R->st[m].p, m + 1 < s1 - s0 ? R->st[m + 1].p : tmp.p)))
+                        return rc;
+                MPD_HIP(hipEventRecord(rec_ev[2 * c + 1], s));
+            }
             for (int step = s1; step > s0; step--) {  // lam holds lam^{step}
-                if (step % sample_every == 0) {
-                    // the residual needs c^{step}: c^{s1} is the next segment's first state
-                    // (or the final field); inside the segment it is seg[step - s0]
-                    const double* cs;
-                    if (step == s1)
-                        cs = (c + 1 < nck) ? ck[c + 1].p : nullptr;
-                    else
-                        cs = seg[step - s0].p;
-                    if (!cs) cs = final_state.p;
+                if (step < s1 && step % sample_every == 0) {
                     int sm = step / sample_every - 1;
-                    hipLaunchKernelGGL(k_receptors, dim3(nblk(nrec)), dim3(TPB), 0, s, cs, drec,
-                                       nrec, dobs.p + (size_t)sm * nrec, (double*)nullptr, lam.p);
+                    hipLaunchKernelGGL(k_receptors, dim3(nblk(nrec)), dim3(TPB), 0, s,
+                                       (const double*)R->st[step - s0].p, drec, nrec,
+                                       dobs.p + (size_t)sm * nrec, (double*)nullptr, lam.p);
                 }
-                if ((rc = adjoint_step(g, w, fw[step - 1 - s0], seg[step - 1 - s0].p, lam.p,
+                if ((rc = adjoint_step(g, w, R->fw[step - 1 - s0], R->st[step - 1 - s0].p, lam.p,
                                        lnext.p, gq.p)))
                     return rc;
                 std::swap(lam.p, lnext.p);
             }
         }
-        (void)cur0;
         MPD_HIP(hipEventRecord(e2, s));
         MPD_HIP(hipMemcpyAsync(g->c[g->cur].p, final_state.p, n * 8, hipMemcpyDeviceToDevice, s));
         if (grad_c0) MPD_HIP(hipMemcpyAsync(grad_c0, lam.p, n * 8, hipMemcpyDeviceToHost, s));
@@ -1474,8 +1561,8 @@ extern "C" int mpd_adjoint_run(mpd_grid g, int nsteps, int checkpoint_every, int
     if (times) {
         times[0] = f1;
         times[1] = f2;
-        double fr = 0.0;
-        for (size_t q = 0; q + 1 < rec_ev.size(); q += 2) {
+        double fr = 0.0;  // (events recorded for the recomputed segments only)
+        for (size_t q = 0; q < 2 * (size_t)std::max(first_kept, 0) && q + 1 < rec_ev.size(); q += 2) {
             float f = 0.f;
             (void)hipEventElapsedTime(&f, rec_ev[q], rec_ev[q + 1]);
             fr += f;

@@ -13,6 +13,8 @@ struct mpd_grid_s {
     int arith = MPD_ARITH_FAST;
     int adj_scatter = 0;  // adjoint: 0 deterministic gathers, 1 atomic scatter
     int adj_tiled = 1;    // adjoint limiter stages: 1 one LDS-tiled kernel, 0 per-cell passes
+    long long adj_tape = 0;  // gradient driver: bytes for kept segment records (-1: free memory)
+    int adj_kept = 0;        // segment records the last gradient run kept from its forward
     int tile_x = 0, tile_y = 0, zc = 0;  // fused tile shape and z-chunk (0: default)
     size_t ncell = 0;
     size_t nface[3] = {0, 0, 0};  // elements per face family (layout of the mode)
