@@ -211,10 +211,20 @@ def main():
         g.step(ks - 1)
         gotk = g.field()
         mass, cmin, cmax_ = g.stats()
+        threads = O.get_threads()
+        O.set_threads(1)
         t0 = time.perf_counter()
         wantk = O.per_run(n, n, n, case["U"], case["V"], case["W"], args.iord, True,
                           case["c"], ks)
         t_cpu = time.perf_counter() - t0
+        # the same restatement over z planes on all the threads OpenMP is given
+        nthr = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or threads
+        O.set_threads(nthr)
+        t0 = time.perf_counter()
+        wantp = O.per_run(n, n, n, case["U"], case["V"], case["W"], args.iord, True,
+                          case["c"], ks)
+        t_par = time.perf_counter() - t0
+        O.set_threads(threads)
         par[f"steps_{ks}"] = {"rel_err": relerr(gotk, wantk),
                               "mass_rel_drift": abs(mass - case["c"].sum()) / case["c"].sum(),
                               "min": cmin}
@@ -224,6 +234,13 @@ def main():
             "sample": f"oracle/cpu_ref.c (FP64 restatement of the reference step with the "
                       f"same periodic + limiter options, 1 thread, {host_cpu_model()}) over "
                       f"{ks} steps of the same {n}^3 iord={args.iord} workload",
+        }
+        line["cpu_parallel"] = {
+            "value": cells * ks / t_par, "unit": "cell-updates/s", "cores": nthr,
+            "kind": "port",
+            "sample": f"the same oracle loops with OpenMP over z planes ({nthr} threads, this "
+                      f"build's parallel CPU, not the reference's), same {ks} steps",
+            "bitwise_equal_to_1_thread": bool(np.array_equal(wantp, wantk)),
         }
     g.close()
     line["parity"] = par

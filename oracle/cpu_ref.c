@@ -7,6 +7,9 @@
 
 #include <float.h>
 #include <stdlib.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <string.h>
 
 /* smolar_int.h:25 -- guard added to every ratio denominator. */
@@ -284,6 +287,26 @@ static void wrap_free(wrap_t* w)
 
 #define PI(i, j, k) ((size_t)(k) * nx * ny + (size_t)(j) * nx + (i))
 
+/* Threads of the periodic loops below (OpenMP over z planes; every cell is computed by
+ * the same expression whatever the thread count, so results do not depend on it). */
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+
+int orc_get_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
 /* Face Courant numbers on the periodic grid, same formula and order as uu.c:132-134
  * (x-face i+1/2 uses dx[i]; the face after the last cell wraps to cell 0). */
 void orc_per_courant(int nx, int ny, int nz,
@@ -292,6 +315,7 @@ void orc_per_courant(int nx, int ny, int nz,
                      const double* v1, const double* w0, const double* w1,
                      double* U, double* V, double* W)
 {
+    #pragma omp parallel for schedule(static)
     for (int k = 0; k < nz; k++) {
         int kp = (k + 1) % nz;
         for (int j = 0; j < ny; j++) {
@@ -314,6 +338,7 @@ void orc_per_donor(int nx, int ny, int nz, const double* U, const double* V,
 {
     wrap_t X, Y, Z;
     if (wrap_make(&X, nx) || wrap_make(&Y, ny) || wrap_make(&Z, nz)) return;
+    #pragma omp parallel for schedule(static)
     for (int k = 0; k < nz; k++)
         for (int j = 0; j < ny; j++)
             for (int i = 0; i < nx; i++) {
@@ -339,6 +364,7 @@ void orc_per_antidiff(int nx, int ny, int nz, const double* c, const double* U0,
 {
     wrap_t X, Y, Z;
     if (wrap_make(&X, nx) || wrap_make(&Y, ny) || wrap_make(&Z, nz)) return;
+    #pragma omp parallel for schedule(static)
     for (int k = 0; k < nz; k++) {
         int km = Z.m1[k], kp = Z.p1[k];
         for (int j = 0; j < ny; j++) {
@@ -436,6 +462,7 @@ void orc_per_limiter_beta(int nx, int ny, int nz, const double* cn, const double
 {
     wrap_t X, Y, Z;
     if (wrap_make(&X, nx) || wrap_make(&Y, ny) || wrap_make(&Z, nz)) return;
+    #pragma omp parallel for schedule(static)
     for (int k = 0; k < nz; k++)
         for (int j = 0; j < ny; j++)
             for (int i = 0; i < nx; i++) {
@@ -472,6 +499,7 @@ static inline double limit_face(double u, double bup_l, double bdn_l, double bup
 void orc_per_limiter_apply(int nx, int ny, int nz, const double* bup, const double* bdn,
                            double* U, double* V, double* W)
 {
+    #pragma omp parallel for schedule(static)
     for (int k = 0; k < nz; k++)
         for (int j = 0; j < ny; j++)
             for (int i = 0; i < nx; i++) {

@@ -76,6 +76,9 @@ void orc_per_limiter_apply(int nx, int ny, int nz, const double* bup, const doub
  * then cout += q when q is non-NULL.  U,V,W are the face Courant numbers. */
 int orc_per_step(int nx, int ny, int nz, const double* U, const double* V, const double* W,
                  int iord, int nonosc, const double* q, const double* cin, double* cout);
+/* Threads of the periodic loops (OpenMP over z planes; results do not depend on it). */
+void orc_set_threads(int n);
+int orc_get_threads(void);
 /* nsteps steps in place on c. */
 int orc_per_run(int nx, int ny, int nz, const double* U, const double* V, const double* W,
                 int iord, int nonosc, const double* q, double* c, int nsteps);

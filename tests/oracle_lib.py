@@ -45,6 +45,10 @@ def _bind(lib):
         lib.orc_per_antidiff.argtypes = [C.c_int] * 3 + [PD] * 7
         lib.orc_per_limiter_beta.argtypes = [C.c_int] * 3 + [PD] * 7
         lib.orc_per_limiter_apply.argtypes = [C.c_int] * 3 + [PD] * 5
+        lib.orc_set_threads.argtypes = [C.c_int]
+        lib.orc_set_threads.restype = None
+        lib.orc_get_threads.argtypes = []
+        lib.orc_get_threads.restype = C.c_int
         lib.orc_tlm_step.argtypes = [C.c_int] * 3 + [PD] * 3 + [C.c_int] + [PD] * 4
         lib.orc_tlm_step.restype = C.c_int
         lib.orc_adj_step.argtypes = [C.c_int] * 3 + [PD] * 3 + [C.c_int] + [PD] * 4
@@ -91,6 +95,15 @@ def call_smolar(fn, nx, ny, ns, dx, dy, ds, dt, u0, u1, v0, v1, w0, w1, c0, c1, 
     a1 = C.addressof(pp1.contents.contents)
     by_addr = {c0.ctypes.data: c0, c1.ctypes.data: c1}
     return rc, by_addr[a0], by_addr[a1]
+
+
+def set_threads(n):
+    """Threads of the oracle's periodic loops (OpenMP; results do not depend on it)."""
+    oracle().orc_set_threads(int(n))
+
+
+def get_threads():
+    return oracle().orc_get_threads()
 
 
 def per_step(nx, ny, nz, U, V, W, iord, nonosc, cin, q=None):

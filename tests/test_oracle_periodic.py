@@ -134,3 +134,25 @@ def test_emission_closed_form_and_affinity():
     a = O.per_step(nx, ny, nz, U, V, W, 2, True, c, q)
     b = O.per_step(nx, ny, nz, U, V, W, 2, True, c)
     np.testing.assert_array_equal(a, b + q)
+
+
+def test_thread_count_does_not_change_results():
+    """The oracle's periodic loops run OpenMP over z planes; every cell is the same
+    expression whatever the thread count, so 1, 3 and 4 threads agree bit for bit."""
+    rng = np.random.default_rng(6)
+    nx, ny, nz = 12, 9, 10
+    n = nx * ny * nz
+    U, V, W = (rng.uniform(-0.2, 0.2, n) for _ in range(3))
+    c = rng.uniform(0, 1, n) ** 2
+    q = rng.uniform(0, 0.01, n)
+    saved = O.get_threads()
+    try:
+        out = []
+        for t in (1, 3, 4):
+            O.set_threads(t)
+            assert O.get_threads() == t
+            out.append(O.per_run(nx, ny, nz, U, V, W, 3, True, c, 4, q))
+    finally:
+        O.set_threads(saved)
+    for o in out[1:]:
+        np.testing.assert_array_equal(o, out[0])
