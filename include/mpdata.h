@@ -94,8 +94,16 @@ int mpd_set_path(mpd_grid g, int path);
  * MPD_ARITH_EXACT.  FRAME grids and the staged path always compute exactly. */
 int mpd_set_arith(mpd_grid g, int arith);
 /* Tile shape of the fused kernels (0,0 = the default 32x8); one of the compiled shapes
- * 32x8, 16x16, 64x8, 16x8, 32x16. */
+ * 32x8, 16x16, 64x8, 16x8, 32x16.  Sets it for both kernels of a pass (the limiter
+ * kernel's own shape, if any, is cleared). */
 int mpd_set_tile(mpd_grid g, int tile_x, int tile_y);
+/* Tile shape and z-chunk of the limiter kernel alone (the second fused kernel of each
+ * nonoscillatory pass); 0,0,0 = those of the first kernel (default); zc 0 = chosen from
+ * the grid size.  The results do not depend on the launch shapes. */
+int mpd_set_limiter_tile(mpd_grid g, int tile_x, int tile_y, int zc);
+/* The launch settings: v[0..2] tile_x, tile_y, zc of the first kernel, v[3..5] those of
+ * the limiter kernel (0 = default / same as the first kernel). */
+int mpd_get_tiles(mpd_grid g, int* v);
 /* Whether mpd_step replays blocks of 8 steps as captured HIP graphs (PERIODIC grids on
  * the fused path; default off: on MI355X the replays measure no faster than launching
  * the kernels, whose gaps are the GPU's own dispatch latency).  The results are
@@ -104,8 +112,9 @@ int mpd_set_graph(mpd_grid g, int on);
 /* Planes per workgroup of the fused kernels (0 = chosen from the grid size). */
 int mpd_set_zchunk(mpd_grid g, int zc);
 /* Times every tile shape x z-chunk length on the grid itself (one warm-up step and nsteps
- * timed steps each), keeps the fastest and reports it; the field buffers are restored, so
- * the grid's state is unchanged.  Results are cached per process by shape and options,
+ * timed steps each), keeps for each of the two fused kernels the shape in which it ran
+ * fastest, and reports the first kernel's (mpd_get_tiles gives both); the field buffers
+ * are restored, so the grid's state is unchanged.  Results are cached per process by shape and options,
  * and in the file named by the environment variable MPD_TUNE_CACHE when set.  PERIODIC
  * grids with iord >= 2 on the fused path only (MPD_ERR_UNSUPPORTED otherwise). */
 int mpd_autotune(mpd_grid g, int nsteps, int* tile_x, int* tile_y, int* zc);

@@ -79,6 +79,8 @@ SIGNATURES = {
     "mpd_set_path": (C.c_int, [C.c_void_p, C.c_int]),
     "mpd_set_tile": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "mpd_set_zchunk": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpd_set_limiter_tile": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "mpd_get_tiles": (C.c_int, [C.c_void_p, PI]),
     "mpd_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "mpd_autotune": (C.c_int, [C.c_void_p, C.c_int, PI, PI, PI]),
     "mpd_set_arith": (C.c_int, [C.c_void_p, C.c_int]),
@@ -278,12 +280,24 @@ class Grid:
     def set_zchunk(self, zc):
         _check(lib().mpd_set_zchunk(self.h, zc))
 
+    def set_limiter_tile(self, tx, ty, zc=0):
+        """Launch shape of the limiter kernel alone (0, 0: that of the first kernel)."""
+        _check(lib().mpd_set_limiter_tile(self.h, tx, ty, zc))
+
+    def tiles(self):
+        """((tile_x, tile_y, zc) of the first kernel, the same of the limiter kernel);
+        zeros mean the defaults / the first kernel's."""
+        v = (C.c_int * 6)()
+        _check(lib().mpd_get_tiles(self.h, v))
+        return tuple(v[0:3]), tuple(v[3:6])
+
     def set_graph(self, on):
         _check(lib().mpd_set_graph(self.h, int(on)))
 
     def autotune(self, nsteps=2):
-        """Picks the fastest tile shape and z-chunk length for this grid (state kept);
-        returns (tile_x, tile_y, zc)."""
+        """Picks, for each of the two fused kernels, the fastest tile shape and z-chunk
+        length on this grid (state kept); returns the first kernel's (tile_x, tile_y, zc)
+        (tiles() gives both)."""
         v = [C.c_int() for _ in range(3)]
         _check(lib().mpd_autotune(self.h, nsteps, *(C.byref(x) for x in v)))
         return tuple(x.value for x in v)

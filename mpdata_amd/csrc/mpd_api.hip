@@ -384,8 +384,8 @@ static bool graphable(const mpd_grid_s* g)
 // or copy nodes).
 static int graph_steps(mpd_grid_s* g)
 {
-    const mpd_grid_s::GraphKey key{g->tile_x, g->tile_y, g->zc, g->arith, g->path,
-                                   g->has_q ? g->q.p : nullptr};
+    const mpd_grid_s::GraphKey key{g->tile_x,  g->tile_y,  g->zc,    g->tile2_x, g->tile2_y,
+                                   g->zc2,     g->arith,   g->path,  g->has_q ? g->q.p : nullptr};
     const int par = g->cur;
     hipStream_t s = stream();
     if (!g->gexec[par] || !(g->gkey[par] == key)) {
@@ -599,6 +599,27 @@ extern "C" int mpd_set_tile(mpd_grid g, int tile_x, int tile_y)
     if (tile_x < 0 || tile_y < 0) return fail(MPD_ERR_ARG, "mpd_set_tile: negative tile");
     g->tile_x = tile_x;
     g->tile_y = tile_y;
+    g->tile2_x = g->tile2_y = g->zc2 = 0;  // the limiter kernel follows
+    return MPD_OK;
+}
+
+extern "C" int mpd_set_limiter_tile(mpd_grid g, int tile_x, int tile_y, int zc)
+{
+    GRID_OR_FAIL(g);
+    if (tile_x < 0 || tile_y < 0 || zc < 0 || (tile_x == 0) != (tile_y == 0))
+        return fail(MPD_ERR_ARG, "mpd_set_limiter_tile: bad tile or z-chunk");
+    g->tile2_x = tile_x;
+    g->tile2_y = tile_y;
+    g->zc2 = tile_x ? zc : 0;
+    return MPD_OK;
+}
+
+extern "C" int mpd_get_tiles(mpd_grid g, int* v)
+{
+    GRID_OR_FAIL(g);
+    if (!v) return fail(MPD_ERR_ARG, "mpd_get_tiles: NULL array");
+    const int t[6] = {g->tile_x, g->tile_y, g->zc, g->tile2_x, g->tile2_y, g->zc2};
+    for (int i = 0; i < 6; i++) v[i] = t[i];
     return MPD_OK;
 }
 
@@ -611,9 +632,13 @@ extern "C" int mpd_set_zchunk(mpd_grid g, int zc)
 }
 
 // Autotuner of one grid: times every compiled tile shape x z-chunk length on the grid
-// itself (one warm-up step, then nsteps timed steps each), keeps the fastest, and
-// restores both field buffers and their order, so tuning leaves the grid unchanged.
-// Results are cached by the grid's shape and options (see tune_lookup).
+// itself (one warm-up step, then nsteps steps with each kernel between events of its own)
+// and keeps, for the antidiffusive kernel F1 and for the limiter kernel F2 separately,
+// the launch shape in which that kernel ran fastest: they share no on-chip state, and
+// F1 (fewer registers) fills a CU with more workgroups than F2.  It restores both field
+// buffers and their order, so tuning leaves the grid unchanged.  Results are cached by
+// the grid's shape and options (see tune_lookup): "grid2" entries hold both shapes; a
+// "grid" entry (one shape) serves both kernels.
 extern "C" int mpd_autotune(mpd_grid g, int nsteps, int* tile_x, int* tile_y, int* zc)
 {
     GRID_OR_FAIL(g);
@@ -623,57 +648,75 @@ extern "C" int mpd_autotune(mpd_grid g, int nsteps, int* tile_x, int* tile_y, in
         return fail(MPD_ERR_UNSUPPORTED, "mpd_autotune: tunes the fused PERIODIC path only");
     if (!g->has_winds || !g->has_field)
         return fail(MPD_ERR_ARG, "mpd_autotune: set the winds (or Courant numbers) and the field first");
-    const std::string key = "grid:i" + std::to_string(g->iord) + "n" + std::to_string(g->nonosc) +
-                            "a" + std::to_string(g->arith) + "q" + std::to_string(g->has_q) +
-                            ":" + std::to_string(g->nx) + "x" + std::to_string(g->ny) + "x" +
-                            std::to_string(g->nz);
+    const std::string opts = "i" + std::to_string(g->iord) + "n" + std::to_string(g->nonosc) +
+                             "a" + std::to_string(g->arith) + "q" + std::to_string(g->has_q) +
+                             ":" + std::to_string(g->nx) + "x" + std::to_string(g->ny) + "x" +
+                             std::to_string(g->nz);
+    const std::string key = "grid2:" + opts;
     int rc;
-    int c[3];
-    if (!tune_lookup(key, 3, c)) {
-        hipStream_t s = stream();
-        DevBuf save[2];
-        for (int b = 0; b < 2; b++) {
-            if ((rc = save[b].alloc(g->ncell))) return rc;
-            MPD_HIP(hipMemcpyAsync(save[b].p, g->c[b].p, g->ncell * 8, hipMemcpyDeviceToDevice, s));
-        }
-        const int cur = g->cur;
-        const int shapes[][2] = {
+    int c[6];
+    if (!tune_lookup(key, 6, c)) {
+        if (tune_lookup("grid:" + opts, 3, c)) {
+            c[3] = c[4] = c[5] = 0;
+        } else {
+            hipStream_t s = stream();
+            DevBuf save[2];
+            for (int b = 0; b < 2; b++) {
+                if ((rc = save[b].alloc(g->ncell))) return rc;
+                MPD_HIP(hipMemcpyAsync(save[b].p, g->c[b].p, g->ncell * 8, hipMemcpyDeviceToDevice, s));
+            }
+            const int cur = g->cur;
+            const int shapes[][2] = {
 #define MPD_TILE(X, Y) {X, Y},
-            MPD_FUSED_TILES(MPD_TILE)
+                MPD_FUSED_TILES(MPD_TILE)
 #undef MPD_TILE
-        };
-        double best = 1e300;
-        int bestc[3] = {g->tile_x, g->tile_y, g->zc};
-        for (auto& sh : shapes) {
-            int last = 0;
-            for (int z : {8, 11, 16, 22, 32, 43, 64}) {
-                z = z < g->nz ? z : g->nz;
-                if (z == last) continue;
-                last = z;
-                g->tile_x = sh[0];
-                g->tile_y = sh[1];
-                g->zc = z;
-                double ms = 0;
-                if ((rc = mpd_step(g, 1))) return rc;
-                if ((rc = mpd_step_timed(g, nsteps, &ms))) return rc;
-                if (ms < best) {
-                    best = ms;
-                    bestc[0] = sh[0];
-                    bestc[1] = sh[1];
-                    bestc[2] = z;
+            };
+            // per kernel: the fastest shape (its kernels of all passes summed)
+            double best[2] = {1e300, 1e300};
+            int bestc[2][3] = {{g->tile_x, g->tile_y, g->zc}, {g->tile_x, g->tile_y, g->zc}};
+            for (auto& sh : shapes) {
+                int last = 0;
+                for (int z : {8, 11, 16, 22, 32, 43, 64}) {
+                    z = z < g->nz ? z : g->nz;
+                    if (z == last) continue;
+                    last = z;
+                    g->tile_x = sh[0];
+                    g->tile_y = sh[1];
+                    g->zc = z;
+                    g->tile2_x = g->tile2_y = g->zc2 = 0;
+                    double ms[16];
+                    int nk = 0;
+                    if ((rc = mpd_step(g, 1))) return rc;
+                    if ((rc = mpd_step_kernel_times(g, nsteps, ms, 16, &nk))) return rc;
+                    double t[2] = {0.0, 0.0};  // kernels alternate F1, F2 (or donor) per pass
+                    for (int k = 0; k < nk; k++) t[k & 1] += ms[k];
+                    for (int f = 0; f < 2; f++)
+                        if (t[f] < best[f]) {
+                            best[f] = t[f];
+                            bestc[f][0] = sh[0];
+                            bestc[f][1] = sh[1];
+                            bestc[f][2] = z;
+                        }
                 }
             }
+            for (int b = 0; b < 2; b++)
+                MPD_HIP(hipMemcpyAsync(g->c[b].p, save[b].p, g->ncell * 8, hipMemcpyDeviceToDevice, s));
+            g->cur = cur;
+            MPD_HIP(hipStreamSynchronize(s));
+            for (int i = 0; i < 3; i++) {
+                c[i] = bestc[0][i];
+                c[3 + i] = bestc[1][i];
+            }
+            if (!g->nonosc) c[3] = c[4] = c[5] = 0;  // the second kernel is no F2
         }
-        for (int b = 0; b < 2; b++)
-            MPD_HIP(hipMemcpyAsync(g->c[b].p, save[b].p, g->ncell * 8, hipMemcpyDeviceToDevice, s));
-        g->cur = cur;
-        MPD_HIP(hipStreamSynchronize(s));
-        tune_store(key, 3, bestc);
-        for (int i = 0; i < 3; i++) c[i] = bestc[i];
+        tune_store(key, 6, c);
     }
     g->tile_x = c[0];
     g->tile_y = c[1];
     g->zc = c[2];
+    g->tile2_x = c[3];
+    g->tile2_y = c[4];
+    g->zc2 = c[5];
     if (tile_x) *tile_x = c[0];
     if (tile_y) *tile_y = c[1];
     if (zc) *zc = c[2];

@@ -740,7 +740,10 @@ int MPD_CAT(periodic_fused_step, MPD_ARITH)(mpd_grid_s* g)
     a.ny = g->ny;
     a.nz = g->nz;
     a.np = (long long)g->nx * g->ny;
-    a.zc = g->zc ? g->zc : choose_zc(g, tx, ty);
+    const int zc1 = g->zc ? g->zc : choose_zc(g, tx, ty);
+    // F2's launch shape: its own when set, else F1's
+    const int tx2 = g->tile2_x ? g->tile2_x : tx, ty2 = g->tile2_x ? g->tile2_y : ty;
+    const int zc2 = g->tile2_x ? (g->zc2 ? g->zc2 : choose_zc(g, tx2, ty2)) : zc1;
     const double* q = g->has_q ? g->q.p : nullptr;
     double* cin = g->c[g->cur].p;
     double* out = g->c[1 - g->cur].p;
@@ -759,6 +762,7 @@ int MPD_CAT(periodic_fused_step, MPD_ARITH)(mpd_grid_s* g)
         a.Ax = tgt[0].p;
         a.Ay = tgt[1].p;
         a.Az = tgt[2].p;
+        a.zc = zc1;
         hipEvent_t ev[2];
         const hipEvent_t* use;
         if ((rc = span(g, ev, &use))) return rc;
@@ -773,7 +777,8 @@ int MPD_CAT(periodic_fused_step, MPD_ARITH)(mpd_grid_s* g)
             a.Lx = last ? nullptr : g->fb[0].p;
             a.Ly = last ? nullptr : g->fb[1].p;
             a.Lz = last ? nullptr : g->fb[2].p;
-            if ((rc = launch(a, tx, ty, false, false, s, use))) return rc;
+            a.zc = zc2;
+            if ((rc = launch(a, tx2, ty2, false, false, s, use))) return rc;
             for (int d = 0; d < 3; d++) prev[d] = g->fb[d].p;
         } else {
             if ((rc = donor_faces(g, tgt[0].p, tgt[1].p, tgt[2].p, pass_in,

@@ -16,6 +16,9 @@ struct mpd_grid_s {
     long long adj_tape = 0;  // gradient driver: bytes for kept segment records (-1: free memory)
     int adj_kept = 0;        // segment records the last gradient run kept from its forward
     int tile_x = 0, tile_y = 0, zc = 0;  // fused tile shape and z-chunk (0: default)
+    // tile shape and z-chunk of the limiter kernel F2 (0: those of F1); the two kernels of
+    // a pass share no on-chip state, so each runs in the launch shape that suits it
+    int tile2_x = 0, tile2_y = 0, zc2 = 0;
     size_t ncell = 0;
     size_t nface[3] = {0, 0, 0};  // elements per face family (layout of the mode)
 
@@ -41,12 +44,12 @@ struct mpd_grid_s {
     // time loop captured as a HIP graph (PERIODIC fused path): a block of steps from each
     // field parity, with the launch configuration it was captured with
     struct GraphKey {
-        int tx, ty, zc, arith, path;
+        int tx, ty, zc, tx2, ty2, zc2, arith, path;
         const double* q;
         bool operator==(const GraphKey& o) const
         {
-            return tx == o.tx && ty == o.ty && zc == o.zc && arith == o.arith &&
-                   path == o.path && q == o.q;
+            return tx == o.tx && ty == o.ty && zc == o.zc && tx2 == o.tx2 && ty2 == o.ty2 &&
+                   zc2 == o.zc2 && arith == o.arith && path == o.path && q == o.q;
         }
     };
     bool use_graph = false;

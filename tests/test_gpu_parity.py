@@ -344,3 +344,56 @@ def test_graph_replays_match_eager_steps(iord, nonosc):
             np.testing.assert_array_equal(gs[0].field(), want)
         for g in gs:
             g.close()
+
+
+@pytest.mark.parametrize("iord", [2, 3])
+def test_limiter_kernel_in_its_own_launch_shape(iord):
+    """The limiter kernel may run in a tile shape and z-chunk of its own: exact arithmetic
+    gives the same fields bit for bit as one shared shape, fast arithmetic stays within
+    the tolerance of the oracle, and graph replays with mixed shapes match eager steps."""
+    for dims in [(70, 19, 9), (33, 17, 3), (40, 24, 13)]:
+        nx, ny, nz = dims
+        U, V, W, c = per_case(nx, ny, nz, 29 + nx)
+        want = O.per_run(nx, ny, nz, U, V, W, iord, True, c, 10)  # graph: a block of 8 + 2
+        for arith in (M.ARITH_EXACT, M.ARITH_FAST):
+            outs = []
+            for lim, graph in (((0, 0, 0), False), ((16, 16, 5), False), ((64, 8, 0), False),
+                               ((16, 8, 2), True)):
+                print(f"case dims={dims} iord={iord} arith={arith} limiter={lim} graph={graph}",
+                      flush=True)
+                g = M.Grid(nx, ny, nz, M.PERIODIC, iord=iord, nonosc=True)
+                g.set_path(2)
+                g.set_arith(arith)
+                g.set_tile(32, 8)
+                g.set_zchunk(4)
+                g.set_limiter_tile(*lim)
+                assert g.tiles() == ((32, 8, 4), lim if lim[0] else (0, 0, 0))
+                g.set_graph(graph)
+                g.set_courant(U, V, W)
+                g.set_field(c)
+                g.step(10)
+                outs.append(g.field())
+                g.close()
+            for o in outs:
+                if arith == M.ARITH_EXACT:
+                    np.testing.assert_array_equal(o, want)
+                else:
+                    assert relerr(o, want) <= TOL
+
+
+def test_autotune_reports_both_kernels_shapes():
+    nx, ny, nz = 40, 24, 13
+    U, V, W, c = per_case(nx, ny, nz, 31)
+    g = M.Grid(nx, ny, nz, M.PERIODIC, iord=2, nonosc=True)
+    g.set_courant(U, V, W)
+    g.set_field(c)
+    before = g.field()
+    f1 = g.autotune(1)
+    t1, t2 = g.tiles()
+    assert t1 == f1
+    shapes = {(32, 8), (16, 16), (64, 8), (16, 8), (32, 16)}
+    assert t1[:2] in shapes and t2[:2] in shapes and 1 <= t1[2] <= nz and 1 <= t2[2] <= nz
+    np.testing.assert_array_equal(g.field(), before)  # tuning leaves the state unchanged
+    g.step(3)
+    assert relerr(g.field(), O.per_run(nx, ny, nz, U, V, W, 2, True, c, 3)) <= TOL
+    g.close()
