@@ -15,8 +15,11 @@ Fixtures (numpy .npz, no pickled objects):
                     region after 3 steps: 16^3 and 24x20x1, orders 1..3.
   ring2d_32.npz     the ring2d driver's configuration at n=32 (reference generators
                     genringf2d/gengauss2d, order 3, dt 0.0125) and c after 1, 10, 50 steps.
+  ring2d_128_1024.npz
+                    the reference's own run `ring2d 128 1024`: sha256 of the ring2d.bin
+                    and of the standard output it writes, and the final c plane.
 
-Usage: python tests/golden/make_golden.py
+Usage: python tests/golden/make_golden.py [ring2d_128]   (ring2d_128: that fixture only)
 """
 import ctypes as C
 import os
@@ -101,8 +104,9 @@ def periodic_cases(rng):
     return d
 
 
-def ring2d_case(n=32, steps=(1, 10, 50)):
-    """tests/init.c:15-103 for m=3: genringf2d wind, gengauss2d cone, replicated layers."""
+def ring2d_setup(n):
+    """tests/init.c:15-103 for m=3: genringf2d wind, gengauss2d cone, replicated layers.
+    Returns (dx, dy, ds, u, v, w, c, xof, yof)."""
     lib = O.ref()
     PD = O.PD
     lib.genringf2d.argtypes = [C.c_int] * 4 + [PD, PD, C.c_double, C.c_double, PD, PD,
@@ -128,6 +132,12 @@ def ring2d_case(n=32, steps=(1, 10, 50)):
     lib.gengauss2d(n // 2 + xof, n // 2 + yof, n, n, radius, 6.0, 6.0, O.ptr(c), 1.0)
     for i in range(1, m):
         c[i * n2:(i + 1) * n2] = c[:n2]
+    return dx, dy, ds, u, v, w, c, xof, yof
+
+
+def ring2d_case(n=32, steps=(1, 10, 50)):
+    m = 3
+    dx, dy, ds, u, v, w, c, _, _ = ring2d_setup(n)
     d = {"dims": np.array([n, n, m]), "dt": np.array(0.0125), "u": u, "v": v, "w": w,
          "c": c, "order": np.array(3)}
     for s in steps:
@@ -135,9 +145,42 @@ def ring2d_case(n=32, steps=(1, 10, 50)):
     return d
 
 
+def ring2d_run(n=128, nt=1024):
+    """The reference's own headline run, `ring2d 128 1024` (tests/ring2d/test.c:10-71),
+    with the reference's generators and smolar(): the sha256 of the ring2d.bin it writes
+    (per step the u, v and c planes of layer 1, after the initial state), the sha256 of
+    its standard output ("step 0", then "step <it> <probe cell, %f>"), and the final c
+    plane of layer 1."""
+    import hashlib
+    m = 3
+    n2 = n * n
+    dx, dy, ds, u, v, w, c, xof, yof = ring2d_setup(n)
+    lib = O.ref()
+    binh, outh = hashlib.sha256(), hashlib.sha256()
+    u1, v1 = u[n2:2 * n2].tobytes(), v[n2:2 * n2].tobytes()
+    binh.update(u1 + v1 + c[n2:2 * n2].tobytes())
+    outh.update(b"step 0\n")
+    c0, c1 = c.copy(), np.zeros_like(c)
+    probe = n // 2 + xof + n * (n // 2 + yof) + n2
+    for it in range(1, nt + 1):
+        rc, a0, a1 = O.call_smolar(lib.smolar, n, n, m, dx, dy, ds, 0.0125, u, u, v, v, w, w,
+                                   c0, c1, 3)
+        assert rc == 0
+        binh.update(u1 + v1 + a1[n2:2 * n2].tobytes())
+        outh.update(("step %d %f\n" % (it, a1[probe])).encode())
+        c0, c1 = a1, a0  # ring2d/test.c:62-63
+    return {"dims": np.array([n, n, m]), "nt": np.array(nt),
+            "bin_sha256": np.array(binh.hexdigest()),
+            "stdout_sha256": np.array(outh.hexdigest()),
+            "c_final_layer1": c0[n2:2 * n2].copy()}
+
+
 def main():
     if not O.have_ref():
         sys.exit("oracle/_ref/libsmolar_ref.so not built (make -C oracle ref)")
+    if sys.argv[1:] == ["ring2d_128"]:  # this fixture alone
+        np.savez_compressed(os.path.join(HERE, "ring2d_128_1024.npz"), **ring2d_run())
+        return
     rng = np.random.default_rng(20101)
     np.savez_compressed(os.path.join(HERE, "frame_3d.npz"),
                         **frame_case(rng, 13, 11, 9, (1, 2, 3, 4), (1, 12), False))
@@ -145,6 +188,7 @@ def main():
                         **frame_case(rng, 17, 15, 3, (1, 2, 3), (1, 12), True))
     np.savez_compressed(os.path.join(HERE, "periodic.npz"), **periodic_cases(rng))
     np.savez_compressed(os.path.join(HERE, "ring2d_32.npz"), **ring2d_case())
+    np.savez_compressed(os.path.join(HERE, "ring2d_128_1024.npz"), **ring2d_run())
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

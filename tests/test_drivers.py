@@ -107,3 +107,57 @@ def test_ring3d_matches_oracle(tmp_path):
                                   v, v, w, w, c0, c1, 3)
         c0, c1 = a1, a0
         np.testing.assert_array_equal(raw[(4 + s) * nn:(5 + s) * nn], c0)
+
+
+def _ring2d_128_fixture():
+    return dict(np.load(os.path.join(GOLD, "ring2d_128_1024.npz")))
+
+
+def test_oracle_reproduces_reference_ring2d_128_1024(tmp_path):
+    """The reference's own headline run, `ring2d 128 1024`: the oracle, started from the
+    driver's restated set-up (written with zero steps, no GPU), writes the same ring2d.bin
+    and the same standard output, hash for hash, as the reference's generators and
+    smolar() did when the fixture was made."""
+    import hashlib
+    d = _ring2d_128_fixture()
+    n, m, nt = 128, 3, int(d["nt"])
+    r = run("ring2d", n, 0, tmp_path)
+    assert r.returncode == 0, r.stderr
+    n2 = n * n
+    u1, v1, c1 = np.fromfile(tmp_path / "ring2d.bin").reshape(3, n2)
+    u, v, c = np.tile(u1, m), np.tile(v1, m), np.tile(c1, m)  # layers replicated (init.c)
+    w = np.zeros(n2 * m)
+    one_n, one_m = np.ones(n), np.ones(m)
+    binh, outh = hashlib.sha256(), hashlib.sha256()
+    binh.update(u1.tobytes() + v1.tobytes() + c1.tobytes())
+    outh.update(b"step 0\n")
+    xof, yof = n // 4, 0
+    probe = n // 2 + xof + n * (n // 2 + yof) + n2
+    c0, cc = c.copy(), np.zeros_like(c)
+    for it in range(1, nt + 1):
+        _, a0, a1 = O.call_smolar(O.oracle().orc_smolar, n, n, m, one_n, one_n, one_m, 0.0125,
+                                  u, u, v, v, w, w, c0, cc, 3)
+        binh.update(u1.tobytes() + v1.tobytes() + a1[n2:2 * n2].tobytes())
+        outh.update(("step %d %f\n" % (it, a1[probe])).encode())
+        c0, cc = a1, a0
+    np.testing.assert_array_equal(c0[n2:2 * n2], d["c_final_layer1"])
+    assert binh.hexdigest() == str(d["bin_sha256"])
+    assert outh.hexdigest() == str(d["stdout_sha256"])
+
+
+@pytest.mark.gpu
+def test_ring2d_128_1024_matches_reference_run(tmp_path):
+    """The ring2d driver on the HIP library at the reference's headline size writes the
+    reference's ring2d.bin (403 MB: every step's planes) and standard output exactly."""
+    import hashlib
+    if M.device_count() < 1:
+        pytest.skip("no GPU")
+    d = _ring2d_128_fixture()
+    r = run("ring2d", 128, int(d["nt"]), tmp_path)
+    assert r.returncode == 0, r.stderr
+    h = hashlib.sha256()
+    with open(tmp_path / "ring2d.bin", "rb") as f:
+        for chunk in iter(lambda: f.read(1 << 24), b""):
+            h.update(chunk)
+    assert hashlib.sha256(r.stdout.encode()).hexdigest() == str(d["stdout_sha256"])
+    assert h.hexdigest() == str(d["bin_sha256"])
