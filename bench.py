@@ -356,6 +356,7 @@ def deform_leg(M, nsteps=200):
             "cell_updates_per_s": cells / (ms * 1e-3), "kernel_ms": kms,
             "achieved_GB/s": ach, "frac_of_hbm_peak": ach / HBM_PEAK_GBS,
             "rel_err_1_step_vs_oracle": err, "oracle_cell_updates_per_s": cells / t_cpu,
+            "oracle_threads": O.get_threads(),
             "mass_rel_drift": abs(mass - mass0) / mass0, "min": cmin}
 
 
