@@ -84,6 +84,15 @@ int mpd_calibration_copy(size_t n, int reps, double* ms);
 /* Mass (sum of c), minimum and maximum of the field, reduced on the device: the
  * failure sentinel for long runs (MPD_ERR_NUMERIC if any value is non-finite). */
 int mpd_field_stats(mpd_grid g, double* mass, double* cmin, double* cmax);
+/* Step sentinel for long runs, checked on the device without host synchronisation:
+ * after every `every` steps of mpd_step (0 = off, the default) the field is scanned for
+ * non-finite values (and negative ones when nonnegative is set); the first step count,
+ * since this call, after which one was seen is kept.  Arming resets it.  (Steps are then
+ * launched one by one: no graph replay.) */
+int mpd_set_sentinel(mpd_grid g, int every, int nonnegative);
+/* Waits for the grid's work; MPD_OK with *first_bad_step = -1 while the sentinel saw
+ * nothing, else MPD_ERR_NUMERIC with the step count (first_bad_step may be NULL). */
+int mpd_sentinel_check(mpd_grid g, long long* first_bad_step);
 /* Kernel-path selection.  PERIODIC grids: 0 = automatic (fused), 1 = staged (one
  * kernel per stage, intermediates in HBM), 2 = fused; grids with more than 2^29 cells
  * per z-plane always take the staged path.  FRAME grids: 1 = the per-face

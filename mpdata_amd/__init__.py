@@ -76,6 +76,8 @@ SIGNATURES = {
     "mpd_field_device_ptr": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "mpd_step_timed": (C.c_int, [C.c_void_p, C.c_int, PD]),
     "mpd_field_stats": (C.c_int, [C.c_void_p, PD, PD, PD]),
+    "mpd_set_sentinel": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "mpd_sentinel_check": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong)]),
     "mpd_set_path": (C.c_int, [C.c_void_p, C.c_int]),
     "mpd_set_tile": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "mpd_set_zchunk": (C.c_int, [C.c_void_p, C.c_int]),
@@ -270,6 +272,21 @@ class Grid:
         m, lo, hi = C.c_double(), C.c_double(), C.c_double()
         _check(lib().mpd_field_stats(self.h, C.byref(m), C.byref(lo), C.byref(hi)))
         return m.value, lo.value, hi.value
+
+    def set_sentinel(self, every, nonnegative=False):
+        """Device check of the field after every `every` steps (0: off) for non-finite
+        (and, with nonnegative, negative) values."""
+        _check(lib().mpd_set_sentinel(self.h, int(every), int(nonnegative)))
+
+    def sentinel_check(self):
+        """The first step count (since arming) after which the sentinel saw a bad value,
+        or None."""
+        step = C.c_longlong(-1)
+        rc = lib().mpd_sentinel_check(self.h, C.byref(step))
+        if rc == 5:  # MPD_ERR_NUMERIC: the sentinel fired
+            return step.value
+        _check(rc)
+        return None
 
     def set_path(self, path):
         _check(lib().mpd_set_path(self.h, path))

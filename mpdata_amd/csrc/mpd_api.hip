@@ -374,7 +374,7 @@ constexpr int kGraphSteps = 8;
 static bool graphable(const mpd_grid_s* g)
 {
     return g->use_graph && g->boundary == MPD_PERIODIC && g->iord >= 2 && g->path != 1 &&
-           fused_plane_fits(g->nx, g->ny) && !g->trace;
+           fused_plane_fits(g->nx, g->ny) && !g->trace && !g->sentinel_every;
 }
 
 // kGraphSteps steps as one graph launch.  The graph of the current parity is captured on
@@ -427,9 +427,41 @@ extern "C" int mpd_step(mpd_grid g, int nsteps)
     if (graphable(g))
         for (; done + kGraphSteps <= nsteps; done += kGraphSteps)
             if ((rc = graph_steps(g))) return rc;
-    for (; done < nsteps; done++)
+    for (; done < nsteps; done++) {
         if ((rc = step_once(g))) return rc;
+        if (g->sentinel_every && ++g->sentinel_steps % g->sentinel_every == 0 &&
+            (rc = sentinel_check_launch(g)))
+            return rc;
+    }
     return MPD_OK;
+}
+
+extern "C" int mpd_set_sentinel(mpd_grid g, int every, int nonnegative)
+{
+    GRID_OR_FAIL(g);
+    if (every < 0) return fail(MPD_ERR_ARG, "mpd_set_sentinel: every %d < 0", every);
+    if (every && !g->sentinel_first)
+        MPD_HIP(hipMalloc((void**)&g->sentinel_first, sizeof(unsigned long long)));
+    if (g->sentinel_first)
+        MPD_HIP(hipMemsetAsync(g->sentinel_first, 0xff, sizeof(unsigned long long), stream()));
+    g->sentinel_every = every;
+    g->sentinel_nonneg = nonnegative ? 1 : 0;
+    g->sentinel_steps = 0;
+    return MPD_OK;
+}
+
+extern "C" int mpd_sentinel_check(mpd_grid g, long long* first_bad_step)
+{
+    GRID_OR_FAIL(g);
+    if (first_bad_step) *first_bad_step = -1;
+    if (!g->sentinel_first) return MPD_OK;
+    unsigned long long f = ~0ull;
+    MPD_HIP(hipMemcpyAsync(&f, g->sentinel_first, sizeof f, hipMemcpyDeviceToHost, stream()));
+    MPD_HIP(hipStreamSynchronize(stream()));
+    if (f == ~0ull) return MPD_OK;
+    if (first_bad_step) *first_bad_step = (long long)f;
+    return fail(MPD_ERR_NUMERIC, "step sentinel: %s value in the field after step %llu",
+                g->sentinel_nonneg ? "a non-finite or negative" : "a non-finite", f);
 }
 
 extern "C" int mpd_set_graph(mpd_grid g, int on)

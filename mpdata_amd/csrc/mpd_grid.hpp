@@ -35,6 +35,13 @@ struct mpd_grid_s {
     mpd::DevBuf bup, bdn;      // limiter ratios
     mpd::DevBuf red;           // reduction partials
 
+    // step sentinel: every `sentinel_every` steps (0: off) a device check records in
+    // *sentinel_first the lowest step count, since the sentinel was armed, after which a
+    // value was non-finite (or negative, with sentinel_nonneg); ~0 while none was
+    unsigned long long* sentinel_first = nullptr;
+    int sentinel_every = 0, sentinel_nonneg = 0;
+    long long sentinel_steps = 0;
+
     // when set, the kernel launches of a step are timed: the fused path launches each
     // kernel between a start and a stop event of its own (pairs appended to *trace,
     // trace_pairs set); the other paths record one event after each launch
@@ -63,6 +70,7 @@ struct mpd_grid_s {
     {
         for (hipGraphExec_t& e : gexec)
             if (e) (void)hipGraphExecDestroy(e);
+        if (sentinel_first) (void)hipFree(sentinel_first);
     }
 };
 
@@ -77,6 +85,8 @@ int courant_from_winds(mpd_grid_s* g, const double* dx, const double* dy, const 
 int frame_step(mpd_grid_s* g);
 int periodic_step(mpd_grid_s* g);
 int field_stats(mpd_grid_s* g, double* mass, double* cmin, double* cmax);
+// Enqueues the step sentinel's check of the current field (step count g->sentinel_steps).
+int sentinel_check_launch(mpd_grid_s* g);
 // Records a timing event after a kernel launch when g->trace is set.
 void trace_mark(mpd_grid_s* g);
 

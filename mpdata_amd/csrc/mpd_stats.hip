@@ -47,7 +47,30 @@ __global__ void k_stats(const double* __restrict__ c, size_t n, double* __restri
     }
 }
 
+// Step sentinel: flags the step count `step` in *first (atomic minimum) if any value of
+// c is non-finite, or negative when nonneg is set; one atomic per wave that saw one.
+__global__ void k_sentinel(const double* __restrict__ c, size_t n, int nonneg,
+                           unsigned long long step, unsigned long long* __restrict__ first)
+{
+    bool bad = false;
+    for (size_t t = (size_t)blockIdx.x * TPB + threadIdx.x; t < n; t += (size_t)gridDim.x * TPB) {
+        const double v = c[t];
+        bad = bad || !isfinite(v) || (nonneg && v < 0.0);
+    }
+    const unsigned long long b = __ballot(bad);
+    if (b && (int)(threadIdx.x % 64) == __ffsll((long long)b) - 1) atomicMin(first, step);
+}
+
 }  // namespace
+
+int sentinel_check_launch(mpd_grid_s* g)
+{
+    hipLaunchKernelGGL(k_sentinel, dim3(NBLK), dim3(TPB), 0, stream(), g->c[g->cur].p, g->ncell,
+                       g->sentinel_nonneg, (unsigned long long)g->sentinel_steps,
+                       g->sentinel_first);
+    MPD_HIP(hipGetLastError());
+    return MPD_OK;
+}
 
 int field_stats(mpd_grid_s* g, double* mass, double* cmin, double* cmax)
 {
