@@ -110,14 +110,14 @@ __device__ __forceinline__ void f1_tile(const FusedArgs& a, int item)
     // own tile cell
     const int rx = 2 + t % TX, ry = 2 + t / TX;
     const int L = ry * RX + rx;
-    const unsigned colb = column<TX, TY>(a.nx, a.ny, bx, by, rx, ry);
+    unsigned colb = column<TX, TY>(a.nx, a.ny, bx, by, rx, ry);
     const bool store = bx * TX + rx - 2 < a.nx && by * TY + ry - 2 < a.ny;
     // own halo cell: ring 1 (loads, donor) or ring 2 (loads), if any
     const bool hl = t < G::NH, h1 = t < G::H1;
     int hx = 0, hy = 0;
     if (hl) G::halo(t, hx, hy);
     const int HL = hy * RX + hx;
-    const unsigned hcolb = column<TX, TY>(a.nx, a.ny, bx, by, hx, hy);
+    unsigned hcolb = column<TX, TY>(a.nx, a.ny, bx, by, hx, hy);
 
     const int z0 = bz * a.zc;
     const int z1 = min(z0 + a.zc, a.nz);
@@ -162,6 +162,8 @@ __device__ __forceinline__ void f1_tile(const FusedArgs& a, int item)
         // the plane index is the same in every lane; without the hint the compiler keeps
         // it, and the LDS slots derived from it, in vector registers
         const int p = __builtin_amdgcn_readfirstlane(pl);
+        keep32(colb);
+        keep32(hcolb);
         const int s3 = (p + 3) % 3;
         const double c0n = nc0, un = nu, vn = nv, wn = nw, fld = nf;
         const double hc0n = hnc0, hun = hnu, hvn = hnv, hwn = hnw, hfld = hnf;
@@ -315,13 +317,13 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
     tile_of(a, item, bx, by, bz);
     const int rx = 2 + t % TX, ry = 2 + t / TX;
     const int L = ry * RX + rx;
-    const unsigned colb = column<TX, TY>(a.nx, a.ny, bx, by, rx, ry);
+    unsigned colb = column<TX, TY>(a.nx, a.ny, bx, by, rx, ry);
     const bool store = bx * TX + rx - 2 < a.nx && by * TY + ry - 2 < a.ny;
     const bool hl = t < G::NH, h1 = t < G::H1;
     int hx = 0, hy = 0;
     if (hl) G::halo(t, hx, hy);
     const int HL = hy * RX + hx;
-    const unsigned hcolb = column<TX, TY>(a.nx, a.ny, bx, by, hx, hy);
+    unsigned hcolb = column<TX, TY>(a.nx, a.ny, bx, by, hx, hy);
 
     const int z0 = bz * a.zc;
     const int z1 = min(z0 + a.zc, a.nz);
@@ -362,8 +364,14 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
             hnaz = ld(a.Az + zp0.off, hcolb);
         }
     }
-    for (int pl = z0 - 1; pl <= z1; pl++) {
+    // the plane loop, unrolled by hand (two planes per trip, an odd last plane after
+    // it): across the two copies the compiler renames the registers the planes below
+    // and above rotate through instead of copying them (the barriers inside keep it
+    // from unrolling a loop of unknown trip count itself)
+    auto plane = [&](const int pl) __attribute__((always_inline)) {
         const int p = __builtin_amdgcn_readfirstlane(pl);  // uniform, as in f1_tile
+        keep32(colb);
+        keep32(hcolb);
         const int s2 = (p + 2) % 2, s3 = (p + 3) % 3;
         const double cnn = ncn, cpn = ncp, axn = nax, ayn = nay, azc = naz;
         const double hcnn = hncn, hcpn = hncp, haxn = hnax, hayn = hnay, hazc = hnaz;
@@ -498,7 +506,13 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
         zp1.next();
         zp0.next();
         zm1.next();
+    };
+    int pl = z0 - 1;
+    for (; pl + 1 <= z1; pl += 2) {
+        plane(pl);
+        plane(pl + 1);
     }
+    if (pl <= z1) plane(pl);
 }
 
 // F1 is held to 128 registers (4 waves per SIMD; a 16-byte spill in the donor variant),

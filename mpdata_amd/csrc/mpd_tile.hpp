@@ -151,6 +151,13 @@ struct ZCursor {
     }
 };
 
+// Marks a 32-bit column offset as redefined at this point (an empty asm statement; it
+// emits no instruction).  Called at the top of a plane loop, it keeps the offset's
+// zero-extension inside the loop, where the loads see "uniform plane base + 32-bit
+// offset" and use the scalar-base addressing form; hoisted out of the loop, the offset
+// became a 64-bit vector value and every load and store needed a 64-bit vector add.
+__device__ __forceinline__ void keep32(unsigned& off) { asm volatile("" : "+v"(off)); }
+
 // Element of a plane at byte offset colb of the thread's column (a 32-bit offset: the
 // host keeps every plane below 4 GiB), so the access is a scalar base plus a vector
 // offset.
