@@ -98,8 +98,19 @@ __device__ __forceinline__ double limit_face(double u, double bup_l, double bdn_
 __device__ __forceinline__ void flux_sums(double fxl, double fxr, double fyl, double fyr,
                                           double fzl, double fzr, double& ain, double& aout)
 {
+#ifdef MPD_FAST_DIV
+    // fast arithmetic: ain + aout = sum of |f| over the six faces and ain - aout = the net
+    // inflow (f_lower - f_upper summed), so both follow from two sums of six (the absolute
+    // values are operand modifiers) instead of twelve selections; clamped at zero, where
+    // the difference of the rounded sums can fall a rounding below it
+    const double sa = ((ab(fxl) + ab(fxr)) + (ab(fyl) + ab(fyr))) + (ab(fzl) + ab(fzr));
+    const double sd = ((fxl - fxr) + (fyl - fyr)) + (fzl - fzr);
+    ain = pos(0.5 * (sa + sd));
+    aout = pos(0.5 * (sa - sd));
+#else
     ain = ((pos(fxl) - neg(fxr)) + (pos(fyl) - neg(fyr))) + (pos(fzl) - neg(fzr));
     aout = ((pos(fxr) - neg(fxl)) + (pos(fyr) - neg(fyl))) + (pos(fzr) - neg(fzl));
+#endif
 }
 
 // Limiter ratios of one cell from the bounds hi/lo, its value c and the six donor

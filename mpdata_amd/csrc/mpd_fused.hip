@@ -229,46 +229,79 @@ __device__ __forceinline__ void f1_tile(const FusedArgs& a, int item)
                 double av = uc;
                 double vr = vk[L + 1 - RX], vru = vk[L + 1], vu = vc, vcc = vk[L - RX];
                 double wr = wmr, wrf = wkr, wf = wm1, wc = wmm;
+                double cr = cc[L + 1], cru = cc[L + 1 + RX], cu = cc[L + RX];
+                double cd = cc[L - RX], crd = cc[L + 1 - RX];
+                double crf = cq[L + 1], cf = c1p, cb = c1m, crb = cm[L + 1];
+#ifdef MPD_FAST_DIV
+                // fast arithmetic: stroke pairs and cell pairs shared between the averages,
+                // the divergence term and the ratio terms (the same sums, reassociated)
+                const double v1 = vr + vcc, v2 = vru + vu, w1 = wr + wc, w2 = wrf + wf;
+                const double b1 = cru + cu, b2 = cd + crd, e1 = crf + cf, e2 = cb + crb;
+                double sv = 0.25 * (v1 + v2);
+                double sw = 0.25 * (w1 + w2);
+                double div = 0.25 * av * ((uk[L + 1] - uk[L - 1]) + (v1 - v2) + (w1 - w2));
+                st(a.Ax + zm1.off, colb,
+                   antidiff(div, av, cr, c, sv, b1 - b2, b1 + b2, sw, e1 - e2, e1 + e2));
+#else
                 double sv = 0.25 * (vr + vru + vu + vcc);
                 double sw = 0.25 * (wr + wrf + wf + wc);
                 double div = 0.25 * av * (uk[L + 1] - uk[L - 1] + vr - vru - vu + vcc + wr - wrf -
                                           wf + wc);
-                double cr = cc[L + 1], cru = cc[L + 1 + RX], cu = cc[L + RX];
-                double cd = cc[L - RX], crd = cc[L + 1 - RX];
-                double crf = cq[L + 1], cf = c1p, cb = c1m, crb = cm[L + 1];
                 st(a.Ax + zm1.off, colb,
                    antidiff(div, av, cr, c, sv, cru + cu - cd - crd, cru + cu + cd + crd, sw,
                             crf + cf - cb - crb, crf + cf + cb + crb));
+#endif
             }
             {  // y-face (vv.c:85-92)
                 double av = vc;
                 double uu = uk[L - 1 + RX], uru = uk[L + RX], ur = uc, ucc = uk[L - 1];
                 double ulu = uk[L - 2 + RX], ul = uk[L - 2];
                 double wu = wmu, wuf = wku, wf = wm1, wc = wmm;
+                double cu = cc[L + RX], cru = cc[L + 1 + RX], cr = cc[L + 1];
+                double cl = cc[L - 1], clu = cc[L - 1 + RX];
+                double cuf = cq[L + RX], cf = c1p, cb = c1m, cub = cm[L + RX];
+#ifdef MPD_FAST_DIV
+                // (the divergence term's x strokes are shifted one cell left: uu, ucc shared)
+                const double u1 = uu + ucc, w1 = wu + wc, w2 = wuf + wf;
+                const double b1 = cru + cr, b2 = cl + clu, e1 = cuf + cf, e2 = cb + cub;
+                double su = 0.25 * (u1 + (uru + ur));
+                double sw = 0.25 * (w1 + w2);
+                double div = 0.25 * av * ((vk[L + RX] - vk[L - RX]) + (w1 - w2) + (u1 - (ulu + ul)));
+                st(a.Ay + zm1.off, colb,
+                   antidiff(div, av, cu, c, su, b1 - b2, b1 + b2, sw, e1 - e2, e1 + e2));
+#else
                 double su = 0.25 * (uu + uru + ur + ucc);
                 double sw = 0.25 * (wu + wuf + wf + wc);
                 double div = 0.25 * av * (vk[L + RX] - vk[L - RX] + wu - wuf - wf + wc + uu - ulu -
                                           ul + ucc);
-                double cu = cc[L + RX], cru = cc[L + 1 + RX], cr = cc[L + 1];
-                double cl = cc[L - 1], clu = cc[L - 1 + RX];
-                double cuf = cq[L + RX], cf = c1p, cb = c1m, cub = cm[L + RX];
                 st(a.Ay + zm1.off, colb,
                    antidiff(div, av, cu, c, su, cru + cr - cl - clu, cru + cr + cl + clu, sw,
                             cuf + cf - cb - cub, cuf + cf + cb + cub));
+#endif
             }
             {  // z-face (ww.c:248-255)
                 double av = wm1;
                 double ur = uc, urf = un, uf = uq[L - 1], ucc = uk[L - 1];
                 double vf = vq[L - RX], vuf = vn, vu = vc, vcc = vk[L - RX];
-                double su = 0.25 * (ur + urf + uf + ucc);
-                double sv = 0.25 * (vf + vuf + vu + vcc);
-                double div = 0.25 * av * (wn - wmm + uf - urf - ur + ucc + vf - vuf - vu + vcc);
                 double cf = c1p, crf = cq[L + 1], cr = cc[L + 1], cl = cc[L - 1];
                 double clf = cq[L - 1], cuf = cq[L + RX], cu = cc[L + RX], cd = cc[L - RX];
                 double cdf = cq[L - RX];
+#ifdef MPD_FAST_DIV
+                const double u1 = uf + ucc, u2 = urf + ur, v1 = vf + vcc, v2 = vuf + vu;
+                const double b1 = crf + cr, b2 = cl + clf, e1 = cuf + cu, e2 = cd + cdf;
+                double su = 0.25 * (u1 + u2);
+                double sv = 0.25 * (v1 + v2);
+                double div = 0.25 * av * ((wn - wmm) + (u1 - u2) + (v1 - v2));
+                st(a.Az + zm1.off, colb,
+                   antidiff(div, av, cf, c, su, b1 - b2, b1 + b2, sv, e1 - e2, e1 + e2));
+#else
+                double su = 0.25 * (ur + urf + uf + ucc);
+                double sv = 0.25 * (vf + vuf + vu + vcc);
+                double div = 0.25 * av * (wn - wmm + uf - urf - ur + ucc + vf - vuf - vu + vcc);
                 st(a.Az + zm1.off, colb,
                    antidiff(div, av, cf, c, su, crf + cr - cl - clf, crf + cr + cl + clf, sv,
                             cuf + cu - cd - cdf, cuf + cu + cd + cdf));
+#endif
             }
         }
         if (DONOR) {
@@ -310,7 +343,12 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
 {
     using G = Geo<TX, TY>;
     constexpr int RX = G::RX, R = G::R;
-    __shared__ double s_cn[2][R], s_cp[3][R], s_ax[2][R], s_ay[2][R], s_bu[2][R], s_bd[2][R];
+    __shared__ double s_cp[3][R], s_ax[2][R], s_ay[2][R], s_bu[2][R], s_bd[2][R];
+    // bounds inputs: SAVE publishes c^n (its bound codes name the element of either field
+    // that holds a bound); otherwise each cell publishes max(c^n, c_p) and min(c^n, c_p),
+    // whose 7-point max / min are the same bounds in 12 instead of 26 operations
+    __shared__ double s_cn[SAVE ? 2 : 1][SAVE ? R : 1], s_hi[SAVE ? 1 : 2][SAVE ? 1 : R],
+        s_lo[SAVE ? 1 : 2][SAVE ? 1 : R];
 
     const int t = G::role(item);
     int bx, by, bz;
@@ -392,14 +430,24 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
             }
         }
         const int k2 = (p + 1) % 2;  // slot of plane p - 1
-        s_cn[s2][L] = cnc;
+        if constexpr (SAVE) {
+            s_cn[s2][L] = cnc;
+        } else {
+            s_hi[s2][L] = mx2(cnc, cpc);
+            s_lo[s2][L] = mn2(cnc, cpc);
+        }
         s_cp[s3][L] = cpc;
         s_ax[s2][L] = axn;
         s_ay[s2][L] = ayn;
         s_bu[k2][L] = bum;
         s_bd[k2][L] = bdm;
         if (hl) {
-            s_cn[s2][HL] = hcnc;
+            if constexpr (SAVE) {
+                s_cn[s2][HL] = hcnc;
+            } else {
+                s_hi[s2][HL] = mx2(hcnc, hcpc);
+                s_lo[s2][HL] = mn2(hcnc, hcpc);
+            }
             s_cp[s3][HL] = hcpc;
             s_ax[s2][HL] = haxn;
             s_ay[s2][HL] = hayn;
@@ -414,30 +462,40 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
         auto ratios = [&](int M, double cn0, double cnlo, double cnhi, double c, double cplo,
                           double cphi, double axr, double ayr, double azlo, double azhi,
                           double axl, double ayl, double& bu, double& bd, int* code) {
-            double hi = cn0, lo = cn0;
-            const double* cnp = s_cn[s2];
-            const double nbn[7] = {cn0, cnp[M - 1], cnp[M + 1], cnp[M - RX], cnp[M + RX], cnlo,
-                                   cnhi};
             const double nbp[7] = {c, cpp[M - 1], cpp[M + 1], cpp[M - RX], cpp[M + RX], cplo,
                                    cphi};
-#pragma unroll
-            for (int m = 0; m < 7; m++) {
-                hi = mx2(hi, mx2(nbn[m], nbp[m]));
-                lo = mn2(lo, mn2(nbn[m], nbp[m]));
-            }
-            if (code) {
-                // which element holds each bound: element 2m + s is nbn[m] (s = 0) or
-                // nbp[m] (s = 1), and of the elements equal to the bound the last one
-                // counts (the convention of a running maximum that replaces on ties)
-                int hc = 0, lc = 0;
+            double hi, lo;
+            if constexpr (!SAVE) {
+                const double* hp = s_hi[s2];
+                const double* lp = s_lo[s2];
+                hi = mx2(mx2(mx2(mx2(cn0, c), hp[M - 1]), mx2(hp[M + 1], hp[M - RX])),
+                         mx2(mx2(hp[M + RX], mx2(cnlo, cplo)), mx2(cnhi, cphi)));
+                lo = mn2(mn2(mn2(mn2(cn0, c), lp[M - 1]), mn2(lp[M + 1], lp[M - RX])),
+                         mn2(mn2(lp[M + RX], mn2(cnlo, cplo)), mn2(cnhi, cphi)));
+            } else {
+                hi = lo = cn0;
+                const double* cnp = s_cn[s2];
+                const double nbn[7] = {cn0,         cnp[M - 1], cnp[M + 1], cnp[M - RX],
+                                       cnp[M + RX], cnlo,       cnhi};
 #pragma unroll
                 for (int m = 0; m < 7; m++) {
-                    hc = nbn[m] == hi ? 2 * m : hc;
-                    lc = nbn[m] == lo ? 2 * m : lc;
-                    hc = nbp[m] == hi ? 2 * m + 1 : hc;
-                    lc = nbp[m] == lo ? 2 * m + 1 : lc;
+                    hi = mx2(hi, mx2(nbn[m], nbp[m]));
+                    lo = mn2(lo, mn2(nbn[m], nbp[m]));
                 }
-                *code = hc | (lc << 8);
+                if (code) {
+                    // which element holds each bound: element 2m + s is nbn[m] (s = 0) or
+                    // nbp[m] (s = 1), and of the elements equal to the bound the last one
+                    // counts (the convention of a running maximum that replaces on ties)
+                    int hc = 0, lc = 0;
+#pragma unroll
+                    for (int m = 0; m < 7; m++) {
+                        hc = nbn[m] == hi ? 2 * m : hc;
+                        lc = nbn[m] == lo ? 2 * m : lc;
+                        hc = nbp[m] == hi ? 2 * m + 1 : hc;
+                        lc = nbp[m] == lo ? 2 * m + 1 : lc;
+                    }
+                    *code = hc | (lc << 8);
+                }
             }
             const double fxl = 0.5 * F2(nbp[1], c, axl), fxr = 0.5 * F2(c, nbp[2], axr);
             const double fyl = 0.5 * F2(nbp[3], c, ayl), fyr = 0.5 * F2(c, nbp[4], ayr);
