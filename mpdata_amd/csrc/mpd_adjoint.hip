@@ -444,8 +444,8 @@ k_adj_lim_t(LimArgs a)
     const bool comp = role >= 0, tcell = role == 0;
     const bool sec = G::second(t, sx, sy);
     const int L = oy * RX + ox, S = sy * RX + sx;
-    const unsigned colb = tile::column<TX, TY>(a.nx, a.ny, bx, by, ox, oy);
-    const unsigned scolb = tile::column<TX, TY>(a.nx, a.ny, bx, by, sx, sy);
+    unsigned colb = tile::column<TX, TY>(a.nx, a.ny, bx, by, ox, oy);
+    unsigned scolb = tile::column<TX, TY>(a.nx, a.ny, bx, by, sx, sy);
     const bool store = tcell && bx * TX + ox - 2 < a.nx && by * TY + oy - 2 < a.ny;
     const int z0 = bz * a.zc;
     const int z1 = min(z0 + a.zc, a.nz);
@@ -530,6 +530,8 @@ k_adj_lim_t(LimArgs a)
 
     for (int pl = z0 - 1; pl <= z1; pl++) {
         const int p = __builtin_amdgcn_readfirstlane(pl);
+        tile::keep32(colb);  // scalar-base addressing (see keep32)
+        tile::keep32(scolb);
         const int s3 = (p + 3) % 3, k3 = (p + 2) % 3;  // slots of planes p, p-1 (c1, A0, A1)
         const int s2 = (p + 2) % 2, k2 = (p + 1) % 2;  // slots of planes p, p-1 (stage 1)
         if (comp) {
@@ -873,8 +875,8 @@ __global__ void __launch_bounds__((tile::GeoB<TX, TY>::NTH)) k_adj_anti_t(AntiAr
     const bool comp = role >= 0, tcell = role == 0;
     const bool sec = G::second(t, sx, sy);
     const int L = oy * RX + ox, S = sy * RX + sx;
-    const unsigned colb = tile::column<TX, TY>(a.nx, a.ny, bx, by, ox, oy);
-    const unsigned scolb = tile::column<TX, TY>(a.nx, a.ny, bx, by, sx, sy);
+    unsigned colb = tile::column<TX, TY>(a.nx, a.ny, bx, by, ox, oy);
+    unsigned scolb = tile::column<TX, TY>(a.nx, a.ny, bx, by, sx, sy);
     const bool store = tcell && bx * TX + ox - 2 < a.nx && by * TY + oy - 2 < a.ny;
     const int z0 = bz * a.zc;
     const int z1 = min(z0 + a.zc, a.nz);
@@ -923,6 +925,8 @@ __global__ void __launch_bounds__((tile::GeoB<TX, TY>::NTH)) k_adj_anti_t(AntiAr
     double accm = 0.0, accc = 0.0, accn = 0.0, l1m = 0.0;
     for (int pl = z0 - 1; pl <= z1; pl++) {
         const int p = __builtin_amdgcn_readfirstlane(pl);
+        tile::keep32(colb);  // scalar-base addressing (see keep32)
+        tile::keep32(scolb);
         const int cm = (p + 2) % 3, cc = (p + 3) % 3, cq = (p + 4) % 3;  // c1 p-1, p, p+1
         const int up = (p + 2) % 2, uq = (p + 3) % 2;                     // U, V p, p+1
         const int wm = (p + 1) % 2, wc = (p + 2) % 2;                     // W p-1, p
