@@ -278,10 +278,12 @@ struct FrameF1Args {
     const double* q7;
 };
 
-// Column offsets of a cell (gx, gy) in each layout and whether the column exists there.
+// Byte offsets of a cell (gx, gy) within a plane of each layout (32-bit: the host keeps
+// every plane below 2^29 cells) and whether the column exists there.  Accesses are a
+// uniform plane base plus these offsets (the scalar-base addressing form).
 struct FrameCol {
     bool c, u, v, w;
-    size_t oc, ou, ov, ow;
+    unsigned oc, ou, ov, ow;
     __device__ FrameCol(const FrameF1Args& a, int gx, int gy)
     {
         const int nx = a.nx, ny = a.ny;
@@ -289,10 +291,17 @@ struct FrameCol {
         u = gx >= 0 && gx <= nx - 2 && gy >= 1 && gy <= ny - 2;
         v = gx >= 1 && gx <= nx - 2 && gy >= 0 && gy <= ny - 2;
         w = gx >= 1 && gx <= nx - 2 && gy >= 1 && gy <= ny - 2;
-        oc = c ? (size_t)gy * nx + gx : 0;
-        ou = u ? (size_t)(gy - 1) * (nx - 1) + gx : 0;
-        ov = v ? (size_t)gy * (nx - 2) + gx - 1 : 0;
-        ow = w ? (size_t)(gy - 1) * (nx - 2) + gx - 1 : 0;
+        oc = c ? (unsigned)(gy * nx + gx) * 8u : 0u;
+        ou = u ? (unsigned)((gy - 1) * (nx - 1) + gx) * 8u : 0u;
+        ov = v ? (unsigned)(gy * (nx - 2) + gx - 1) * 8u : 0u;
+        ow = w ? (unsigned)((gy - 1) * (nx - 2) + gx - 1) * 8u : 0u;
+    }
+    __device__ void keep()
+    {
+        tile::keep32(oc);
+        tile::keep32(ou);
+        tile::keep32(ov);
+        tile::keep32(ow);
     }
 };
 
@@ -314,28 +323,30 @@ __global__ void __launch_bounds__((tile::Geo<TX, TY>::NT)) k_frame_f1(FrameF1Arg
     const int rx = 2 + t % TX, ry = 2 + t / TX;
     const int L = ry * RX + rx;
     const int gx = bx * TX + rx - 2, gy = by * TY + ry - 2;
-    const FrameCol P(a, gx, gy);
+    FrameCol P(a, gx, gy);
     const bool hl = t < G::NH, h1 = t < G::H1;
     int hx = 0, hy = 0;
     if (hl) G::halo(t, hx, hy);
     const int HL = hy * RX + hx;
     const int hgx = bx * TX + hx - 2, hgy = by * TY + hy - 2;
-    const FrameCol H(a, hl ? hgx : -1, hl ? hgy : -1);
+    FrameCol H(a, hl ? hgx : -1, hl ? hgy : -1);
     const bool frame_xy = gx == 0 || gx == nx - 1 || gy == 0 || gy == ny - 1;
     const bool hframe_xy = hgx == 0 || hgx == nx - 1 || hgy == 0 || hgy == ny - 1;
 
     // plane loads: zero outside the grid or the layout
     auto ldc = [&](const FrameCol& q, int k) {
-        return (q.c && k >= 0 && k < ns) ? a.cin[(size_t)k * np + q.oc] : 0.0;
+        return (q.c && k >= 0 && k < ns) ? tile::ld(a.cin + (size_t)k * np, q.oc) : 0.0;
     };
     auto ldu = [&](const FrameCol& q, int k) {
-        return (q.u && k >= 1 && k <= ns - 2) ? a.U0[(size_t)(k - 1) * upl + q.ou] : 0.0;
+        return (q.u && k >= 1 && k <= ns - 2) ? tile::ld(a.U0 + (size_t)(k - 1) * upl, q.ou)
+                                              : 0.0;
     };
     auto ldv = [&](const FrameCol& q, int k) {
-        return (q.v && k >= 1 && k <= ns - 2) ? a.V0[(size_t)(k - 1) * vpl + q.ov] : 0.0;
+        return (q.v && k >= 1 && k <= ns - 2) ? tile::ld(a.V0 + (size_t)(k - 1) * vpl, q.ov)
+                                              : 0.0;
     };
     auto ldw = [&](const FrameCol& q, int k) {
-        return (q.w && k >= 0 && k <= ns - 2) ? a.W0[(size_t)k * wpl + q.ow] : 0.0;
+        return (q.w && k >= 0 && k <= ns - 2) ? tile::ld(a.W0 + (size_t)k * wpl, q.ow) : 0.0;
     };
 
     const int z0 = bz * a.zc, z1 = min(z0 + a.zc, ns);
@@ -373,6 +384,8 @@ __global__ void __launch_bounds__((tile::Geo<TX, TY>::NT)) k_frame_f1(FrameF1Arg
     }
     for (int pl = p0; pl <= p1; pl++) {
         const int p = __builtin_amdgcn_readfirstlane(pl);
+        P.keep();
+        H.keep();
         const int s3 = (p + 3) % 3;
         const double c0n = nc0, un = nu, vn = nv, wn = nw, fld = nf;
         const double hc0n = hnc0, hun = hnu, hvn = hnv, hwn = hnw, hfld = hnf;
@@ -410,7 +423,7 @@ __global__ void __launch_bounds__((tile::Geo<TX, TY>::NT)) k_frame_f1(FrameF1Arg
             if (P.c && !frame_xy && p != 0 && p != ns - 1)
                 c1p = donor(c0c, s_c0[L - 1], s_c0[L + 1], s_c0[L - RX], s_c0[L + RX], c0m,
                             c0n, s_u[s3][L - 1], un, s_v[s3][L - RX], vn, wm1, wn);
-            if (P.c && p >= z0 && p < z1) a.c1out[(size_t)p * np + P.oc] = c1p;
+            if (P.c && p >= z0 && p < z1) tile::st(a.c1out + (size_t)p * np, P.oc, c1p);
         } else {
             c1p = fld;
         }
@@ -457,14 +470,14 @@ __global__ void __launch_bounds__((tile::Geo<TX, TY>::NT)) k_frame_f1(FrameF1Arg
                 double cr = cc[L + 1], cru = cc[L + 1 + RX], cu = cc[L + RX];
                 double cd = cc[L - RX], crd = cc[L + 1 - RX];
                 double crf = cq[L + 1], cf = c1p, cb = c1m, crb = cm[L + 1];
-                a.U1[(size_t)(k - 1) * upl + P.ou] =
-                    antidiff(div, av, cr, c, sv, cru + cu - cd - crd, cru + cu + cd + crd, sw,
-                             crf + cf - cb - crb, crf + cf + cb + crb);
+                tile::st(a.U1 + (size_t)(k - 1) * upl, P.ou,
+                         antidiff(div, av, cr, c, sv, cru + cu - cd - crd, cru + cu + cd + crd,
+                                  sw, crf + cf - cb - crb, crf + cf + cb + crb));
             }
             const bool yedge = gy == 0 || gy == ny - 2;
             if (P.v && k >= 1 && k <= ns - 2 && !yedge && gx == 1) {
-                const size_t o = (size_t)(k - 1) * vpl + P.ov;
-                a.V1[o] = a.q7 ? a.q7[o] : 0.0;
+                const size_t o = (size_t)(k - 1) * vpl;
+                tile::st(a.V1 + o, P.ov, a.q7 ? tile::ld(a.q7 + o, P.ov) : 0.0);
             } else if (P.v && k >= 1 && k <= ns - 2) {  // y-face (vv.c:58-104)
                 const double av = vc;
                 double div, su, sw;
@@ -485,9 +498,9 @@ __global__ void __launch_bounds__((tile::Geo<TX, TY>::NT)) k_frame_f1(FrameF1Arg
                 double cu = cc[L + RX], cru = cc[L + 1 + RX], cr = cc[L + 1];
                 double cl = cc[L - 1], clu = cc[L - 1 + RX];
                 double cuf = cq[L + RX], cf = c1p, cb = c1m, cub = cm[L + RX];
-                a.V1[(size_t)(k - 1) * vpl + P.ov] =
-                    antidiff(div, av, cu, c, su, cru + cr - cl - clu, cru + cr + cl + clu, sw,
-                             cuf + cf - cb - cub, cuf + cf + cb + cub);
+                tile::st(a.V1 + (size_t)(k - 1) * vpl, P.ov,
+                         antidiff(div, av, cu, c, su, cru + cr - cl - clu, cru + cr + cl + clu,
+                                  sw, cuf + cf - cb - cub, cuf + cf + cb + cub));
             }
             if (P.w && k <= ns - 2) {  // z-face (ww.c:222-270)
                 const double av = wm1;
@@ -507,9 +520,9 @@ __global__ void __launch_bounds__((tile::Geo<TX, TY>::NT)) k_frame_f1(FrameF1Arg
                 double cf = c1p, crf = cq[L + 1], cr = cc[L + 1], cl = cc[L - 1];
                 double clf = cq[L - 1], cuf = cq[L + RX], cu = cc[L + RX], cd = cc[L - RX];
                 double cdf = cq[L - RX];
-                a.W1[(size_t)k * wpl + P.ow] =
-                    antidiff(div, av, cf, c, su, crf + cr - cl - clf, crf + cr + cl + clf, sv,
-                             cuf + cu - cd - cdf, cuf + cu + cd + cdf);
+                tile::st(a.W1 + (size_t)k * wpl, P.ow,
+                         antidiff(div, av, cf, c, su, crf + cr - cl - clf, crf + cr + cl + clf,
+                                  sv, cuf + cu - cd - cdf, cuf + cu + cd + cdf));
             }
         }
         if (DONOR) {
@@ -549,6 +562,8 @@ int frame_f1(const mpd_grid_s* g, bool donor1, const double* cin, double* const 
     a.zc = zc;
     const long long items = cols * ((a.ns + zc - 1) / zc);
     if (items > 0x7fffffff) return fail(MPD_ERR_UNSUPPORTED, "FRAME grid too large for one launch");
+    if ((long long)a.nx * a.ny > (1LL << 29))  // 32-bit byte offsets within a plane
+        return fail(MPD_ERR_UNSUPPORTED, "FRAME tiled path: planes of at most 2^29 cells");
     a.cin = cin;
     a.U0 = prev[0];
     a.V0 = prev[1];
@@ -584,7 +599,7 @@ int frame_step(mpd_grid_s* g)
     FrameIdx x{g->nx, g->ny, g->nz, (size_t)g->nx * g->ny};
     // the tiled kernel forms the donor field together with the first pass's velocities;
     // mpd_set_path(g, 1) keeps the per-face kernels
-    const bool tiled = g->path != 1 && g->iord > 1;
+    const bool tiled = g->path != 1 && g->iord > 1 && (long long)g->nx * g->ny <= (1LL << 29);
     double* set[2][3];
     for (int d = 0; d < 3; d++) set[0][d] = set[1][d] = g->face[d].p;
     if (g->iord > 1) {
