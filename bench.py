@@ -443,6 +443,11 @@ def adjoint_leg(M, n, nsteps=512, every=16, K=32, nrec=64):
     twin.close()
     g = fresh()
     c0 = g.field()
+    # the gradient run's steps are exact-arithmetic fused steps: their launch shapes are
+    # tuned in that arithmetic (the plain forward it is compared with uses the same)
+    g.set_arith(M.ARITH_EXACT)
+    g.autotune(2)
+    tuned = tuned_config(g)
     # records of as many final segments as the device memory holds are kept from the
     # forward sweep (the rest are recomputed from their checkpoints)
     g.set_adjoint_tape(-1)
@@ -507,7 +512,7 @@ def adjoint_leg(M, n, nsteps=512, every=16, K=32, nrec=64):
     return {"workload": f"periodic {n}^3 deformational wind (seed 5), 8-source emission, "
                         f"{nsteps} steps, {nrec} receptors every {every} steps, checkpoints "
                         f"every {K}, iord 2 nonoscillatory",
-            "J": J, "forward_ms_per_step": tp / nsteps,
+            "autotuned": tuned, "J": J, "forward_ms_per_step": tp / nsteps,
             "forward_sweep_ms_per_step": tf / nsteps, "backward_ms_per_step": tb / nsteps,
             "adjoint_over_forward": (tf + tb - tp) / tp,
             "kept_segments": kept, "segments": -(-nsteps // K),

@@ -796,7 +796,10 @@ int MPD_CAT(fused_store_step, MPD_ARITH)(mpd_grid_s* g, const double* cin, doubl
         a.bu = bu;
         a.bd = bd;
         a.code = code;
-        return launch(a, tx, ty, false, false, s, nullptr);
+        // the limiter kernel in its own launch shape, as in the plain step
+        const int tx2 = g->tile2_x ? g->tile2_x : tx, ty2 = g->tile2_x ? g->tile2_y : ty;
+        a.zc = g->tile2_x ? (g->zc2 ? g->zc2 : choose_zc(g, tx2, ty2)) : a.zc;
+        return launch(a, tx2, ty2, false, false, s, nullptr);
     }
     return donor_faces(g, A[0], A[1], A[2], c1, q, cout, s, nullptr);
 }
