@@ -370,6 +370,7 @@ def ensemble_leg(M, n):
     cells = b.cells()
     setup = time.perf_counter() - t0
     tx, ty, zc, pers = b.autotune(2)
+    lim = b.tiles()[1]
     b.step(2, M.BATCH_GRAPH)
     ms_graph = b.step_timed(100, M.BATCH_GRAPH) / 100
     ms_fused = b.step_timed(10, M.BATCH_FUSED) / 10
@@ -380,7 +381,9 @@ def ensemble_leg(M, n):
     return {"workload": f"{n} periodic instances from {{64^3, 96^3, 128^3, 160x160x96, "
                         f"192x192x64}} (seed 4), rotation + 10% seeded modes, iord 2 "
                         "nonoscillatory", "cells": cells, "setup_s": setup,
-            "autotuned": {"tile": [tx, ty], "zc": zc, "persistent": pers},
+            "autotuned": dict({"tile": [tx, ty], "zc": zc, "persistent": pers},
+                              **({"limiter_tile": list(lim[:2]), "limiter_zc": lim[2]}
+                                 if lim[0] else {})),
             "graph_ms_per_step": ms_graph, "fused_launch_ms_per_step": ms_fused,
             "per_instance_ms_per_step": ms_per,
             "graph_cell_updates_per_s": cells / (ms_graph * 1e-3),

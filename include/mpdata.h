@@ -212,8 +212,15 @@ int mpd_batch_instance(mpd_batch b, int i, mpd_grid* g);
 int mpd_batch_cells(mpd_batch b, long long* cells);
 int mpd_batch_set_arith(mpd_batch b, int arith);
 /* Tile shape, z-chunk and grid kind (persistent = 1: one resident wave of workgroups
- * striding over the work items; 0: one workgroup per item). */
+ * striding over the work items; 0: one workgroup per item), for both kernels of a pass
+ * (clears the limiter kernel's own shape). */
 int mpd_batch_set_tile(mpd_batch b, int tile_x, int tile_y, int zc, int persistent);
+/* Tile shape and z-chunk of the limiter kernel alone (0, 0, 0: those of the first
+ * kernel); the results do not depend on it. */
+int mpd_batch_set_limiter_tile(mpd_batch b, int tile_x, int tile_y, int zc);
+/* v[0..3]: tile_x, tile_y, zc, persistent of the first kernel; v[4..6]: the limiter
+ * kernel's tile_x, tile_y, zc (0 = the first kernel's). */
+int mpd_batch_get_tiles(mpd_batch b, int* v);
 /* Times every tile shape x z-chunk x grid kind over nsteps steps on the batch itself
  * (fields restored afterwards), applies and returns the fastest (any output may be
  * NULL).  Cached per process; also in the file named by MPD_TUNE_CACHE when set. */

@@ -104,6 +104,8 @@ SIGNATURES = {
     "mpd_batch_cells": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong)]),
     "mpd_batch_set_arith": (C.c_int, [C.c_void_p, C.c_int]),
     "mpd_batch_set_tile": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "mpd_batch_set_limiter_tile": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "mpd_batch_get_tiles": (C.c_int, [C.c_void_p, PI]),
     "mpd_batch_autotune": (C.c_int, [C.c_void_p, C.c_int, PI, PI, PI, PI]),
     "mpd_batch_step": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "mpd_batch_step_timed": (C.c_int, [C.c_void_p, C.c_int, C.c_int, PD]),
@@ -418,6 +420,17 @@ class Batch:
 
     def set_tile(self, tx, ty, zc, persistent=True):
         _check(lib().mpd_batch_set_tile(self.h, tx, ty, zc, int(persistent)))
+
+    def set_limiter_tile(self, tx, ty, zc=0):
+        """Launch shape of the limiter kernel alone (0, 0: that of the first kernel)."""
+        _check(lib().mpd_batch_set_limiter_tile(self.h, tx, ty, zc))
+
+    def tiles(self):
+        """((tile_x, tile_y, zc, persistent) of the first kernel, (tile_x, tile_y, zc) of
+        the limiter kernel; zeros: the first kernel's)."""
+        v = (C.c_int * 7)()
+        _check(lib().mpd_batch_get_tiles(self.h, v))
+        return tuple(v[0:4]), tuple(v[4:7])
 
     def autotune(self, nsteps=2):
         v = [C.c_int() for _ in range(4)]

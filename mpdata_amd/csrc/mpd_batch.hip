@@ -34,18 +34,22 @@ using namespace mpd;
 struct mpd_batch_s {
     int ninst = 0, iord = 2, nonosc = 1, arith = MPD_ARITH_FAST;
     int tx = 32, ty = 8, zc = 16, persistent = 1;
+    int tx2 = 0, ty2 = 0, zc2 = 0;  // the limiter kernel's launch shape (0: F1's)
     int cur = 0;
     long long cells = 0;
     std::vector<mpd_grid_s*> g;
-    // device tables: [parity][pass-1][0 = F1, 1 = F2][instance]
+    // device tables: [parity][pass-1][0 = F1, 1 = F2][instance]; first work item of
+    // each instance for F1 (ditem0[0 .. ninst)) and for F2 (ditem0[ninst .. 2 ninst)),
+    // whose tilings may differ
     FusedArgs* dtab = nullptr;
     int* ditem0 = nullptr;
-    int nitems = 0;
+    int nitems = 0, nitems2 = 0;
     std::vector<FusedArgs> htab;
     std::vector<int> hitem0;
     // cached graphs of the time loop, one per starting field parity
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
-    int g_steps[2] = {-1, -1}, g_key[2] = {-1, -1};
+    int g_steps[2] = {-1, -1};
+    long long g_key[2] = {-1, -1};
     bool tuned = false;
 
     ~mpd_batch_s()
@@ -66,9 +70,19 @@ int table_index(const mpd_batch_s* b, int parity, int pass, int f2, int inst)
 }
 
 // the key of the launch configuration a cached graph was captured with
-int config_key(const mpd_batch_s* b)
+long long config_key(const mpd_batch_s* b)
 {
-    return ((b->tx * 131 + b->ty) * 131 + b->zc) * 4 + b->persistent * 2 + b->arith;
+    long long k = ((b->tx * 131LL + b->ty) * 131 + b->zc) * 4 + b->persistent * 2 + b->arith;
+    return ((k * 131 + b->tx2) * 131 + b->ty2) * 131 + b->zc2;
+}
+
+// Launch shape of the first kernel (f2 = false) or of the limiter kernel (f2 = true).
+void shape_of(const mpd_batch_s* b, bool f2, int& tx, int& ty, int& zc)
+{
+    const bool own = f2 && b->tx2;
+    tx = own ? b->tx2 : b->tx;
+    ty = own ? b->ty2 : b->ty;
+    zc = own ? b->zc2 : b->zc;
 }
 
 // Fills the per-instance descriptors of every pass for both field parities (the buffer
@@ -78,8 +92,10 @@ int build_tables(mpd_batch_s* b)
     int rc;
     const int npass = b->iord - 1;
     b->htab.assign((size_t)2 * npass * 2 * b->ninst, FusedArgs{});
-    b->hitem0.assign(b->ninst, 0);
-    int items = 0;
+    b->hitem0.assign(2 * (size_t)b->ninst, 0);
+    int items = 0, items2 = 0;
+    int tx2, ty2, zc2;
+    shape_of(b, true, tx2, ty2, zc2);
     for (int i = 0; i < b->ninst; i++) {
         mpd_grid_s* g = b->g[i];
         if ((rc = ensure_fused_scratch(g))) return rc;
@@ -88,10 +104,15 @@ int build_tables(mpd_batch_s* b)
         geo.ny = g->ny;
         geo.nz = g->nz;
         geo.np = (long long)g->nx * g->ny;
+        FusedArgs geo2 = geo;
         geo.zc = std::min(b->zc, g->nz);
         fused_geometry(geo, b->tx, b->ty);
+        geo2.zc = std::min(zc2, g->nz);
+        fused_geometry(geo2, tx2, ty2);
         b->hitem0[i] = items;
+        b->hitem0[b->ninst + i] = items2;
         items += fused_items(geo);
+        items2 += fused_items(geo2);
         for (int par = 0; par < 2; par++) {
             const double* cin = g->c[par].p;
             double* out = g->c[1 - par].p;
@@ -110,6 +131,10 @@ int build_tables(mpd_batch_s* b)
                 f1.Ay = g->fa[1].p;
                 f1.Az = g->fa[2].p;
                 FusedArgs f2 = f1;
+                f2.tiles_x = geo2.tiles_x;
+                f2.tiles_y = geo2.tiles_y;
+                f2.zc = geo2.zc;
+                f2.nchunks = geo2.nchunks;
                 f2.cn = cin;
                 f2.cp = (p == 1) ? g->cp[0].p : field;
                 f2.q = (last && g->has_q) ? g->q.p : nullptr;
@@ -125,14 +150,15 @@ int build_tables(mpd_batch_s* b)
         }
     }
     b->nitems = items;
+    b->nitems2 = items2;
     if (!b->dtab) {
         MPD_HIP(hipMalloc((void**)&b->dtab, b->htab.size() * sizeof(FusedArgs)));
-        MPD_HIP(hipMalloc((void**)&b->ditem0, b->ninst * sizeof(int)));
+        MPD_HIP(hipMalloc((void**)&b->ditem0, 2 * (size_t)b->ninst * sizeof(int)));
     }
     hipStream_t s = stream();
     MPD_HIP(hipMemcpyAsync(b->dtab, b->htab.data(), b->htab.size() * sizeof(FusedArgs),
                            hipMemcpyHostToDevice, s));
-    MPD_HIP(hipMemcpyAsync(b->ditem0, b->hitem0.data(), b->ninst * sizeof(int),
+    MPD_HIP(hipMemcpyAsync(b->ditem0, b->hitem0.data(), b->hitem0.size() * sizeof(int),
                            hipMemcpyHostToDevice, s));
     MPD_HIP(hipStreamSynchronize(s));  // the host tables are reused on the next call
     return MPD_OK;
@@ -149,11 +175,14 @@ int cu_count()
 
 int grid_for(const mpd_batch_s* b, bool f1)
 {
-    if (!b->persistent) return b->nitems;
-    int per = b->arith == MPD_ARITH_FAST ? fused_resident_blocks_fast(b->tx, b->ty, f1)
-                                         : fused_resident_blocks_exact(b->tx, b->ty, f1);
+    const int items = f1 ? b->nitems : b->nitems2;
+    if (!b->persistent) return items;
+    int tx, ty, zc;
+    shape_of(b, !f1, tx, ty, zc);
+    int per = b->arith == MPD_ARITH_FAST ? fused_resident_blocks_fast(tx, ty, f1)
+                                         : fused_resident_blocks_exact(tx, ty, f1);
     long long grid = (long long)std::max(per, 1) * cu_count();
-    if (grid >= b->nitems) return b->nitems;
+    if (grid >= items) return items;
     grid -= grid % 8;
     return (int)std::max<long long>(grid, 1);
 }
@@ -167,11 +196,13 @@ int launch_step(mpd_batch_s* b, int par)
     for (int p = 1; p < b->iord; p++) {
         const FusedArgs* t1 = b->dtab + table_index(b, par, p, 0, 0);
         const FusedArgs* t2 = b->dtab + table_index(b, par, p, 1, 0);
+        int tx2, ty2, zc2;
+        shape_of(b, true, tx2, ty2, zc2);
         if ((rc = launch(b->tx, b->ty, true, p == 1, t1, b->ditem0, b->ninst, b->nitems,
                          grid_for(b, true), s)))
             return rc;
-        if ((rc = launch(b->tx, b->ty, false, false, t2, b->ditem0, b->ninst, b->nitems,
-                         grid_for(b, false), s)))
+        if ((rc = launch(tx2, ty2, false, false, t2, b->ditem0 + b->ninst, b->ninst,
+                         b->nitems2, grid_for(b, false), s)))
             return rc;
     }
     return MPD_OK;
@@ -188,7 +219,8 @@ void set_parity(mpd_batch_s* b, int par)
 int ensure_graph(mpd_batch_s* b, int nsteps)
 {
     hipStream_t s = stream();
-    const int key = config_key(b), par = b->cur;
+    const long long key = config_key(b);
+    const int par = b->cur;
     if (b->gexec[par] && b->g_steps[par] == nsteps && b->g_key[par] == key) return MPD_OK;
     if (b->gexec[par]) {
         (void)hipGraphExecDestroy(b->gexec[par]);
@@ -332,10 +364,40 @@ extern "C" int mpd_batch_set_tile(mpd_batch b, int tx, int ty, int zc, int persi
     b->ty = ty;
     b->zc = zc;
     b->persistent = persistent ? 1 : 0;
+    b->tx2 = b->ty2 = b->zc2 = 0;  // the limiter kernel follows
     for (auto* g : b->g) {
         g->tile_x = tx;
         g->tile_y = ty;
+        g->tile2_x = g->tile2_y = g->zc2 = 0;
     }
+    return MPD_OK;
+}
+
+extern "C" int mpd_batch_set_limiter_tile(mpd_batch b, int tx, int ty, int zc)
+{
+    BATCH_OR_FAIL(b);
+    bool ok = tx == 0 && ty == 0;
+#define MPD_TILE(X, Y) ok = ok || (tx == X && ty == Y && zc >= 1);
+    MPD_FUSED_TILES(MPD_TILE)
+#undef MPD_TILE
+    if (!ok) return fail(MPD_ERR_ARG, "mpd_batch_set_limiter_tile: %dx%d zc %d", tx, ty, zc);
+    b->tx2 = tx;
+    b->ty2 = ty;
+    b->zc2 = tx ? zc : 0;
+    for (auto* g : b->g) {  // per-instance launches use the same tile shapes
+        g->tile2_x = tx;
+        g->tile2_y = ty;
+        g->zc2 = 0;
+    }
+    return MPD_OK;
+}
+
+extern "C" int mpd_batch_get_tiles(mpd_batch b, int* v)
+{
+    BATCH_OR_FAIL(b);
+    if (!v) return fail(MPD_ERR_ARG, "mpd_batch_get_tiles: NULL array");
+    const int t[7] = {b->tx, b->ty, b->zc, b->persistent, b->tx2, b->ty2, b->zc2};
+    for (int i = 0; i < 7; i++) v[i] = t[i];
     return MPD_OK;
 }
 
@@ -391,63 +453,95 @@ extern "C" int mpd_batch_step_timed(mpd_batch b, int nsteps, int mode, double* m
 }
 
 // Autotuner: times every compiled tile shape x z-chunk x grid kind on the batch itself
-// (MPD_BATCH_FUSED, nsteps steps each after one warm-up step) and keeps the fastest.  The
-// fields are saved before and restored after, so tuning leaves the ensemble unchanged.
-// Results are cached per process by the ensemble's shape signature; with MPD_TUNE_CACHE
-// set they are also appended to / read from that file.
+// (MPD_BATCH_FUSED, nsteps steps each after one warm-up step) and keeps the fastest; then,
+// with that kept for the first kernel, every shape x z-chunk for the limiter kernel
+// alone, and with the best of those kept, every shape x z-chunk for the first kernel
+// again.  The fields are saved before and restored after, so tuning leaves the ensemble
+// unchanged.  Results are cached per process by the ensemble's shape signature; with
+// MPD_TUNE_CACHE set they are also appended to / read from that file ("batch2" entries
+// hold both kernels' shapes; a "batch" entry serves both kernels).
 extern "C" int mpd_batch_autotune(mpd_batch b, int nsteps, int* tx, int* ty, int* zc,
                                   int* persistent)
 {
     BATCH_OR_FAIL(b);
     if (nsteps < 1) return fail(MPD_ERR_ARG, "mpd_batch_autotune: nsteps < 1");
-    std::string key = "batch:i" + std::to_string(b->iord) + "n" + std::to_string(b->nonosc) +
-                      "a" + std::to_string(b->arith);
+    std::string opts = "i" + std::to_string(b->iord) + "n" + std::to_string(b->nonosc) + "a" +
+                       std::to_string(b->arith);
     for (auto* g : b->g)
-        key += ":" + std::to_string(g->nx) + "x" + std::to_string(g->ny) + "x" +
-               std::to_string(g->nz);
+        opts += ":" + std::to_string(g->nx) + "x" + std::to_string(g->ny) + "x" +
+                std::to_string(g->nz);
+    const std::string key = "batch2:" + opts;
     int rc;
-    int c[4];
-    if (!tune_lookup(key, 4, c)) {
-        std::vector<DevBuf> save(b->ninst);
-        hipStream_t s = stream();
-        for (int i = 0; i < b->ninst; i++) {
-            mpd_grid_s* g = b->g[i];
-            if ((rc = save[i].alloc(g->ncell))) return rc;
-            MPD_HIP(hipMemcpyAsync(save[i].p, g->c[g->cur].p, g->ncell * 8, hipMemcpyDeviceToDevice, s));
-        }
-        const int start = b->cur;
-        double best = 1e300;
-        int bestc[4] = {b->tx, b->ty, b->zc, b->persistent};
-        const int shapes[][2] = {
+    int c[7];
+    if (!tune_lookup(key, 7, c)) {
+        if (tune_lookup("batch:" + opts, 4, c)) {
+            c[4] = c[5] = c[6] = 0;
+        } else {
+            std::vector<DevBuf> save(b->ninst);
+            hipStream_t s = stream();
+            for (int i = 0; i < b->ninst; i++) {
+                mpd_grid_s* g = b->g[i];
+                if ((rc = save[i].alloc(g->ncell))) return rc;
+                MPD_HIP(hipMemcpyAsync(save[i].p, g->c[g->cur].p, g->ncell * 8, hipMemcpyDeviceToDevice, s));
+            }
+            const int start = b->cur;
+            const int shapes[][2] = {
 #define MPD_TILE(X, Y) {X, Y},
-            MPD_FUSED_TILES(MPD_TILE)
+                MPD_FUSED_TILES(MPD_TILE)
 #undef MPD_TILE
-        };
-        for (auto& sh : shapes)
-            for (int z : {8, 16, 32, 64})
-                for (int pers : {0, 1}) {
-                    if ((rc = mpd_batch_set_tile(b, sh[0], sh[1], z, pers))) return rc;
-                    double ms = 0;
-                    if ((rc = mpd_batch_step(b, 1, MPD_BATCH_FUSED))) return rc;
-                    if ((rc = mpd_batch_step_timed(b, nsteps, MPD_BATCH_FUSED, &ms))) return rc;
+            };
+            // one timed trial of the configuration set on the batch
+            auto trial = [&](double& ms) {
+                int r = mpd_batch_step(b, 1, MPD_BATCH_FUSED);
+                return r ? r : mpd_batch_step_timed(b, nsteps, MPD_BATCH_FUSED, &ms);
+            };
+            double best = 1e300, ms = 0;
+            int f1[4] = {b->tx, b->ty, b->zc, b->persistent}, f2[3] = {0, 0, 0};
+            for (auto& sh : shapes)  // both kernels in one shape
+                for (int z : {8, 16, 32, 64})
+                    for (int pers : {0, 1}) {
+                        if ((rc = mpd_batch_set_tile(b, sh[0], sh[1], z, pers)) || (rc = trial(ms)))
+                            return rc;
+                        if (ms < best) {
+                            best = ms;
+                            f1[0] = sh[0], f1[1] = sh[1], f1[2] = z, f1[3] = pers;
+                        }
+                    }
+            for (auto& sh : shapes)  // the limiter kernel in a shape of its own
+                for (int z : {8, 16, 32, 64}) {
+                    if ((rc = mpd_batch_set_tile(b, f1[0], f1[1], f1[2], f1[3])) ||
+                        (rc = mpd_batch_set_limiter_tile(b, sh[0], sh[1], z)) || (rc = trial(ms)))
+                        return rc;
                     if (ms < best) {
                         best = ms;
-                        bestc[0] = sh[0];
-                        bestc[1] = sh[1];
-                        bestc[2] = z;
-                        bestc[3] = pers;
+                        f2[0] = sh[0], f2[1] = sh[1], f2[2] = z;
                     }
                 }
-        set_parity(b, start);
-        for (int i = 0; i < b->ninst; i++) {
-            mpd_grid_s* g = b->g[i];
-            MPD_HIP(hipMemcpyAsync(g->c[g->cur].p, save[i].p, g->ncell * 8, hipMemcpyDeviceToDevice, s));
+            if (f2[0])  // and the first kernel again, with the limiter kernel's shape kept
+                for (auto& sh : shapes)
+                    for (int z : {8, 16, 32, 64}) {
+                        if ((rc = mpd_batch_set_tile(b, sh[0], sh[1], z, f1[3])) ||
+                            (rc = mpd_batch_set_limiter_tile(b, f2[0], f2[1], f2[2])) ||
+                            (rc = trial(ms)))
+                            return rc;
+                        if (ms < best) {
+                            best = ms;
+                            f1[0] = sh[0], f1[1] = sh[1], f1[2] = z;
+                        }
+                    }
+            set_parity(b, start);
+            for (int i = 0; i < b->ninst; i++) {
+                mpd_grid_s* g = b->g[i];
+                MPD_HIP(hipMemcpyAsync(g->c[g->cur].p, save[i].p, g->ncell * 8, hipMemcpyDeviceToDevice, s));
+            }
+            MPD_HIP(hipStreamSynchronize(s));
+            for (int i = 0; i < 4; i++) c[i] = f1[i];
+            for (int i = 0; i < 3; i++) c[4 + i] = f2[i];
         }
-        MPD_HIP(hipStreamSynchronize(s));
-        tune_store(key, 4, bestc);
-        for (int i = 0; i < 4; i++) c[i] = bestc[i];
+        tune_store(key, 7, c);
     }
     if ((rc = mpd_batch_set_tile(b, c[0], c[1], c[2], c[3]))) return rc;
+    if (c[4] && (rc = mpd_batch_set_limiter_tile(b, c[4], c[5], c[6]))) return rc;
     if (tx) *tx = c[0];
     if (ty) *ty = c[1];
     if (zc) *zc = c[2];

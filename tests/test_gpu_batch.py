@@ -81,20 +81,28 @@ def make_batch(iord, arith, seed=7):
     return b, init
 
 
-MODES = [(M.BATCH_PER_INSTANCE, None), (M.BATCH_FUSED, (32, 8, 16, 1)),
-         (M.BATCH_FUSED, (16, 16, 8, 0)), (M.BATCH_GRAPH, (32, 16, 8, 1)),
-         (M.BATCH_GRAPH, (16, 8, 32, 0))]
+# (mode, tile shape / z-chunk / persistent of both kernels, limiter kernel's own shape)
+MODES = [(M.BATCH_PER_INSTANCE, None, None), (M.BATCH_FUSED, (32, 8, 16, 1), None),
+         (M.BATCH_FUSED, (16, 16, 8, 0), None), (M.BATCH_GRAPH, (32, 16, 8, 1), None),
+         (M.BATCH_GRAPH, (16, 8, 32, 0), None),
+         (M.BATCH_FUSED, (16, 16, 8, 0), (32, 8, 4)), (M.BATCH_FUSED, (64, 8, 8, 1), (16, 8, 3)),
+         (M.BATCH_GRAPH, (32, 8, 16, 1), (64, 8, 16)),
+         (M.BATCH_PER_INSTANCE, (32, 8, 16, 1), (16, 16, 8))]
 
 
 @pytest.mark.parametrize("iord", [2, 3])
 @pytest.mark.parametrize("arith", [M.ARITH_EXACT, M.ARITH_FAST])
 def test_batch_modes_agree_and_match_oracle(iord, arith):
     results = []
-    for mode, tile in MODES:
-        print(f"case iord={iord} arith={arith} mode={mode} tile={tile}", flush=True)
+    for mode, tile, lim in MODES:
+        print(f"case iord={iord} arith={arith} mode={mode} tile={tile} limiter={lim}",
+              flush=True)
         b, init = make_batch(iord, arith)
         if tile:
             b.set_tile(*tile)
+        if lim:
+            b.set_limiter_tile(*lim)
+            assert b.tiles() == (tile, lim)
         b.step(3, mode)
         results.append([g.field() for g in b.instances])
         b.close()
@@ -131,8 +139,12 @@ def test_autotune_restores_fields():
     b, _ = make_batch(2, M.ARITH_FAST)
     before = [g.field() for g in b.instances]
     tx, ty, zc, pers = b.autotune(1)
-    assert (tx, ty) in [(32, 8), (16, 16), (64, 8), (16, 8), (32, 16)]
-    assert zc in (8, 16, 32) and pers in (0, 1)
+    shapes = [(32, 8), (16, 16), (64, 8), (16, 8), (32, 16)]
+    assert (tx, ty) in shapes
+    assert zc in (8, 16, 32, 64) and pers in (0, 1)
+    first, lim = b.tiles()
+    assert first == (tx, ty, zc, pers)
+    assert lim == (0, 0, 0) or (lim[:2] in shapes and lim[2] in (8, 16, 32, 64))
     for g, f in zip(b.instances, before):
         np.testing.assert_array_equal(g.field(), f)
     b.step(2, M.BATCH_GRAPH)
