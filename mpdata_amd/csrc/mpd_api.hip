@@ -1,5 +1,6 @@
 // C ABI of the library: the drop-in smolar()/msmolar()/msmolar_gather() of the
 // reference (smolar.h:45-48, 95-99, 132-136) and the device-resident mpd_* interface.
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <string>
@@ -716,10 +717,13 @@ extern "C" int mpd_autotune(mpd_grid g, int nsteps, int* tile_x, int* tile_y, in
                     g->tile_y = sh[1];
                     g->zc = z;
                     g->tile2_x = g->tile2_y = g->zc2 = 0;
-                    double ms[16];
+                    // two kernels per corrective pass
+                    std::vector<double> ms(2 * (size_t)g->iord, 0.0);
                     int nk = 0;
                     if ((rc = mpd_step(g, 1))) return rc;
-                    if ((rc = mpd_step_kernel_times(g, nsteps, ms, 16, &nk))) return rc;
+                    if ((rc = mpd_step_kernel_times(g, nsteps, ms.data(), (int)ms.size(), &nk)))
+                        return rc;
+                    nk = std::min(nk, (int)ms.size());
                     double t[2] = {0.0, 0.0};  // kernels alternate F1, F2 (or donor) per pass
                     for (int k = 0; k < nk; k++) t[k & 1] += ms[k];
                     for (int f = 0; f < 2; f++)

@@ -397,3 +397,19 @@ def test_autotune_reports_both_kernels_shapes():
     g.step(3)
     assert relerr(g.field(), O.per_run(nx, ny, nz, U, V, W, 2, True, c, 3)) <= TOL
     g.close()
+
+
+def test_autotune_with_many_passes():
+    """iord 11: twenty kernels per step, more than the autotuner's first guess of a
+    per-step kernel count; it times them all and the tuned grid matches the oracle."""
+    nx, ny, nz = 20, 12, 8
+    U, V, W, c = per_case(nx, ny, nz, 41)
+    g = M.Grid(nx, ny, nz, M.PERIODIC, iord=11, nonosc=True)
+    g.set_courant(U, V, W)
+    g.set_field(c)
+    g.autotune(1)
+    t1, t2 = g.tiles()
+    assert t1[0] > 0 and t1[2] >= 1
+    g.step(2)
+    assert relerr(g.field(), O.per_run(nx, ny, nz, U, V, W, 11, True, c, 2)) <= TOL
+    g.close()
