@@ -161,3 +161,55 @@ def test_ring2d_128_1024_matches_reference_run(tmp_path):
             h.update(chunk)
     assert hashlib.sha256(r.stdout.encode()).hexdigest() == str(d["stdout_sha256"])
     assert h.hexdigest() == str(d["bin_sha256"])
+
+
+def _ring_state(prog, n, tmp_path):
+    """The drivers' initial configuration (written with zero steps): winds and field on
+    the n x n x m grid (m = 3 for ring2d, whose layers are replicas, n for ring3d)."""
+    r = run(prog, n, 0, tmp_path)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(tmp_path / f"{prog}.bin")
+    if prog == "ring2d":
+        m = 3
+        u1, v1, c1 = raw.reshape(3, n * n)
+        u, v, c = np.tile(u1, m), np.tile(v1, m), np.tile(c1, m)
+        w = np.zeros(n * n * m)
+    else:
+        m = n
+        u, v, w, c = (raw[i * n ** 3:(i + 1) * n ** 3].copy() for i in range(4))
+    return m, u, v, w, c
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prog,n,nt,decomp", [("ring2d", 64, 16, (2, 2, 1)),
+                                              ("ring3d", 32, 8, (2, 2, 2))])
+def test_msmolar_equals_smolar_as_in_mring(prog, n, nt, decomp, tmp_path):
+    """The calling pattern of the reference's equivalence tests, tests/mring2d/test.c:85-155
+    and tests/mring3d/test.c:76-130: smolar() on one configuration and msmolar() with a
+    decomposition on a clone, both callers swapping their pointers after every step, then
+    msmolar_gather().  msmolar_gather writes the newest field to *c1 and the last step's
+    previous-pass field to *c0 (msmolar_gather.c:19-27); after the callers' swaps the
+    serial side's newest field is its *c0 and its *c1 is smolar's previous-pass field
+    (smolar.c:44-58), so those are the pairs compared -- bit for bit, the device-resident
+    context being exact.  (The reference compares the two *c1, |dc| <= 1e-6, and its own
+    1-cell-overlap decomposition is approximate.)"""
+    if M.device_count() < 1:
+        pytest.skip("no GPU")
+    m, u, v, w, c = _ring_state(prog, n, tmp_path)
+    one_n, one_m = np.ones(n), np.ones(m)
+    a0, a1 = c.copy(), np.zeros_like(c)
+    b0, b1 = c.copy(), np.zeros_like(c)
+    ms = M.MSmolar()
+    for _ in range(nt):
+        a0, a1 = M.smolar(n, n, m, one_n, one_n, one_m, 0.0125, u, u, v, v, w, w, a0, a1,
+                          order=3)
+        b0, b1 = ms.step(n, n, m, one_n, one_n, one_m, 0.0125, u, u, v, v, w, w, b0, b1,
+                         order=3, sx=decomp[0], sy=decomp[1], ss=decomp[2],
+                         mode=M.MSMOLAR_OPENMP)
+        a0, a1 = a1, a0  # mring2d/test.c:133-139
+        b0, b1 = b1, b0
+    ms.gather(b0, b1)
+    ms.close()
+    np.testing.assert_array_equal(b1, a0)  # the newest field
+    np.testing.assert_array_equal(b0, a1)  # the last step's previous-pass field
+    assert not np.array_equal(a0, c) and not np.array_equal(a0, a1)
