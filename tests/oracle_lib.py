@@ -1,5 +1,6 @@
 """ctypes bindings to the CPU oracle (oracle/liboracle.so) and, where it was built, the
-reference's own core compiled from its sources (oracle/_ref/libsmolar_ref.so).
+reference's own core compiled from its sources (oracle/_ref/libsmolar_ref.so, and
+oracle/_ref/libmsmolar_ref.so with its decomposed step).
 
 Test infrastructure only: used by tests/, __graft_entry__.smoke() and bench.py's
 CPU-baseline leg as the checker, never by the product.
@@ -12,6 +13,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libsmolar_ref.so")
+MS_REF_SO = os.path.join(ROOT, "oracle", "_ref", "libmsmolar_ref.so")
 
 PD = C.POINTER(C.c_double)
 PPD = C.POINTER(PD)
@@ -56,11 +58,16 @@ def _bind(lib):
     if hasattr(lib, "smolar"):
         lib.smolar.argtypes = _SMOLAR_ARGS
         lib.smolar.restype = C.c_int
+    if hasattr(lib, "msmolar"):
+        for name in ("msmolar", "msmolar_gather"):
+            getattr(lib, name).argtypes = [C.POINTER(C.c_void_p)] + _SMOLAR_ARGS + [C.c_int] * 4
+            getattr(lib, name).restype = C.c_int
     return lib
 
 
 _oracle = None
 _ref = None
+_ms_ref = None
 
 
 def oracle():
@@ -79,6 +86,17 @@ def ref():
     if _ref is None:
         _ref = _bind(C.CDLL(REF_SO))
     return _ref
+
+
+def have_ms_ref():
+    return os.path.exists(MS_REF_SO)
+
+
+def ms_ref():
+    global _ms_ref
+    if _ms_ref is None:
+        _ms_ref = _bind(C.CDLL(MS_REF_SO))
+    return _ms_ref
 
 
 def call_smolar(fn, nx, ny, ns, dx, dy, ds, dt, u0, u1, v0, v1, w0, w1, c0, c1, order,
