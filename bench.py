@@ -374,13 +374,21 @@ def deform_leg(M, nsteps=200):
 
 def ensemble_leg(M, n):
     """Config 4: n instances of uneven sizes, iord 2 nonoscillatory, 100 steps as one
-    autotuned hipGraph replay against per-instance launch sequences."""
+    autotuned hipGraph replay against per-instance launch sequences; afterwards one
+    instance of each size against a single grid (bitwise) and the smallest against the
+    oracle."""
     import mpdata_amd.configs as K
+
+    import oracle_lib as O
 
     t0 = time.perf_counter()
     b = K.ensemble(n, seed=4)
     cells = b.cells()
     setup = time.perf_counter() - t0
+    # the first instance of each size, checked after the run against single grids
+    dims = K.ensemble_dims(n, seed=4)
+    firsts = sorted({d: i for i, d in reversed(list(enumerate(dims)))}.values())
+    start = {i: (b.instances[i].get_courant(), b.instances[i].field()) for i in firsts}
     tx, ty, zc, pers = b.autotune(2)
     lim = b.tiles()[1]
     b.step(2, M.BATCH_GRAPH)
@@ -388,7 +396,22 @@ def ensemble_leg(M, n):
     ms_fused = b.step_timed(10, M.BATCH_FUSED) / 10
     b.step(1, M.BATCH_PER_INSTANCE)
     ms_per = b.step_timed(10, M.BATCH_PER_INSTANCE) / 10
+    taken = 2 + 100 + 10 + 1 + 10
     mass_ok = all(np.isfinite(g.stats()[0]) for g in b.instances[:4])
+    # every mode gives the bits of a single grid (default launch shape), and the smallest
+    # instance stays within the tolerance of the oracle over the whole run
+    same, err = True, None
+    for i in firsts:
+        (U, V, W), c = start[i]
+        got = b.instances[i].field()
+        g = M.Grid(*dims[i], M.PERIODIC, iord=2, nonosc=True)
+        g.set_courant(U, V, W)
+        g.set_field(c)
+        g.step(taken)
+        same = same and bool(np.array_equal(got, g.field()))
+        g.close()
+        if dims[i] == min(dims, key=lambda d: d[0] * d[1] * d[2]):
+            err = relerr(got, O.per_run(*dims[i], U, V, W, 2, True, c, taken))
     b.close()
     return {"workload": f"{n} periodic instances from {{64^3, 96^3, 128^3, 160x160x96, "
                         f"192x192x64}} (seed 4), rotation + 10% seeded modes, iord 2 "
@@ -400,7 +423,10 @@ def ensemble_leg(M, n):
             "per_instance_ms_per_step": ms_per,
             "graph_cell_updates_per_s": cells / (ms_graph * 1e-3),
             "per_instance_cell_updates_per_s": cells / (ms_per * 1e-3),
-            "speedup_graph_vs_per_instance": ms_per / ms_graph, "finite": mass_ok}
+            "speedup_graph_vs_per_instance": ms_per / ms_graph, "finite": mass_ok,
+            f"bitwise_vs_single_grids_after_{taken}_steps":
+                {"instances": firsts, "equal": same},
+            f"rel_err_smallest_vs_oracle_after_{taken}_steps": err}
 
 
 def large_leg(M, n, iord, arith):
