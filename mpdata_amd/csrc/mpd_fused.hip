@@ -402,10 +402,11 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
             hnaz = ld(a.Az + zp0.off, hcolb);
         }
     }
-    // the plane loop, unrolled by hand (two planes per trip, an odd last plane after
-    // it): across the two copies the compiler renames the registers the planes below
-    // and above rotate through instead of copying them (the barriers inside keep it
-    // from unrolling a loop of unknown trip count itself)
+    // the plane loop, unrolled by hand (four planes per trip, then two, then an odd last
+    // plane): across the copies the compiler renames the registers the planes below and
+    // above rotate through instead of copying them (the barriers inside keep it from
+    // unrolling a loop of unknown trip count itself).  The SAVE variant, held to three
+    // waves per SIMD, would spill with four copies and keeps two.
     auto plane = [&](const int pl) __attribute__((always_inline)) {
         const int p = __builtin_amdgcn_readfirstlane(pl);  // uniform, as in f1_tile
         keep32(colb);
@@ -415,19 +416,28 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
         const double hcnn = hncn, hcpn = hncp, haxn = hnax, hayn = hnay, hazc = hnaz;
         const double qk = nq;
         if (a.q && p >= z0 && p < z1) nq = ld(a.q + zp0.off, colb);
-        if (p < z1) {  // the next iteration's values: fields at p + 2, faces at p + 1
-            ncn = ld(a.cn + zp2.off, colb);
-            ncp = ld(a.cp + zp2.off, colb);
-            nax = ld(a.Ax + zp1.off, colb);
-            nay = ld(a.Ay + zp1.off, colb);
-            naz = ld(a.Az + zp1.off, colb);
-            if (hl) {
-                hncn = ld(a.cn + zp2.off, hcolb);
-                hncp = ld(a.cp + zp2.off, hcolb);
-                hnax = ld(a.Ax + zp1.off, hcolb);
-                hnay = ld(a.Ay + zp1.off, hcolb);
-                if (h1) hnaz = ld(a.Az + zp1.off, hcolb);
-            }
+        {
+            // the next iteration's values: fields at p + 2, faces at p + 1.  Loaded by every
+            // lane in every iteration (the last one reloads planes already in the cache, a
+            // lane without a halo cell the wrapped column of region cell (0, 0)): loads
+            // under a condition kept the previous values alive beside the new ones and
+            // cost ~15 register copies per plane.  Fresh copies of the column offsets keep
+            // the loads in the scalar-base form.
+            const bool more = p < z1;
+            const size_t o2 = more ? zp2.off : zp1.off, o1 = more ? zp1.off : zp0.off;
+            unsigned c2 = colb, h2 = hcolb;
+            keep32(c2);
+            keep32(h2);
+            ncn = ld(a.cn + o2, c2);
+            ncp = ld(a.cp + o2, c2);
+            nax = ld(a.Ax + o1, c2);
+            nay = ld(a.Ay + o1, c2);
+            naz = ld(a.Az + o1, c2);
+            hncn = ld(a.cn + o2, h2);
+            hncp = ld(a.cp + o2, h2);
+            hnax = ld(a.Ax + o1, h2);
+            hnay = ld(a.Ay + o1, h2);
+            hnaz = ld(a.Az + o1, h2);
         }
         const int k2 = (p + 1) % 2;  // slot of plane p - 1
         if constexpr (SAVE) {
@@ -566,6 +576,14 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
         zm1.next();
     };
     int pl = z0 - 1;
+    if constexpr (!SAVE) {
+        for (; pl + 3 <= z1; pl += 4) {
+            plane(pl);
+            plane(pl + 1);
+            plane(pl + 2);
+            plane(pl + 3);
+        }
+    }
     for (; pl + 1 <= z1; pl += 2) {
         plane(pl);
         plane(pl + 1);
