@@ -121,11 +121,14 @@ int mpd_set_graph(mpd_grid g, int on);
 /* Planes per workgroup of the fused kernels (0 = chosen from the grid size). */
 int mpd_set_zchunk(mpd_grid g, int zc);
 /* Times every tile shape x z-chunk length on the grid itself (one warm-up step and nsteps
- * timed steps each), keeps for each of the two fused kernels the shape in which it ran
- * fastest, and reports the first kernel's (mpd_get_tiles gives both); the field buffers
- * are restored, so the grid's state is unchanged.  Results are cached per process by shape and options,
- * and in the file named by the environment variable MPD_TUNE_CACHE when set.  PERIODIC
- * grids with iord >= 2 on the fused path only (MPD_ERR_UNSUPPORTED otherwise). */
+ * timed steps each, each kernel between events of its own), then times whole steps of the
+ * leading pairings (the two fastest shapes of each of the two fused kernels, and the
+ * fastest shape for both together; three interleaved rounds) and keeps the fastest.  It
+ * reports the first kernel's shape (mpd_get_tiles gives both).  The field buffers are
+ * restored, and the trial steps neither capture graphs nor count for the step sentinel,
+ * so the grid's state is unchanged.  Results are cached per process by shape and
+ * options, and in the file named by the environment variable MPD_TUNE_CACHE when set.
+ * PERIODIC grids with iord >= 2 on the fused path only (MPD_ERR_UNSUPPORTED otherwise). */
 int mpd_autotune(mpd_grid g, int nsteps, int* tile_x, int* tile_y, int* zc);
 
 /* ------------------------------------------------------------ generators --- */

@@ -1,6 +1,8 @@
 // C ABI of the library: the drop-in smolar()/msmolar()/msmolar_gather() of the
 // reference (smolar.h:45-48, 95-99, 132-136) and the device-resident mpd_* interface.
 #include <algorithm>
+#include <array>
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <string>
@@ -699,14 +701,37 @@ extern "C" int mpd_autotune(mpd_grid g, int nsteps, int* tile_x, int* tile_y, in
                 MPD_HIP(hipMemcpyAsync(save[b].p, g->c[b].p, g->ncell * 8, hipMemcpyDeviceToDevice, s));
             }
             const int cur = g->cur;
+            // trial steps neither capture graphs nor count for the step sentinel (restored
+            // on every way out)
+            struct Quiet {
+                mpd_grid_s* g;
+                bool graph;
+                int every;
+                long long steps;
+                ~Quiet()
+                {
+                    g->use_graph = graph;
+                    g->sentinel_every = every;
+                    g->sentinel_steps = steps;
+                }
+            } quiet{g, g->use_graph, g->sentinel_every, g->sentinel_steps};
+            g->use_graph = false;
+            g->sentinel_every = 0;
             const int shapes[][2] = {
 #define MPD_TILE(X, Y) {X, Y},
                 MPD_FUSED_TILES(MPD_TILE)
 #undef MPD_TILE
             };
-            // per kernel: the fastest shape (its kernels of all passes summed)
-            double best[2] = {1e300, 1e300};
-            int bestc[2][3] = {{g->tile_x, g->tile_y, g->zc}, {g->tile_x, g->tile_y, g->zc}};
+            // per kernel: the two fastest shapes (its kernels of all passes summed); and
+            // the fastest shape for both kernels together
+            double best[2][2] = {{1e300, 1e300}, {1e300, 1e300}}, bestj = 1e300;
+            int bestc[2][2][3], bestjc[3] = {g->tile_x, g->tile_y, g->zc};
+            for (int f = 0; f < 2; f++)
+                for (int r = 0; r < 2; r++) {
+                    bestc[f][r][0] = g->tile_x;
+                    bestc[f][r][1] = g->tile_y;
+                    bestc[f][r][2] = g->zc;
+                }
             for (auto& sh : shapes) {
                 int last = 0;
                 for (int z : {8, 11, 16, 22, 32, 43, 64}) {
@@ -726,24 +751,58 @@ extern "C" int mpd_autotune(mpd_grid g, int nsteps, int* tile_x, int* tile_y, in
                     nk = std::min(nk, (int)ms.size());
                     double t[2] = {0.0, 0.0};  // kernels alternate F1, F2 (or donor) per pass
                     for (int k = 0; k < nk; k++) t[k & 1] += ms[k];
-                    for (int f = 0; f < 2; f++)
-                        if (t[f] < best[f]) {
-                            best[f] = t[f];
-                            bestc[f][0] = sh[0];
-                            bestc[f][1] = sh[1];
-                            bestc[f][2] = z;
+                    const int here[3] = {sh[0], sh[1], z};
+                    for (int f = 0; f < 2; f++) {
+                        if (t[f] < best[f][0]) {
+                            best[f][1] = best[f][0];
+                            std::copy(bestc[f][0], bestc[f][0] + 3, bestc[f][1]);
+                            best[f][0] = t[f];
+                            std::copy(here, here + 3, bestc[f][0]);
+                        } else if (t[f] < best[f][1]) {
+                            best[f][1] = t[f];
+                            std::copy(here, here + 3, bestc[f][1]);
                         }
+                    }
+                    if (t[0] + t[1] < bestj) {
+                        bestj = t[0] + t[1];
+                        std::copy(here, here + 3, bestjc);
+                    }
                 }
             }
+            // Kernel times alone do not rank whole steps reliably (how one kernel leaves
+            // the caches for the next depends on both shapes): the leading pairings are
+            // timed as whole steps, interleaved over three rounds, best of each kept.
+            std::vector<std::array<int, 6>> cand;
+            auto add = [&](const int* k1, const int* k2) {
+                std::array<int, 6> x = {k1[0], k1[1], k1[2], k2[0], k2[1], k2[2]};
+                if (!g->nonosc) x[3] = x[4] = x[5] = 0;  // the second kernel is no F2
+                if (std::find(cand.begin(), cand.end(), x) == cand.end()) cand.push_back(x);
+            };
+            add(bestc[0][0], bestc[1][0]);
+            add(bestc[0][0], bestc[1][1]);
+            add(bestc[0][1], bestc[1][0]);
+            add(bestjc, bestjc);
+            const double est = std::max(best[0][0] + best[1][0], 1e-3);
+            const int ns = std::max(nsteps, std::min(256, (int)std::ceil(2.0 / est)));
+            std::vector<double> tstep(cand.size(), 1e300);
+            for (int round = 0; round < 3; round++)
+                for (size_t i = 0; i < cand.size(); i++) {
+                    g->tile_x = cand[i][0];
+                    g->tile_y = cand[i][1];
+                    g->zc = cand[i][2];
+                    g->tile2_x = cand[i][3];
+                    g->tile2_y = cand[i][4];
+                    g->zc2 = cand[i][5];
+                    double msr = 0.0;
+                    if ((rc = mpd_step(g, 1)) || (rc = mpd_step_timed(g, ns, &msr))) return rc;
+                    tstep[i] = std::min(tstep[i], msr);
+                }
+            const size_t win = std::min_element(tstep.begin(), tstep.end()) - tstep.begin();
             for (int b = 0; b < 2; b++)
                 MPD_HIP(hipMemcpyAsync(g->c[b].p, save[b].p, g->ncell * 8, hipMemcpyDeviceToDevice, s));
             g->cur = cur;
             MPD_HIP(hipStreamSynchronize(s));
-            for (int i = 0; i < 3; i++) {
-                c[i] = bestc[0][i];
-                c[3 + i] = bestc[1][i];
-            }
-            if (!g->nonosc) c[3] = c[4] = c[5] = 0;  // the second kernel is no F2
+            std::copy(cand[win].begin(), cand[win].end(), c);
         }
         tune_store(key, 6, c);
     }

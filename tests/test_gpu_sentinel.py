@@ -96,3 +96,24 @@ def test_sentinel_on_a_frame_grid_and_with_graph_replay_requested():
     g.step(16)
     assert g.sentinel_check() is None
     g.close()
+
+
+def test_autotune_trial_steps_do_not_count():
+    """The autotuner's trial steps neither run the sentinel nor advance its step count:
+    armed every 4 steps, 3 steps, a tuning (many trial steps), a NaN planted from the
+    host, 1 step -> flagged after step 4."""
+    dims = (36, 20, 11)  # a shape no other test tunes
+    U, V, W, c = case(*dims, 7)
+    g = grid(dims, True, U, V, W, c)
+    g.set_sentinel(4, nonnegative=True)
+    g.step(3)
+    before = g.field()
+    g.autotune(1)
+    np.testing.assert_array_equal(g.field(), before)
+    assert g.sentinel_check() is None
+    bad = before.copy()
+    bad[123] = np.nan
+    g.set_field(bad)
+    g.step(1)
+    assert g.sentinel_check() == 4
+    g.close()
