@@ -92,6 +92,15 @@ __device__ __forceinline__ double limit_face(double u, double bup_l, double bdn_
     return mn2(1.0, mn2(bdn_l, bup_r)) * pos(u) + mn2(1.0, mn2(bup_l, bdn_r)) * neg(u);
 }
 
+// The same for ratios already clamped to at most 1 where they were computed (min is
+// exact, so min(1, min(a, b)) = min(min(1, a), min(1, b)): the same values, and the clamp
+// is paid once per cell instead of four times per face pair).
+__device__ __forceinline__ double limit_face_clamped(double u, double bup_l, double bdn_l,
+                                                     double bup_r, double bdn_r)
+{
+    return mn2(bdn_l, bup_r) * pos(u) + mn2(bup_l, bdn_r) * neg(u);
+}
+
 // Inflow and outflow sums of one cell from its six donor fluxes (0.5 F2; lower and upper
 // face of x, y, z).  The adjoint recomputes them with this same function, so its values
 // are those of the forward bit for bit.

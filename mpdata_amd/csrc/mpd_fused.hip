@@ -511,6 +511,10 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
             const double fyl = 0.5 * F2(nbp[3], c, ayl), fyr = 0.5 * F2(c, nbp[4], ayr);
             const double fzl = 0.5 * F2(cplo, c, azlo), fzr = 0.5 * F2(c, cphi, azhi);
             beta(hi, lo, c, fxl, fxr, fyl, fyr, fzl, fzr, bu, bd);
+            if constexpr (!SAVE) {  // clamped once here (SAVE records the ratios themselves)
+                bu = mn2(bu, 1.0);
+                bd = mn2(bd, 1.0);
+            }
         };
         const double axl = s_ax[s2][L - 1], ayd = s_ay[s2][L - RX];
         double buc, bdc;
@@ -532,12 +536,16 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
             const double* bd = s_bd[k2];
             const double* cq = s_cp[k3];
             const double b0u = bum, b0d = bdm;
-            const double uxr = limit_face(axm, b0u, b0d, bu[L + 1], bd[L + 1]);
-            const double uxl = limit_face(axlm, bu[L - 1], bd[L - 1], b0u, b0d);
-            const double vyr = limit_face(aym, b0u, b0d, bu[L + RX], bd[L + RX]);
-            const double vyl = limit_face(aydm, bu[L - RX], bd[L - RX], b0u, b0d);
-            const double wzr = limit_face(azm, b0u, b0d, buc, bdc);
-            const double wzl = limit_face(azm2, bum2, bdm2, b0u, b0d);
+            auto lim = [](double u, double bul, double bdl, double bur, double bdr) {
+                return SAVE ? limit_face(u, bul, bdl, bur, bdr)
+                            : limit_face_clamped(u, bul, bdl, bur, bdr);
+            };
+            const double uxr = lim(axm, b0u, b0d, bu[L + 1], bd[L + 1]);
+            const double uxl = lim(axlm, bu[L - 1], bd[L - 1], b0u, b0d);
+            const double vyr = lim(aym, b0u, b0d, bu[L + RX], bd[L + RX]);
+            const double vyl = lim(aydm, bu[L - RX], bd[L - RX], b0u, b0d);
+            const double wzr = lim(azm, b0u, b0d, buc, bdc);
+            const double wzl = lim(azm2, bum2, bdm2, b0u, b0d);
             double r = donor(cpm, cq[L - 1], cq[L + 1], cq[L - RX], cq[L + RX], cpm2, cpc, uxl,
                              uxr, vyl, vyr, wzl, wzr);
             if (store) {
