@@ -165,6 +165,10 @@ __device__ __forceinline__ void f1_tile(const FusedArgs& a, int item)
         keep32(colb);
         keep32(hcolb);
         const int s3 = (p + 3) % 3;
+        // raised issue priority from the next plane's loads to the first barrier, so that
+        // a wave's loads and LDS writes go out ahead of other waves' arithmetic (measured
+        // 0.3 % on the step; the same in f2_tile 0.5 %)
+        __builtin_amdgcn_s_setprio(2);
         const double c0n = nc0, un = nu, vn = nv, wn = nw, fld = nf;
         const double hc0n = hnc0, hun = hnu, hvn = hnv, hwn = hnw, hfld = hnf;
         if (p < z1) {  // the next iteration's values: c^n at p + 2, faces at p + 1
@@ -194,6 +198,7 @@ __device__ __forceinline__ void f1_tile(const FusedArgs& a, int item)
             s_v[s3][HL] = hvn;
         }
         lds_barrier();
+        __builtin_amdgcn_s_setprio(0);
         double c1p;
         if (DONOR) {
             c1p = donor(c0c, s_c0[L - 1], s_c0[L + 1], s_c0[L - RX], s_c0[L + RX], c0m, c0n,
@@ -412,6 +417,7 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
         keep32(colb);
         keep32(hcolb);
         const int s2 = (p + 2) % 2, s3 = (p + 3) % 3;
+        __builtin_amdgcn_s_setprio(2);  // until the barrier, as in f1_tile
         const double cnn = ncn, cpn = ncp, axn = nax, ayn = nay, azc = naz;
         const double hcnn = hncn, hcpn = hncp, haxn = hnax, hayn = hnay, hazc = hnaz;
         const double qk = nq;
@@ -467,6 +473,7 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
             s_bd[k2][HL] = hbdm;
         }
         lds_barrier();
+        __builtin_amdgcn_s_setprio(0);
         // limiter ratios of plane p (see cpu_ref.c)
         const double* cpp = s_cp[s3];
         auto ratios = [&](int M, double cn0, double cnlo, double cnhi, double c, double cplo,
