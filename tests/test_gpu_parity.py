@@ -295,12 +295,17 @@ def test_autotune_keeps_state_and_parity():
             f.close()
 
 
-@pytest.mark.parametrize("zc", [1, 2, 5, 9, 40])
-def test_zchunk_lengths_match_oracle(zc):
+@pytest.mark.parametrize("iord", [2, 3])
+@pytest.mark.parametrize("zc", [1, 2, 3, 4, 5, 6, 9, 40])
+def test_zchunk_lengths_match_oracle(zc, iord):
+    """Every z-chunk length, bitwise: a chunk of zc planes streams zc + 2 planes through
+    loops unrolled by four, then two, then one, so 1..9 cover each remainder (nz = 9
+    also leaves short last chunks); iord 3 adds the limited faces stored for the next
+    pass."""
     nx, ny, nz = 33, 17, 9
     U, V, W, c = per_case(nx, ny, nz, 8)
-    want = O.per_run(nx, ny, nz, U, V, W, 2, True, c, 2)
-    g = M.Grid(nx, ny, nz, M.PERIODIC, iord=2, nonosc=True)
+    want = O.per_run(nx, ny, nz, U, V, W, iord, True, c, 2)
+    g = M.Grid(nx, ny, nz, M.PERIODIC, iord=iord, nonosc=True)
     g.set_arith(M.ARITH_EXACT)
     g.set_zchunk(zc)
     g.set_courant(U, V, W)
