@@ -1314,6 +1314,10 @@ extern "C" int mpd_sample_receptors(mpd_grid g, int nsteps, int nrec, const int*
     clear_error();
     if (!g || nsteps < 1 || nrec < 1 || !receptors || sample_every < 1 || !out)
         return fail(MPD_ERR_ARG, "mpd_sample_receptors: bad arguments");
+    // the loop below steps with periodic_step(), whose kernels index the face arrays in
+    // the periodic layout (ncell elements each); a FRAME grid's arrays are shorter
+    if (g->boundary != MPD_PERIODIC)
+        return fail(MPD_ERR_UNSUPPORTED, "mpd_sample_receptors: PERIODIC grids only");
     if (!g->has_winds || !g->has_field)
         return fail(MPD_ERR_ARG, "mpd_sample_receptors: set the Courant numbers and the field first");
     for (int q = 0; q < nrec; q++)

@@ -209,3 +209,21 @@ def test_kept_segment_records_give_the_same_gradient(nonosc):
     assert abs(out[0][0] - Jc) <= 1e-13 * Jc
     assert np.abs(out[0][1] - g0c).max() <= 1e-11 * np.abs(g0c).max()
     assert np.abs(out[0][2] - gqc).max() <= 1e-11 * np.abs(gqc).max()
+
+
+def test_receptor_sampling_rejects_frame_grids():
+    """mpd_sample_receptors steps with the periodic kernels; a FRAME grid (face arrays in
+    the reference's shorter layouts) is refused before any kernel is launched."""
+    nx, ny, nz = 9, 8, 7
+    n = nx * ny * nz
+    g = M.Grid(nx, ny, nz, M.FRAME, iord=2, nonosc=False)
+    one = np.ones(nx), np.ones(ny), np.ones(nz)
+    z = np.zeros(n)
+    g.set_winds(*one, 1.0, z, z, z, z, z, z)
+    c = np.linspace(0.5, 1.5, n)
+    g.set_field(c)
+    with pytest.raises(M.MpdError) as e:
+        g.sample_receptors(4, np.array([1, 5], np.int32), 2)
+    assert e.value.code == 4  # MPD_ERR_UNSUPPORTED
+    np.testing.assert_array_equal(g.field(), c)  # nothing was stepped
+    g.close()
