@@ -67,6 +67,9 @@ int mpd_get_field(mpd_grid g, double* c);
 int mpd_get_previous_field(mpd_grid g, double* c);
 /* Per-step emission increment (host array, nx*ny*nz), or NULL to clear. */
 int mpd_set_emission(mpd_grid g, const double* q);
+/* The emission increment set by mpd_set_emission or mpd_generate_blobs (host array,
+ * nx*ny*nz); MPD_ERR_ARG when none is set. */
+int mpd_get_emission(mpd_grid g, double* q);
 /* Enqueue nsteps steps; returns without waiting. */
 int mpd_step(mpd_grid g, int nsteps);
 int mpd_synchronize(mpd_grid g);

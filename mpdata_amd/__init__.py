@@ -71,6 +71,7 @@ SIGNATURES = {
     "mpd_get_field": (C.c_int, [C.c_void_p, PD]),
     "mpd_get_previous_field": (C.c_int, [C.c_void_p, PD]),
     "mpd_set_emission": (C.c_int, [C.c_void_p, PD]),
+    "mpd_get_emission": (C.c_int, [C.c_void_p, PD]),
     "mpd_step": (C.c_int, [C.c_void_p, C.c_int]),
     "mpd_synchronize": (C.c_int, [C.c_void_p]),
     "mpd_field_device_ptr": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
@@ -258,6 +259,11 @@ class Grid:
         else:
             q = np.ascontiguousarray(q, dtype=np.float64)
             _check(lib().mpd_set_emission(self.h, _p(q)))
+
+    def emission(self):
+        out = np.empty(self.n)
+        _check(lib().mpd_get_emission(self.h, _p(out)))
+        return out
 
     def step(self, nsteps=1):
         _check(lib().mpd_step(self.h, nsteps))

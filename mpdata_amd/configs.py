@@ -16,6 +16,11 @@ seed gives the same configuration on any host.  Heavy per-cell work runs on the 
                       each the ring3d rotation about its own centre plus a seeded
                       divergence-free perturbation of 10% of the rotation speed, max
                       Courant 0.4, its own Gaussian blob
+  inversion_grid(n)   source inversion: periodic n^3, the deform wind and field of seed 5
+                      with an emission of 8 seeded positive Gaussians; with `factors`,
+                      the twin run whose emission amplitudes are scaled by them
+  inversion_receptors(ncell)  64 seeded distinct receptor cells and the twin's factors
+                      1 + 0.2 xi, xi ~ U[-1, 1] per source
 """
 import ctypes as C
 import math
@@ -190,3 +195,28 @@ def ensemble(n=192, seed=4, iord=2):
         c = np.array([nx / 2, ny / 2, nz / 2]) + off
         generate_blobs(g, [c], [8.0], [1.0], periodic=True)
     return b
+
+
+INVERSION_SEED = 5
+
+
+def inversion_receptors(ncell, nrec=64, nsources=8, spread=0.2, seed=5005):
+    """(receptor cells: nrec distinct seeded indices below ncell, sorted; the twin run's
+    emission factors 1 + spread*xi, xi ~ U[-1, 1], one per source)."""
+    rng = SplitMix64(seed)
+    rec = sorted({rng.randint(ncell) for _ in range(nrec * 2)})[:nrec]
+    factors = [1.0 + spread * rng.uniform(-1.0, 1.0) for _ in range(nsources)]
+    return np.array(rec, np.int32), factors
+
+
+def inversion_grid(n, nonosc=True, factors=None, iord=2):
+    """A periodic n^3 grid holding the source-inversion case: the deformational wind and
+    32 Gaussians of seed 5 and an emission of 8 seeded positive Gaussians (amplitudes
+    scaled by `factors` for the twin run that makes the observations)."""
+    shape = (n, n, n)
+    g = Grid(n, n, n, PERIODIC, iord=iord, nonosc=nonosc)
+    cen, sig, amp = deform(g, shape, seed=INVERSION_SEED, emission_blobs=8)
+    if factors is not None:
+        generate_blobs(g, cen, sig, [a * f for a, f in zip(amp, factors)], periodic=True,
+                       target=TARGET_EMISSION)
+    return g

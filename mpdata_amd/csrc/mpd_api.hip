@@ -371,6 +371,16 @@ extern "C" int mpd_set_emission(mpd_grid g, const double* q)
     return sync();
 }
 
+extern "C" int mpd_get_emission(mpd_grid g, double* q)
+{
+    GRID_OR_FAIL(g);
+    if (!q) return fail(MPD_ERR_ARG, "mpd_get_emission: NULL array");
+    if (!g->has_q) return fail(MPD_ERR_ARG, "mpd_get_emission: no emission is set");
+    int rc = d2h(q, g->q.p, g->ncell);
+    if (rc) return rc;
+    return sync();
+}
+
 // Steps per captured graph: even, so a replay leaves the field parity unchanged.
 constexpr int kGraphSteps = 8;
 
