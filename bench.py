@@ -1,20 +1,32 @@
 """MPDATA step benchmark on one MI355X.
 
-Workload (default): ring3d plain MPDATA -- 128^3 triply periodic FP64 grid, solid-body
-rotation about the axis (1,1,1)/sqrt(3) through the centre scaled to a maximum face
-Courant number of 0.4, a Gaussian blob (sigma 8 cells) 30 cells off the axis, iord=2
-with the nonoscillatory limiter, 128 steps.  A step is one MPDATA time step over the
-whole grid; the fields are resident in HBM before timing starts.
+Workload (the headline line): the large single-instance case -- a 512 x 512 x 256
+triply periodic FP64 grid, divergence-free deformational wind curl(A) of 4 seeded
+Fourier modes per potential component scaled to a maximum face Courant number of 0.5,
+the field a seeded sum of 32 positive Gaussians (sigma 4..16 cells), iord=3 with the
+nonoscillatory limiter.  A step is one MPDATA time step over the whole grid; the fields
+are resident in HBM before timing starts.  The largest single-GPU configuration, and the
+one whose working set (~7 GB) does not fit the 256 MB Infinity Cache.
 
 Prints ONE JSON line:
   value        cell-updates/s over the K timed steps (HIP events on the launch stream)
   roofline     algorithmic bytes of a step (40 B per cell-update: read c^n and the three
                face Courant numbers, write c^{n+1}) over the summed average durations of
-               the step's kernels, against MI355X's 8 TB/s HBM peak; per-kernel figures
-               in roofline.kernels
-  parity       max|gpu - oracle| / max|oracle| after 1 step and after the full run
-  cpu_baseline the oracle (FP64 C restatement of the reference step, 1 thread) timed on
-               the host cores over the same run, which is also the parity reference
+               the step's kernels (each launched between HIP events of its own), against
+               MI355X's 8 TB/s HBM peak; traffic = rocprof HBM bytes of a step (FETCH_SIZE
+               + WRITE_SIZE, calibrated against a plain copy) when measured for the same
+               launch shapes; per-kernel figures in roofline.kernels
+  parity       max|gpu - oracle| / max|oracle| after the CPU-baseline steps, and the whole
+               timed trajectory against the exact-arithmetic kernels (which equal the
+               oracle bit for bit)
+  cpu_baseline the oracle (FP64 C restatement of the reference step with the periodic and
+               limiter options, 1 thread) timed on the host over the first steps of the
+               same workload, which are also the parity reference; cpu_calibration times
+               the reference's own smolar() (compiled from its sources, when present)
+               against the restatement's on the same frame-mode input
+Legs (objects on the same line): ring3d 128^3 (the cache-resident rotation case), the
+256^3 rotation, the 192-instance ensemble, ring2d, the drop-in frame path and the
+source-inversion adjoint.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps S] [--no-cpu]
   python bench.py --calibrate-copy N     (plain device copy of N doubles, for counters)
@@ -30,6 +42,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published peak
 BYTES_PER_CELL = 40    # c^n + U + V + W read, c^{n+1} written (FP64)
+DEFORM_SHAPE = (512, 512, 256)
 
 
 def face_courant_from_cell_winds(u, v, w, dt=1.0):
@@ -82,39 +95,109 @@ def host_cpu_model():
     return "unknown"
 
 
+def host_cores():
+    """Logical CPUs of the host and those this process may run on."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"logical_cpus": os.cpu_count(), "available_to_process": avail,
+            "model": host_cpu_model()}
+
+
 def relerr(got, want):
     return float(np.abs(got - want).max() / np.abs(want).max())
+
+
+def step_kernels(M, iord, kms, path=0):
+    """Names and own input/output bytes per cell of the kernels of one step, in launch
+    order, with their measured average durations."""
+    if path == 1:
+        return [{"name": "staged", "avg_ms": t, "io_bytes_per_cell": None, "io_GB/s": None}
+                for t in kms]
+    names, own = [], []
+    for p in range(1, iord):
+        names += [f"k_f1 (pass {p})", f"k_f2 (pass {p})"]
+        last = p == iord - 1
+        own += [64 if p == 1 else 56, 48 if last else 72]
+    return [{"name": nm, "avg_ms": t, "io_bytes_per_cell": b}
+            for nm, t, b in zip(names, kms, own)]
+
+
+def roofline(cells, kms, kernels, tfile, tuned):
+    """Step-level roofline: 40 algorithmic bytes per cell over the summed kernel time."""
+    kern_ms = sum(kms)
+    achieved = cells * BYTES_PER_CELL / (kern_ms * 1e-3) / 1e9
+    for k in kernels:
+        b = k["io_bytes_per_cell"]
+        k["io_GB/s"] = (cells * b / (k["avg_ms"] * 1e-3) / 1e9) if b else None
+    dominant = max(kernels, key=lambda k: k["avg_ms"])
+    traffic = None
+    if tfile and os.path.exists(tfile):
+        t = json.load(open(tfile))
+        # counters measured with the same launch shapes only
+        if t.get("config") == tuned:
+            traffic = t.get("hbm_bytes_per_step")
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "algorithmic_bytes_per_step": cells * BYTES_PER_CELL, "step_kernel_ms": kern_ms,
+            "dominant_kernel": dominant["name"], "kernels": kernels}
+
+
+def cpu_calibration(O, n=128, order=2, reps=2):
+    """The restatement's frame-mode step (orc_smolar) against the reference's own
+    smolar() compiled from its sources (oracle/_ref), same n^3 input, best of reps."""
+    if not O.have_ref():
+        return None
+    case = rotation_case(n)
+    u, v, w, c = case["U"], case["V"], case["W"], case["c"]
+    one = np.ones(n)
+    out, best = {}, {}
+    for name, fn in (("reference", O.ref().smolar), ("restatement", O.oracle().orc_smolar)):
+        best[name] = 1e30
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            _, _, res = O.call_smolar(fn, n, n, n, one, one, one, 1.0, u, u, v, v, w, w,
+                                      c.copy(), np.zeros_like(c), order)
+            best[name] = min(best[name], time.perf_counter() - t0)
+        out[name] = res
+    cells = (n - 2) ** 3
+    return {"input": f"frame mode {n}^3 order {order}, rotation winds (the reference's "
+                     "semantics, no limiter), 1 thread",
+            "reference_cell_updates_per_s": cells / best["reference"],
+            "restatement_cell_updates_per_s": cells / best["restatement"],
+            "restatement_over_reference_time": best["restatement"] / best["reference"],
+            "bitwise_equal": bool(np.array_equal(out["reference"], out["restatement"]))}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=128)
-    ap.add_argument("--warmup", type=int, default=64)
-    ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--iord", type=int, default=2)
-    ap.add_argument("--path", type=int, default=0, help="0 auto, 1 staged, 2 fused")
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--arith", type=int, default=1, help="0 exact, 1 fast")
-    ap.add_argument("--cpu-steps", type=int, default=128,
-                    help="oracle steps: the CPU baseline sample and the parity reference")
+    ap.add_argument("--cpu-steps", type=int, default=3,
+                    help="oracle steps of the headline workload: the CPU baseline sample "
+                         "and the parity reference")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="N")
+    ap.add_argument("--ring3d", type=int, default=128,
+                    help="edge of the cache-resident rotation leg (0: skip)")
     ap.add_argument("--ensemble", type=int, default=192,
                     help="instances of the batched-ensemble comparison (0: skip)")
     ap.add_argument("--large", type=int, default=256,
-                    help="edge of the large single-grid roofline run (0: skip)")
+                    help="edge of the large rotation roofline leg (0: skip)")
     ap.add_argument("--adjoint", type=int, default=256,
                     help="edge of the source-inversion adjoint leg (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=512,
                     help="time steps of the adjoint leg's loss window")
+    ap.add_argument("--adjoint-checks", type=int, default=1,
+                    help="full-size dot-product test, FD directions and eps sweeps (0: skip)")
     ap.add_argument("--dropin", type=int, default=128,
                     help="edge of the drop-in (FRAME, smolar semantics) leg (0: skip)")
     ap.add_argument("--ring2d", type=int, default=128,
                     help="edge of the periodic 2-D ring leg, 1024 steps (0: skip)")
-    ap.add_argument("--deform", type=int, default=1,
-                    help="run the 512x512x256 iord-3 deformational leg (0: skip)")
-    ap.add_argument("--deform-oracle-steps", type=int, default=1,
-                    help="steps of the deformational leg checked against the oracle")
+    ap.add_argument("--legs", type=int, default=1, help="0: the headline only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU (every box has one); the "
@@ -133,58 +216,64 @@ def main():
                           "GB/s": nbytes / ms / 1e6}))
         return
 
-    n = args.n
-    case = rotation_case(n)
-    cells = n ** 3
+    line = headline(M, args)
+    if args.legs:
+        if args.ring3d:
+            line["ring3d"] = ring3d_leg(M, args.ring3d, args.arith)
+        if args.ensemble:
+            line["ensemble"] = ensemble_leg(M, args.ensemble)
+        if args.large:
+            line["large"] = large_leg(M, args.large, 2, args.arith)
+        if args.ring2d:
+            line["ring2d"] = ring2d_leg(M, args.ring2d)
+        if args.dropin:
+            line["dropin"] = dropin_leg(M, args.dropin)
+        if args.adjoint:
+            line["adjoint"] = adjoint_leg(M, args.adjoint, nsteps=args.adjoint_steps,
+                                          checks=bool(args.adjoint_checks))
+    print(json.dumps(line))
 
-    def make_grid():
-        g = M.Grid(n, n, n, M.PERIODIC, iord=args.iord, nonosc=True)
-        g.set_path(args.path)
-        g.set_arith(args.arith)
-        g.set_courant(case["U"], case["V"], case["W"])
-        g.set_field(case["c"])
-        return g
 
-    g = make_grid()
-    tuned = None
-    if args.path != 1 and args.iord >= 2:
-        g.autotune(2)  # each kernel's tile shape and z-chunk, chosen on this grid (state kept)
-        tuned = tuned_config(g)
+def headline(M, args):
+    """The 512 x 512 x 256 iord-3 deformational case: autotuned launch shapes, W warm-up
+    steps, K timed steps; then the oracle (1 thread, the CPU baseline) over the first S
+    steps of the same workload against a GPU run of S steps, and the whole timed
+    trajectory against the exact-arithmetic kernels."""
+    import mpdata_amd.configs as K
+    import oracle_lib as O
+
+    shape = DEFORM_SHAPE
+    nx, ny, nz = shape
+    cells = nx * ny * nz
+    g = M.Grid(*shape, M.PERIODIC, iord=3, nonosc=True)
+    g.set_arith(args.arith)
+    K.deform(g, shape, seed=3)
+    c0 = g.field()
+    mass0 = c0.sum()
+    g.autotune(1)  # each kernel's tile shape and z-chunk on this grid (state kept)
+    tuned = tuned_config(g)
+    print("bench: headline tuned", tuned, file=sys.stderr, flush=True)
     g.step(args.warmup)
     g.synchronize()
     t0 = time.perf_counter()
     ev_ms = g.step_timed(args.steps)
     wall = time.perf_counter() - t0
-    kms = g.kernel_times(8)
-    g.close()
+    kms = g.kernel_times(2)
+    mass, cmin, _ = g.stats()
     ms_per_step = ev_ms / args.steps
     value = cells * args.steps / (ev_ms * 1e-3)
-
-    # kernels of one step, in launch order, with their own input/output bytes per cell
-    if args.path == 1:
-        names = ["staged"] * len(kms)
-        own = [None] * len(kms)
-    else:
-        names, own = [], []
-        for p in range(1, args.iord):
-            names += [f"k_f1 (pass {p})", f"k_f2 (pass {p})"]
-            last = p == args.iord - 1
-            own += [64 if p == 1 else 56, 48 if last else 72]
-    kern_ms = sum(kms)
-    achieved = cells * BYTES_PER_CELL / (kern_ms * 1e-3) / 1e9
-    kernels = [{"name": nm, "avg_ms": t, "io_bytes_per_cell": b,
-                "io_GB/s": (cells * b / (t * 1e-3) / 1e9) if b else None}
-               for nm, t, b in zip(names, kms, own)]
-    dominant = max(kernels, key=lambda k: k["avg_ms"])
-
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_rotation{n}_iord{args.iord}.json")
-    if os.path.exists(tfile) and args.path == 0 and args.arith == 1:
-        t = json.load(open(tfile))
-        # counters measured with the same tile shape and z-chunk length only
-        if t.get("config") == tuned:
-            traffic = t.get("hbm_bytes_per_step")
-
+    kernels = step_kernels(M, 3, kms)
+    tfile = os.path.join(ROOT, "profiles", "traffic_deform512_iord3.json")
+    roof = roofline(cells, kms, kernels, tfile if args.arith == 1 else None, tuned)
+    # the whole trajectory (warm-up, timed and kernel-timing steps; the autotuner restores
+    # the state it found) again in exact arithmetic, which equals the oracle's bit for bit
+    taken = args.warmup + args.steps + 2
+    fast = g.field()
+    g.set_arith(M.ARITH_EXACT)
+    g.set_field(c0)
+    g.step(taken)
+    err_run = relerr(fast, g.field())
+    del fast
     line = {
         "metric": "cell-updates/s at <=1e-12 rel err (FP64 MPDATA step, 1 GPU)",
         "value": value,
@@ -197,79 +286,106 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (analytic rotation wind and Gaussian blob, no RNG)",
-        "config": {"workload": f"ring3d: periodic {n}^3 solid-body rotation about (1,1,1), "
-                               f"max Courant 0.4, Gaussian blob sigma 8, iord={args.iord} "
-                               "nonoscillatory", "grid": [n, n, n], "iord": args.iord,
-                   "nonoscillatory": True, "boundary": "periodic",
-                   "path": ["auto(fused)", "staged", "fused"][args.path],
-                   "arith": ["exact", "fast"][args.arith], "autotuned": tuned},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_step": cells * BYTES_PER_CELL,
-                     "step_kernel_ms": kern_ms, "dominant_kernel": dominant["name"],
-                     "kernels": kernels},
+        "data": "synthetic (seeded splitmix64 deformational wind and Gaussians, generated "
+                "on the device)",
+        "config": {"workload": "large single instance: periodic 512x512x256 FP64, "
+                               "divergence-free deformational wind (4 Fourier modes per "
+                               "potential component, seed 3, max Courant 0.5), 32 seeded "
+                               "Gaussians sigma 4..16, iord=3 nonoscillatory",
+                   "grid": list(shape), "iord": 3, "nonoscillatory": True,
+                   "boundary": "periodic", "arith": ["exact", "fast"][args.arith],
+                   "autotuned": tuned},
+        "roofline": roof,
         "wall_s_timed": wall,
+        "mass_rel_drift": abs(mass - mass0) / mass0, "min": cmin,
     }
-
-    import oracle_lib as O
-
-    g = make_grid()
-    g.step(1)
-    got1 = g.field()
-    want1 = O.per_step(n, n, n, case["U"], case["V"], case["W"], args.iord, True, case["c"])
-    par = {"steps_1": {"rel_err": relerr(got1, want1)}}
-    if not args.no_cpu:
-        ks = max(1, args.cpu_steps)
-        g.step(ks - 1)
-        gotk = g.field()
-        mass, cmin, cmax_ = g.stats()
+    par = {f"steps_{taken}_vs_exact_kernels": {"rel_err": err_run}}
+    if not args.no_cpu and args.cpu_steps > 0:
+        ks = args.cpu_steps
+        g.set_arith(args.arith)
+        g.set_field(c0)
+        g.step(ks)
+        got = g.field()
+        U, V, W = g.get_courant()
+        g.close()
         threads = O.get_threads()
         O.set_threads(1)
         t0 = time.perf_counter()
-        wantk = O.per_run(n, n, n, case["U"], case["V"], case["W"], args.iord, True,
-                          case["c"], ks)
+        want = O.per_run(*shape, U, V, W, 3, True, c0, ks)
         t_cpu = time.perf_counter() - t0
-        # the same restatement over z planes on all the threads OpenMP is given
+        print(f"bench: oracle 1 thread {t_cpu:.1f} s for {ks} steps", file=sys.stderr,
+              flush=True)
         nthr = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or threads
         O.set_threads(nthr)
         t0 = time.perf_counter()
-        wantp = O.per_run(n, n, n, case["U"], case["V"], case["W"], args.iord, True,
-                          case["c"], ks)
+        wantp = O.per_run(*shape, U, V, W, 3, True, c0, ks)
         t_par = time.perf_counter() - t0
         O.set_threads(threads)
-        par[f"steps_{ks}"] = {"rel_err": relerr(gotk, wantk),
-                              "mass_rel_drift": abs(mass - case["c"].sum()) / case["c"].sum(),
-                              "min": cmin}
+        del U, V, W
+        par[f"steps_{ks}_vs_oracle"] = {"rel_err": relerr(got, want)}
         line["cpu_baseline"] = {
             "value": cells * ks / t_cpu, "unit": "cell-updates/s", "cores": 1,
             "kind": "port",
             "sample": f"oracle/cpu_ref.c (FP64 restatement of the reference step with the "
-                      f"same periodic + limiter options, 1 thread, {host_cpu_model()}) over "
-                      f"{ks} steps of the same {n}^3 iord={args.iord} workload",
+                      f"same periodic + limiter options, 1 thread of {host_cpu_model()}) "
+                      f"over the first {ks} steps of the same 512x512x256 iord=3 workload",
+            "host": host_cores(),
         }
         line["cpu_parallel"] = {
             "value": cells * ks / t_par, "unit": "cell-updates/s", "cores": nthr,
             "kind": "port",
             "sample": f"the same oracle loops with OpenMP over z planes ({nthr} threads, this "
                       f"build's parallel CPU, not the reference's), same {ks} steps",
-            "bitwise_equal_to_1_thread": bool(np.array_equal(wantp, wantk)),
+            "bitwise_equal_to_1_thread": bool(np.array_equal(wantp, want)),
         }
-    g.close()
+        del want, wantp, got
+        cal = cpu_calibration(O)
+        if cal:
+            line["cpu_calibration"] = cal
+    else:
+        g.close()
     line["parity"] = par
-    if args.ensemble:
-        line["ensemble"] = ensemble_leg(M, args.ensemble)
-    if args.large:
-        line["large"] = large_leg(M, args.large, args.iord, args.arith)
-    if args.ring2d:
-        line["ring2d"] = ring2d_leg(M, args.ring2d)
-    if args.dropin:
-        line["dropin"] = dropin_leg(M, args.dropin)
-    if args.deform:
-        line["deform"] = deform_leg(M, oracle_steps=args.deform_oracle_steps)
-    if args.adjoint:
-        line["adjoint"] = adjoint_leg(M, args.adjoint, nsteps=args.adjoint_steps)
-    print(json.dumps(line))
+    return line
+
+
+def ring3d_leg(M, n, arith, steps=128):
+    """The cache-resident rotation case: n^3 periodic, solid-body rotation about (1,1,1)
+    at max Courant 0.4, Gaussian blob sigma 8, iord 2 nonoscillatory; autotuned, 64
+    warm-up and `steps` timed steps, the step's roofline, and the run against the
+    oracle's (16 threads) after 1 and `steps` steps."""
+    import oracle_lib as O
+
+    case = rotation_case(n)
+    cells = n ** 3
+    g = M.Grid(n, n, n, M.PERIODIC, iord=2, nonosc=True)
+    g.set_arith(arith)
+    g.set_courant(case["U"], case["V"], case["W"])
+    g.set_field(case["c"])
+    g.autotune(2)
+    tuned = tuned_config(g)
+    g.step(64)
+    ev_ms = g.step_timed(steps)
+    kms = g.kernel_times(8)
+    ms = ev_ms / steps
+    tfile = os.path.join(ROOT, "profiles", f"traffic_rotation{n}_iord2.json")
+    roof = roofline(cells, kms, step_kernels(M, 2, kms), tfile if arith == 1 else None, tuned)
+    # parity on the timed grid (its tuned launch shapes), from the initial field
+    g.set_field(case["c"])
+    g.step(1)
+    got1 = g.field()
+    g.step(steps - 1)
+    gotk = g.field()
+    mass, cmin, _ = g.stats()
+    g.close()
+    want1 = O.per_step(n, n, n, case["U"], case["V"], case["W"], 2, True, case["c"])
+    wantk = O.per_run(n, n, n, case["U"], case["V"], case["W"], 2, True, case["c"], steps)
+    return {"workload": f"ring3d: periodic {n}^3 solid-body rotation about (1,1,1), max "
+                        "Courant 0.4, Gaussian blob sigma 8, iord=2 nonoscillatory "
+                        "(fits the Infinity Cache)",
+            "autotuned": tuned, "ms_per_step": ms, "cell_updates_per_s": cells / (ms * 1e-3),
+            "roofline": roof, "rel_err_1_step_vs_oracle": relerr(got1, want1),
+            f"rel_err_{steps}_steps_vs_oracle": relerr(gotk, wantk),
+            "mass_rel_drift": abs(mass - case["c"].sum()) / case["c"].sum(), "min": cmin}
 
 
 def ring2d_leg(M, n, nsteps=1024):
@@ -321,63 +437,6 @@ def dropin_leg(M, n, nsteps=64, order=2):
                         f"velocities, order {order}, device-resident (the msmolar path)",
             "ms_per_step": ms, "cell_updates_per_s": (n - 2) ** 3 / (ms * 1e-3),
             "rel_err_1_step_vs_oracle": relerr(got, want)}
-
-
-def deform_leg(M, nsteps=200, oracle_steps=1):
-    """Config 3: 512 x 512 x 256 periodic, deformational wind from 4 seeded Fourier modes
-    per potential component (max Courant 0.5), 32 seeded Gaussians, iord 3
-    nonoscillatory, 200 timed steps; oracle_steps steps against the oracle at full size
-    first (one by default: the oracle runs ~1 s per step on 16 cores), and the whole
-    trajectory against the exact-arithmetic kernels afterwards."""
-    import mpdata_amd.configs as K
-    import oracle_lib as O
-
-    shape = (512, 512, 256)
-    cells = shape[0] * shape[1] * shape[2]
-    g = M.Grid(*shape, M.PERIODIC, iord=3, nonosc=True)
-    K.deform(g, shape, seed=3)
-    U, V, W = g.get_courant()
-    c0 = g.field()
-    mass0 = c0.sum()
-    g.step(oracle_steps)
-    got = g.field()
-    t0 = time.perf_counter()
-    want = c0
-    for done in range(0, oracle_steps, 25):  # in chunks, a progress line after each
-        want = O.per_run(*shape, U, V, W, 3, True, want, min(25, oracle_steps - done))
-        if oracle_steps > 1:
-            print(f"deform: oracle step {min(done + 25, oracle_steps)} of {oracle_steps}",
-                  file=sys.stderr, flush=True)
-    t_cpu = (time.perf_counter() - t0) / oracle_steps
-    err = relerr(got, want)
-    del U, V, W, got, want
-    g.autotune(1)
-    g.step(2)
-    ms = g.step_timed(nsteps) / nsteps
-    kms = g.kernel_times(2)
-    mass, cmin, _ = g.stats()
-    tuned = tuned_config(g)
-    # the whole trajectory (oracle_steps + 2 + nsteps + 2 steps; the autotuner restores
-    # the state it found) again in exact arithmetic, which equals the oracle's bit for bit
-    taken = oracle_steps + 2 + nsteps + 2
-    fast = g.field()
-    g.set_arith(M.ARITH_EXACT)
-    g.set_field(c0)
-    g.step(taken)
-    err_run = relerr(fast, g.field())
-    del fast, c0
-    g.close()
-    ach = cells * BYTES_PER_CELL / (sum(kms) * 1e-3) / 1e9
-    return {"workload": "periodic 512x512x256 deformational wind (seed 3, 4 modes per "
-                        "component, max Courant 0.5), 32 Gaussians, iord 3 nonoscillatory",
-            "autotuned": tuned, "steps": nsteps, "ms_per_step": ms,
-            "cell_updates_per_s": cells / (ms * 1e-3), "kernel_ms": kms,
-            "achieved_GB/s": ach, "frac_of_hbm_peak": ach / HBM_PEAK_GBS,
-            f"rel_err_{oracle_steps}_step{'s' if oracle_steps > 1 else ''}_vs_oracle": err,
-            f"rel_err_{taken}_steps_vs_exact": err_run,
-            "oracle_cell_updates_per_s": cells / t_cpu,
-            "oracle_threads": O.get_threads(),
-            "mass_rel_drift": abs(mass - mass0) / mass0, "min": cmin}
 
 
 def ensemble_leg(M, n):
@@ -473,40 +532,37 @@ def large_leg(M, n, iord, arith):
             "rel_err_1_step_vs_oracle": err1, f"rel_err_{taken}_steps_vs_exact": err_run}
 
 
-def adjoint_leg(M, n, nsteps=512, every=16, K=32, nrec=64):
-    """Config 5: n^3 periodic, deformational wind (seed 5), 32 Gaussian blobs, emission of
-    8 seeded positive Gaussians added every step; 64 seeded receptors sampled every 16
-    steps against a twin run whose emission amplitudes are scaled by (1 + 0.2 xi);
-    gradient of J = 1/2 sum (c - obs)^2 w.r.t. c^0 and the emission, checkpoints every 32
-    steps, iord 2 with the limiter.  The gradient run keeps the records of as many final
-    segments as the device memory holds and recomputes the others; it is repeated with
-    only the last segment kept (pure checkpointing, same gradient bit for bit).  The
-    adjoint's cost is the gradient run's time beyond a plain exact forward of the same
-    steps, in forward steps: (forward sweep + backward - plain forward) / plain forward.
-    Also reported: central
-    differences along a direction scaled by c^0 (limiter on, and the same with the limiter
-    off, where the step is differentiable), and the gradient against the CPU oracle's at
-    32^3 x 64 steps."""
+def adjoint_leg(M, n, nsteps=512, every=16, K=32, nrec=64, checks=True):
+    """Source inversion: n^3 periodic, deformational wind and 32 Gaussians (seed 5),
+    emission of 8 seeded positive Gaussians added every step; 64 seeded receptors sampled
+    every 16 steps against a twin run whose emission amplitudes are scaled by
+    (1 + 0.2 xi); gradient of J = 1/2 sum (c - obs)^2 w.r.t. c^0 and the emission,
+    checkpoints every 32 steps, iord 2 with the limiter.  The gradient run keeps the
+    records of as many final segments as the device memory holds and recomputes the
+    others; it is repeated with only the last segment kept (pure checkpointing, same
+    gradient bit for bit).  The adjoint's cost is the gradient run's time beyond a plain
+    exact forward of the same steps, in forward steps:
+    (forward sweep + backward - plain forward) / plain forward.
+
+    With `checks`: at full size, the dot-product test of one GPU adjoint step (a state of
+    the run) against the oracle's tangent-linear model in both controls; central
+    differences of J along 4 seeded c^0 directions and one emission direction at eps
+    1e-6; and eps sweeps 1e-4 .. 1e-9 along one c^0 direction with the limiter off and
+    on.  J is piecewise smooth in either case: the limiter's max/min selections, and even
+    without them the donor fluxes' upwinding on the sign of the c-dependent antidiffusive
+    velocity, put kinks in it, which a difference quotient straddles when eps is too
+    large; the sweeps show where the quotient meets the adjoint.  Also the gradient
+    against the CPU oracle's at 32^3 x 64 steps."""
     import mpdata_amd.configs as K_
     import oracle_lib as O
 
-    rng = K_.SplitMix64(5005)
-    rec = np.array(sorted({rng.randint(n ** 3) for _ in range(nrec * 2)})[:nrec], np.int32)
-    xi = [rng.uniform(-1.0, 1.0) for _ in range(8)]
-
-    def fresh(nonosc=True, scale=None, size=n):
-        g = M.Grid(size, size, size, M.PERIODIC, iord=2, nonosc=nonosc)
-        cen, sig, amp = K_.deform(g, (size, size, size), seed=5, emission_blobs=8)
-        if scale is not None:
-            K_.generate_blobs(g, cen, sig, [a * f for a, f in zip(amp, scale)],
-                              periodic=True, target=K_.TARGET_EMISSION)
-        return g
-
-    twin = fresh(scale=[1.0 + 0.2 * x for x in xi])
+    rec, factors = K_.inversion_receptors(n ** 3, nrec=nrec)
+    twin = K_.inversion_grid(n, factors=factors)
     obs = twin.sample_receptors(nsteps, rec, every)
     twin.close()
-    g = fresh()
+    g = K_.inversion_grid(n)
     c0 = g.field()
+    q0 = g.emission()
     # the gradient run's steps are exact-arithmetic fused steps: their launch shapes are
     # tuned in that arithmetic (the plain forward it is compared with uses the same)
     g.set_arith(M.ARITH_EXACT)
@@ -524,72 +580,109 @@ def adjoint_leg(M, n, nsteps=512, every=16, K=32, nrec=64):
     same = bool(J_c == J and np.array_equal(g0_c, g0) and np.array_equal(gq_c, gq))
     # a plain forward of the same steps in the driver's (exact) arithmetic
     g.set_field(c0)
-    g.set_arith(M.ARITH_EXACT)
     tp = g.step_timed(nsteps)
+    print(f"bench: adjoint {(tf + tb - tp) / tp:.3f}x, checkpointing only "
+          f"{(tf_c + tb_c - tp) / tp:.3f}x", file=sys.stderr, flush=True)
+    out = {"workload": f"periodic {n}^3 deformational wind (seed 5), 8-source emission, "
+                       f"{nsteps} steps, {nrec} receptors every {every} steps, checkpoints "
+                       f"every {K}, iord 2 nonoscillatory",
+           "autotuned": tuned, "J": J, "forward_ms_per_step": tp / nsteps,
+           "forward_sweep_ms_per_step": tf / nsteps, "backward_ms_per_step": tb / nsteps,
+           "adjoint_over_forward": (tf + tb - tp) / tp,
+           "kept_segments": kept, "segments": -(-nsteps // K),
+           "recompute_ms_per_step": tr / nsteps,
+           "adjoint_steps_only_over_forward": (tb - tr) / tp,
+           "checkpointing_only": {
+               "forward_sweep_ms_per_step": tf_c / nsteps,
+               "backward_ms_per_step": tb_c / nsteps,
+               "recompute_ms_per_step": tr_c / nsteps,
+               "adjoint_over_forward": (tf_c + tb_c - tp) / tp,
+               "same_gradient_bitwise": same},
+           "grad_norms": {"c0": float(np.linalg.norm(g0)), "q": float(np.linalg.norm(gq))}}
+    if checks:
+        out.update(adjoint_checks(M, O, g, n, nsteps, every, K, rec, obs, c0, q0, g0, gq))
     g.close()
-    d = np.random.default_rng(5).standard_normal(c0.size) * c0
-
-    def fd_check(nonosc, grad, epss):
-        res = {}
-        for e in epss:
-            vals = []
-            for sgn in (1, -1):
-                gx = fresh(nonosc)
-                gx.set_field(c0 + sgn * e * d)
-                vals.append(gx.receptor_gradient(nsteps, K, rec, every, obs, grads=False)[0])
-                gx.close()
-            fd, an = (vals[0] - vals[1]) / (2 * e), float(grad @ d)
-            res[f"{e:g}"] = {"fd": fd, "adjoint": an,
-                             "rel_diff": abs(fd - an) / max(abs(an), 1e-300)}
-        return res
-
-    fd_lim = fd_check(True, g0, (1e-6, 1e-8))
-    g = fresh(nonosc=False)
-    _, g0u, _, _ = g.receptor_gradient(nsteps, K, rec, every, obs)
-    g.close()
-    fd_unlim = fd_check(False, g0u, (1e-6,))
 
     # CPU reference gradient at 32^3 x 64 steps (same generators, limiter on)
-    m, ms, mrec = 32, 64, np.array(sorted({rng.randint(32 ** 3) for _ in range(40)})[:16], np.int32)
-    gs = fresh(size=m)
+    m, ms = 32, 64
+    mrec, mfac = K_.inversion_receptors(m ** 3, nrec=16)
+    gs = K_.inversion_grid(m)
     U, V, W = gs.get_courant()
-    cs = gs.field()
-    # the emission increment is read back through a zero-wind step: c + q - c
-    gz = M.Grid(m, m, m, M.PERIODIC, iord=2, nonosc=True)
-    z = np.zeros(m ** 3)
-    gz.set_courant(z, z, z)
-    gz.set_field(z)
-    K_.deform(gz, (m, m, m), seed=5, emission_blobs=8)
-    gz.set_courant(z, z, z)
-    gz.set_field(z)
-    gz.step(1)
-    qs = gz.field()
-    gz.close()
-    obs_s = np.zeros((ms // 16, mrec.size))
+    cs, qs = gs.field(), gs.emission()
+    tw = K_.inversion_grid(m, factors=mfac)
+    obs_s = tw.sample_receptors(ms, mrec, 16)
+    tw.close()
     Jg, g0g, gqg, _ = gs.receptor_gradient(ms, 16, mrec, 16, obs_s)
     gs.close()
     Jc, g0c, gqc = O.receptor_loss_and_gradient(m, m, m, U, V, W, True, cs, qs, ms, mrec, 16,
                                                 obs_s)
-    cpu = {"grid": [m, m, m], "steps": ms, "J_rel_diff": abs(Jg - Jc) / Jc,
-           "grad_c0_rel_diff": float(np.abs(g0g - g0c).max() / np.abs(g0c).max()),
-           "grad_q_rel_diff": float(np.abs(gqg - gqc).max() / np.abs(gqc).max())}
-    return {"workload": f"periodic {n}^3 deformational wind (seed 5), 8-source emission, "
-                        f"{nsteps} steps, {nrec} receptors every {every} steps, checkpoints "
-                        f"every {K}, iord 2 nonoscillatory",
-            "autotuned": tuned, "J": J, "forward_ms_per_step": tp / nsteps,
-            "forward_sweep_ms_per_step": tf / nsteps, "backward_ms_per_step": tb / nsteps,
-            "adjoint_over_forward": (tf + tb - tp) / tp,
-            "kept_segments": kept, "segments": -(-nsteps // K),
-            "recompute_ms_per_step": tr / nsteps,
-            "adjoint_steps_only_over_forward": (tb - tr) / tp,
-            "checkpointing_only": {
-                "forward_sweep_ms_per_step": tf_c / nsteps,
-                "backward_ms_per_step": tb_c / nsteps,
-                "recompute_ms_per_step": tr_c / nsteps,
-                "adjoint_over_forward": (tf_c + tb_c - tp) / tp,
-                "same_gradient_bitwise": same},
-            "fd_limiter_on": fd_lim, "fd_limiter_off": fd_unlim, "cpu_reference": cpu,
-            "grad_norms": {"c0": float(np.linalg.norm(g0)), "q": float(np.linalg.norm(gq))}}
+    out["cpu_reference"] = {
+        "grid": [m, m, m], "steps": ms, "J_rel_diff": abs(Jg - Jc) / Jc,
+        "grad_c0_rel_diff": float(np.abs(g0g - g0c).max() / np.abs(g0c).max()),
+        "grad_q_rel_diff": float(np.abs(gqg - gqc).max() / np.abs(gqc).max())}
+    return out
+
+
+def adjoint_checks(M, O, g, n, nsteps, every, K, rec, obs, c0, q0, g0, gq):
+    """Full-size checks of the source-inversion gradient (see adjoint_leg)."""
+    res = {}
+    # dot-product test of one adjoint step at a state of the run (8 steps in), both
+    # controls: <TLM(dc, dq), lam> against <dc, ADJ lam> + <dq, lq>
+    g.set_field(c0)
+    g.step(8)
+    c = g.field()
+    U, V, W = g.get_courant()
+    rng = np.random.default_rng(2024)
+    lam = rng.standard_normal(c.size)
+    lq = np.zeros(c.size)
+    lin = g.adjoint_step(c, lam, lq)
+    dc = rng.standard_normal(c.size) * c
+    dq = rng.standard_normal(c.size) * q0
+    t = O.tlm_step(n, n, n, U, V, W, True, c, dc, dq)
+    lhs, rhs = float(t @ lam), float(dc @ lin) + float(dq @ lq)
+    res["dot_product_test"] = {"state": "c^8 of the run", "lhs_tlm": lhs, "rhs_adjoint": rhs,
+                               "rel_diff": abs(lhs - rhs) / abs(lhs)}
+    del t, lin, lam, lq, U, V, W, c
+    print(f"bench: adjoint dot-product test {abs(lhs - rhs) / abs(lhs):.2e}", file=sys.stderr,
+          flush=True)
+
+    def loss(grid, cc, qq):
+        grid.set_field(cc)
+        if qq is not None:
+            grid.set_emission(qq)
+        return grid.receptor_gradient(nsteps, K, rec, every, obs, grads=False)[0]
+
+    def central(grid, d_c, d_q, eps, grad_c, grad_q):
+        if d_c is not None:
+            jp, jm = loss(grid, c0 + eps * d_c, None), loss(grid, c0 - eps * d_c, None)
+            an = float(grad_c @ d_c)
+        else:
+            jp, jm = loss(grid, c0, q0 + eps * d_q), loss(grid, c0, q0 - eps * d_q)
+            grid.set_emission(q0)
+            an = float(grad_q @ d_q)
+        fd = (jp - jm) / (2 * eps)
+        return {"fd": fd, "adjoint": an, "rel_diff": abs(fd - an) / max(abs(an), 1e-300)}
+
+    dirs = np.random.default_rng(5)
+    # 4 seeded c^0 directions and one emission direction, scaled by the fields themselves
+    # (relative perturbations: the fields span many orders of magnitude), eps 1e-6
+    res["fd_eps_1e-6"] = {
+        "c0_directions": [central(g, dirs.standard_normal(c0.size) * c0, None, 1e-6, g0, gq)
+                          for _ in range(4)],
+        "q_direction": central(g, None, dirs.standard_normal(c0.size) * q0, 1e-6, g0, gq)}
+    d = dirs.standard_normal(c0.size) * c0
+    epss = (1e-4, 1e-5, 1e-6, 1e-7, 1e-8, 1e-9)
+    res["eps_sweep_limiter_on"] = {f"{e:g}": central(g, d, None, e, g0, gq) for e in epss}
+    # the same with the limiter off: its own gradient
+    gu = M.Grid(n, n, n, M.PERIODIC, iord=2, nonosc=False)
+    gu.set_arith(M.ARITH_EXACT)
+    gu.set_courant(*g.get_courant())
+    gu.set_emission(q0)
+    gu.set_field(c0)
+    _, g0u, gqu, _ = gu.receptor_gradient(nsteps, K, rec, every, obs)
+    res["eps_sweep_limiter_off"] = {f"{e:g}": central(gu, d, None, e, g0u, gqu) for e in epss}
+    gu.close()
+    return res
 
 
 if __name__ == "__main__":
