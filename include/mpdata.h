@@ -134,6 +134,22 @@ int mpd_set_zchunk(mpd_grid g, int zc);
  * PERIODIC grids with iord >= 2 on the fused path only (MPD_ERR_UNSUPPORTED otherwise). */
 int mpd_autotune(mpd_grid g, int nsteps, int* tile_x, int* tile_y, int* zc);
 
+/* One PERIODIC step c_out = A(c_in) (+ q) on caller-owned DEVICE arrays that allocates
+ * no device memory: no counterpart in the reference.  For callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It computes what mpd_step computes on a PERIODIC grid with the same Courant numbers
+ * U, V, W (periodic face layout), iord (2..64), limiter option and emission q (NULL:
+ * none), in fast arithmetic on the fused path, and writes c_out (distinct from c_in).
+ * The fused path's scratch is a fixed device array of 256 MiB per device, held until the
+ * process ends: at most 8 Mi cells at iord 2, 4 Mi cells at iord >= 3 (MPD_ERR_ARG
+ * beyond); z-planes of at most 2^29 cells.  Kernels are enqueued on `stream` (a
+ * hipStream_t; NULL = the null stream) and the call returns without waiting.  The first
+ * call on a device looks up that device's scratch address.  Every call on a device
+ * shares its scratch array: calls that overlap on one device race. */
+int mpd_step_periodic_noalloc(int nx, int ny, int nz, int iord, int nonosc, const double* U,
+                              const double* V, const double* W, const double* q,
+                              const double* c_in, double* c_out, void* stream);
+
 /* ------------------------------------------------------------ generators --- */
 #define MPD_MAX_MODES 8
 #define MPD_MAX_BLOBS 64

@@ -73,6 +73,7 @@ SIGNATURES = {
     "mpd_set_emission": (C.c_int, [C.c_void_p, PD]),
     "mpd_get_emission": (C.c_int, [C.c_void_p, PD]),
     "mpd_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpd_step_periodic_noalloc": (C.c_int, [C.c_int] * 5 + [C.c_void_p] * 7),
     "mpd_synchronize": (C.c_int, [C.c_void_p]),
     "mpd_field_device_ptr": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "mpd_step_timed": (C.c_int, [C.c_void_p, C.c_int, PD]),

@@ -839,7 +839,7 @@ int MPD_CAT(fused_store_step, MPD_ARITH)(mpd_grid_s* g, const double* cin, doubl
 
 int MPD_CAT(periodic_fused_step, MPD_ARITH)(mpd_grid_s* g)
 {
-    hipStream_t s = stream();
+    hipStream_t s = launch_stream(g);
     int rc;
     if ((rc = ensure_fused_scratch(g))) return rc;
     int tx = g->tile_x ? g->tile_x : 32, ty = g->tile_y ? g->tile_y : 8;

@@ -59,6 +59,11 @@ struct mpd_grid_s {
                    zc2 == o.zc2 && arith == o.arith && path == o.path && q == o.q;
         }
     };
+    // launch stream of the fused step: the library's stream unless a caller supplies its
+    // own (the allocation-free entry point launches on the caller's stream)
+    bool caller_stream_set = false;
+    hipStream_t caller_stream = nullptr;
+
     bool use_graph = false;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     GraphKey gkey[2] = {};
@@ -89,5 +94,10 @@ int field_stats(mpd_grid_s* g, double* mass, double* cmin, double* cmax);
 int sentinel_check_launch(mpd_grid_s* g);
 // Records a timing event after a kernel launch when g->trace is set.
 void trace_mark(mpd_grid_s* g);
+// The stream a grid's fused step launches on.
+inline hipStream_t launch_stream(const mpd_grid_s* g)
+{
+    return g->caller_stream_set ? g->caller_stream : stream();
+}
 
 }  // namespace mpd
