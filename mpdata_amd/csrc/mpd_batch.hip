@@ -100,6 +100,8 @@ int build_tables(mpd_batch_s* b)
         mpd_grid_s* g = b->g[i];
         if ((rc = ensure_fused_scratch(g))) return rc;
         FusedArgs geo{};
+        // streaming stores when the whole batch's arrays exceed the caches
+        geo.nt = fused_streaming((size_t)b->cells);
         geo.nx = g->nx;
         geo.ny = g->ny;
         geo.nz = g->nz;
@@ -198,11 +200,12 @@ int launch_step(mpd_batch_s* b, int par)
         const FusedArgs* t2 = b->dtab + table_index(b, par, p, 1, 0);
         int tx2, ty2, zc2;
         shape_of(b, true, tx2, ty2, zc2);
+        const bool nt = fused_streaming((size_t)b->cells);
         if ((rc = launch(b->tx, b->ty, true, p == 1, t1, b->ditem0, b->ninst, b->nitems,
-                         grid_for(b, true), s)))
+                         grid_for(b, true), nt, s)))
             return rc;
         if ((rc = launch(tx2, ty2, false, false, t2, b->ditem0 + b->ninst, b->ninst,
-                         b->nitems2, grid_for(b, false), s)))
+                         b->nitems2, grid_for(b, false), nt, s)))
             return rc;
     }
     return MPD_OK;

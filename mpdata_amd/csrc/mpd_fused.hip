@@ -44,6 +44,17 @@ namespace {
 
 using namespace tile;
 
+// A store of the fused kernels: streaming in the NT instantiations (chosen at launch from
+// FusedArgs::nt; a run-time branch on the flag measured no gain at all).
+template <bool NT>
+__device__ __forceinline__ void put(double* plane, unsigned colb, double v)
+{
+    if constexpr (NT)
+        st_nt(plane, colb, v);
+    else
+        st(plane, colb, v);
+}
+
 // tile (bx, by) and z-chunk bz of work item b of an instance
 __device__ __forceinline__ void tile_of(const FusedArgs& a, int b, int& bx, int& by, int& bz)
 {
@@ -97,7 +108,7 @@ __device__ __forceinline__ int item_of(const Work& w, int it)
 // follow its write), c1, U and V three planes each (plane p is written while slower waves
 // may still read planes p-2 and p-1).  The own-column values of the planes below and
 // above, and the W neighbours of plane k-1, are carried in registers.
-template <int TX, int TY, bool DONOR>
+template <int TX, int TY, bool DONOR, bool NT>
 __device__ __forceinline__ void f1_tile(const FusedArgs& a, int item)
 {
     using G = Geo<TX, TY>;
@@ -203,7 +214,7 @@ __device__ __forceinline__ void f1_tile(const FusedArgs& a, int item)
         if (DONOR) {
             c1p = donor(c0c, s_c0[L - 1], s_c0[L + 1], s_c0[L - RX], s_c0[L + RX], c0m, c0n,
                         s_u[s3][L - 1], un, s_v[s3][L - RX], vn, wm1, wn);
-            if (store && p >= z0 && p < z1) st(a.c1out + zp0.off, colb, c1p);
+            if (store && p >= z0 && p < z1) put<NT>(a.c1out + zp0.off, colb, c1p);
         } else {
             c1p = fld;
         }
@@ -245,14 +256,14 @@ __device__ __forceinline__ void f1_tile(const FusedArgs& a, int item)
                 double sv = 0.25 * (v1 + v2);
                 double sw = 0.25 * (w1 + w2);
                 double div = 0.25 * av * ((uk[L + 1] - uk[L - 1]) + (v1 - v2) + (w1 - w2));
-                st(a.Ax + zm1.off, colb,
+                put<NT>(a.Ax + zm1.off, colb,
                    antidiff(div, av, cr, c, sv, b1 - b2, b1 + b2, sw, e1 - e2, e1 + e2));
 #else
                 double sv = 0.25 * (vr + vru + vu + vcc);
                 double sw = 0.25 * (wr + wrf + wf + wc);
                 double div = 0.25 * av * (uk[L + 1] - uk[L - 1] + vr - vru - vu + vcc + wr - wrf -
                                           wf + wc);
-                st(a.Ax + zm1.off, colb,
+                put<NT>(a.Ax + zm1.off, colb,
                    antidiff(div, av, cr, c, sv, cru + cu - cd - crd, cru + cu + cd + crd, sw,
                             crf + cf - cb - crb, crf + cf + cb + crb));
 #endif
@@ -272,14 +283,14 @@ __device__ __forceinline__ void f1_tile(const FusedArgs& a, int item)
                 double su = 0.25 * (u1 + (uru + ur));
                 double sw = 0.25 * (w1 + w2);
                 double div = 0.25 * av * ((vk[L + RX] - vk[L - RX]) + (w1 - w2) + (u1 - (ulu + ul)));
-                st(a.Ay + zm1.off, colb,
+                put<NT>(a.Ay + zm1.off, colb,
                    antidiff(div, av, cu, c, su, b1 - b2, b1 + b2, sw, e1 - e2, e1 + e2));
 #else
                 double su = 0.25 * (uu + uru + ur + ucc);
                 double sw = 0.25 * (wu + wuf + wf + wc);
                 double div = 0.25 * av * (vk[L + RX] - vk[L - RX] + wu - wuf - wf + wc + uu - ulu -
                                           ul + ucc);
-                st(a.Ay + zm1.off, colb,
+                put<NT>(a.Ay + zm1.off, colb,
                    antidiff(div, av, cu, c, su, cru + cr - cl - clu, cru + cr + cl + clu, sw,
                             cuf + cf - cb - cub, cuf + cf + cb + cub));
 #endif
@@ -297,13 +308,13 @@ __device__ __forceinline__ void f1_tile(const FusedArgs& a, int item)
                 double su = 0.25 * (u1 + u2);
                 double sv = 0.25 * (v1 + v2);
                 double div = 0.25 * av * ((wn - wmm) + (u1 - u2) + (v1 - v2));
-                st(a.Az + zm1.off, colb,
+                put<NT>(a.Az + zm1.off, colb,
                    antidiff(div, av, cf, c, su, b1 - b2, b1 + b2, sv, e1 - e2, e1 + e2));
 #else
                 double su = 0.25 * (ur + urf + uf + ucc);
                 double sv = 0.25 * (vf + vuf + vu + vcc);
                 double div = 0.25 * av * (wn - wmm + uf - urf - ur + ucc + vf - vuf - vu + vcc);
-                st(a.Az + zm1.off, colb,
+                put<NT>(a.Az + zm1.off, colb,
                    antidiff(div, av, cf, c, su, crf + cr - cl - clf, crf + cr + cl + clf, sv,
                             cuf + cu - cd - cdf, cuf + cu + cd + cdf));
 #endif
@@ -343,7 +354,7 @@ __device__ __forceinline__ void f1_tile(const FusedArgs& a, int item)
 // later are carried in registers.  SAVE also stores the ratios and the bound codes of
 // every plane (the adjoint's forward record); it is a separate instantiation so the
 // plain step does not carry its registers.
-template <int TX, int TY, bool SAVE>
+template <int TX, int TY, bool SAVE, bool NT>
 __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
 {
     using G = Geo<TX, TY>;
@@ -529,9 +540,13 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
         ratios(L, cnc, cnm, cnn, cpc, cpm, cpn, axn, ayn, azm, azc, axl, ayd, buc, bdc,
                SAVE ? &code : nullptr);
         if (SAVE && store && p >= z0 && p < z1) {
-            st(a.bu + zp0.off, colb, buc);
-            st(a.bd + zp0.off, colb, bdc);
-            *(int*)((char*)(a.code + zp0.off) + colb / 2) = code;
+            put<NT>(a.bu + zp0.off, colb, buc);
+            put<NT>(a.bd + zp0.off, colb, bdc);
+            int* cp_ = (int*)((char*)(a.code + zp0.off) + colb / 2);
+            if constexpr (NT)
+                __builtin_nontemporal_store(code, cp_);
+            else
+                *cp_ = code;
         }
         if (h1)  // (the plane p - 1 values are dead: published above)
             ratios(HL, hcnc, hcnm, hcnn, hcpc, hcpm, hcpn, haxn, hayn, hazm, hazc,
@@ -557,11 +572,11 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
                              uxr, vyl, vyr, wzl, wzr);
             if (store) {
                 if (a.q) r = r + qk;
-                st(a.cout + zm1.off, colb, r);
+                put<NT>(a.cout + zm1.off, colb, r);
                 if (a.Lx) {
-                    st(a.Lx + zm1.off, colb, uxr);
-                    st(a.Ly + zm1.off, colb, vyr);
-                    st(a.Lz + zm1.off, colb, wzr);
+                    put<NT>(a.Lx + zm1.off, colb, uxr);
+                    put<NT>(a.Ly + zm1.off, colb, vyr);
+                    put<NT>(a.Lz + zm1.off, colb, wzr);
                 }
             }
         }
@@ -611,7 +626,7 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
 // at that bound and keeps its natural allocation (3 waves per SIMD); its SAVE variant is
 // held to the same 3 waves (168 registers), below which its natural allocation would
 // fall, for 4-wave tiles (8-wave tiles are held to one workgroup per CU by LDS anyway).
-template <int TX, int TY, bool DONOR>
+template <int TX, int TY, bool DONOR, bool NT>
 __global__ void __launch_bounds__((Geo<TX, TY>::NT)) __attribute__((amdgpu_waves_per_eu(4)))
 k_f1(FusedArgs a, Work w)
 {
@@ -624,13 +639,13 @@ k_f1(FusedArgs a, Work w)
             __builtin_memcpy(&x, (const FusedArgs*)((ConstArgs*)w.table + i), sizeof x);
             item -= item0[i];
         }
-        f1_tile<TX, TY, DONOR>(x, item);
+        f1_tile<TX, TY, DONOR, NT>(x, item);
         lds_barrier();  // the next item reuses the LDS planes
     }
 }
 
-template <int TX, int TY, bool SAVE>
-__global__ void __launch_bounds__((Geo<TX, TY>::NT)) __attribute__((amdgpu_waves_per_eu(SAVE && Geo<TX, TY>::NT <= 256 ? 3 : 1)))
+template <int TX, int TY, bool SAVE, bool NT>
+__global__ void __launch_bounds__((Geo<TX, TY>::NT)) __attribute__((amdgpu_waves_per_eu(Geo<TX, TY>::NT <= 256 ? 3 : 1)))
 k_f2(FusedArgs a, Work w)
 {
     for (int it = blockIdx.x; it < w.nitems; it += gridDim.x) {
@@ -642,7 +657,7 @@ k_f2(FusedArgs a, Work w)
             __builtin_memcpy(&x, (const FusedArgs*)((ConstArgs*)w.table + i), sizeof x);
             item -= item0[i];
         }
-        f2_tile<TX, TY, SAVE>(x, item);
+        f2_tile<TX, TY, SAVE, NT>(x, item);
         lds_barrier();
     }
 }
@@ -687,21 +702,31 @@ void launch_k(K kernel, dim3 grid, dim3 block, hipStream_t s, const hipEvent_t* 
         hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
 }
 
-template <int TX, int TY>
-int launch_pair(const FusedArgs& a, const Work& w, int grid, bool f1, bool donor, hipStream_t s,
-                const hipEvent_t* ev)
+template <int TX, int TY, bool NT>
+void launch_nt(const FusedArgs& a, const Work& w, int grid, bool f1, bool donor, hipStream_t s,
+               const hipEvent_t* ev)
 {
     using G = Geo<TX, TY>;
     if (f1) {
         if (donor)
-            launch_k(k_f1<TX, TY, true>, dim3(grid), dim3(G::NT), s, ev, a, w);
+            launch_k(k_f1<TX, TY, true, NT>, dim3(grid), dim3(G::NT), s, ev, a, w);
         else
-            launch_k(k_f1<TX, TY, false>, dim3(grid), dim3(G::NT), s, ev, a, w);
+            launch_k(k_f1<TX, TY, false, NT>, dim3(grid), dim3(G::NT), s, ev, a, w);
     } else if (a.bu) {
-        launch_k(k_f2<TX, TY, true>, dim3(grid), dim3(G::NT), s, ev, a, w);
+        launch_k(k_f2<TX, TY, true, NT>, dim3(grid), dim3(G::NT), s, ev, a, w);
     } else {
-        launch_k(k_f2<TX, TY, false>, dim3(grid), dim3(G::NT), s, ev, a, w);
+        launch_k(k_f2<TX, TY, false, NT>, dim3(grid), dim3(G::NT), s, ev, a, w);
     }
+}
+
+template <int TX, int TY>
+int launch_pair(const FusedArgs& a, const Work& w, int grid, bool f1, bool donor, hipStream_t s,
+                const hipEvent_t* ev)
+{
+    if (a.nt)
+        launch_nt<TX, TY, true>(a, w, grid, f1, donor, s, ev);
+    else
+        launch_nt<TX, TY, false>(a, w, grid, f1, donor, s, ev);
     MPD_HIP(hipGetLastError());
     return MPD_OK;
 }
@@ -776,10 +801,11 @@ int choose_zc(const mpd_grid_s* g, int tx, int ty)
 
 int MPD_CAT(fused_batch_launch, MPD_ARITH)(int tx, int ty, bool f1, bool donor,
                                            const FusedArgs* table, const int* item0, int ninst,
-                                           int nitems, int grid, hipStream_t s)
+                                           int nitems, int grid, bool nt, hipStream_t s)
 {
     Work w{table, item0, ninst, nitems, (nitems % 8 == 0 && grid % 8 == 0) ? 1 : 0};
     FusedArgs none{};
+    none.nt = nt;
     return launch_any(none, w, grid, tx, ty, f1, donor, s);
 }
 
@@ -789,9 +815,9 @@ int MPD_CAT(fused_resident_blocks, MPD_ARITH)(int tx, int ty, bool f1)
     hipError_t e = hipErrorInvalidValue;
 #define MPD_TILE(X, Y)                                                                      \
     if (tx == X && ty == Y)                                                                 \
-        e = f1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_f1<X, Y, true>,         \
+        e = f1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_f1<X, Y, true, false>,       \
                                                              Geo<X, Y>::NT, 0)              \
-               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_f2<X, Y, false>,        \
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_f2<X, Y, false, false>,      \
                                                              Geo<X, Y>::NT, 0);
     MPD_FUSED_TILES(MPD_TILE)
 #undef MPD_TILE
@@ -811,6 +837,7 @@ int MPD_CAT(fused_store_step, MPD_ARITH)(mpd_grid_s* g, const double* cin, doubl
     a.nz = g->nz;
     a.np = (long long)g->nx * g->ny;
     a.zc = g->zc ? g->zc : choose_zc(g, tx, ty);
+    a.nt = fused_streaming(g->ncell);
     a.cin = cin;
     a.U = g->face[0].p;
     a.V = g->face[1].p;
@@ -848,6 +875,7 @@ int MPD_CAT(periodic_fused_step, MPD_ARITH)(mpd_grid_s* g)
     a.ny = g->ny;
     a.nz = g->nz;
     a.np = (long long)g->nx * g->ny;
+    a.nt = fused_streaming(g->ncell);
     const int zc1 = g->zc ? g->zc : choose_zc(g, tx, ty);
     // F2's launch shape: its own when set, else F1's
     const int tx2 = g->tile2_x ? g->tile2_x : tx, ty2 = g->tile2_x ? g->tile2_y : ty;

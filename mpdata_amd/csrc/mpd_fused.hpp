@@ -32,7 +32,15 @@ struct FusedArgs {
     // field 0 = c^n, 1 = pass input; ties go to the later element)
     double *bu, *bd;
     int* code;
+    // stores of this launch stream past the caches (non-temporal): set when the arrays a
+    // step writes and reads back cannot stay in the 256 MB Infinity Cache
+    int nt;
 };
+
+// Whether the fused kernels of a grid (or batch) of `cells` cells use streaming stores.
+// Measured on MI355X: at 256^3 and 512x512x256 they shorten the step by 3-5 %; at 128^3,
+// whose intermediates the Infinity Cache holds, they lengthen it by 15 %.
+inline bool fused_streaming(size_t cells) { return cells >= (size_t(1) << 22); }
 
 // Tile counts of an instance for a tile shape (zc must be set).
 inline void fused_geometry(FusedArgs& a, int tx, int ty)
@@ -49,11 +57,13 @@ inline bool fused_plane_fits(long long nx, long long ny) { return nx * ny <= (1L
 
 // One F1 (f1 = true; donor = first pass) or F2 launch over a device table of instance
 // descriptors; item0[i] = first work item of instance i; grid workgroups stride over
-// the nitems items.
+// the nitems items; nt: the streaming-store kernels (the table's nt fields must agree).
 int fused_batch_launch_exact(int tx, int ty, bool f1, bool donor, const FusedArgs* table,
-                             const int* item0, int ninst, int nitems, int grid, hipStream_t s);
+                             const int* item0, int ninst, int nitems, int grid, bool nt,
+                             hipStream_t s);
 int fused_batch_launch_fast(int tx, int ty, bool f1, bool donor, const FusedArgs* table,
-                            const int* item0, int ninst, int nitems, int grid, hipStream_t s);
+                            const int* item0, int ninst, int nitems, int grid, bool nt,
+                            hipStream_t s);
 // One exact-arithmetic iord-2 step cin -> cout of a PERIODIC grid that also stores the
 // forward's intermediates for the adjoint: the donor field c1, the antidiffusive faces
 // A[3] and, with the limiter, its data (see FusedArgs; null pointers when unlimited).

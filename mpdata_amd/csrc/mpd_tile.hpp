@@ -169,6 +169,12 @@ __device__ __forceinline__ void st(double* plane, unsigned colb, double v)
 {
     *(double*)((char*)plane + colb) = v;
 }
+// The same as a streaming (non-temporal) store: the line is written out without being
+// kept in the caches, for outputs far larger than the L2 and the Infinity Cache.
+__device__ __forceinline__ void st_nt(double* plane, unsigned colb, double v)
+{
+    __builtin_nontemporal_store(v, (double*)((char*)plane + colb));
+}
 
 // Byte offset of the column of region cell (cx, cy) of tile (bx, by) of an nx x ny plane
 // (periodic wrap).
