@@ -97,8 +97,11 @@ int mpd_set_sentinel(mpd_grid g, int every, int nonnegative);
  * nothing, else MPD_ERR_NUMERIC with the step count (first_bad_step may be NULL). */
 int mpd_sentinel_check(mpd_grid g, long long* first_bad_step);
 /* Kernel-path selection.  PERIODIC grids: 0 = automatic (fused), 1 = staged (one
- * kernel per stage, intermediates in HBM), 2 = fused; grids with more than 2^29 cells
- * per z-plane always take the staged path.  FRAME grids: 1 = the per-face
+ * kernel per stage, intermediates in HBM), 2 = fused (two kernels per corrective pass:
+ * antidiffusive velocities, then limiter and limited donor pass), 3 = one kernel per
+ * corrective pass (nonoscillatory grids; the donor field and the antidiffusive
+ * velocities stay on chip; its z-chunk is the one mpd_set_zchunk sets); grids with more
+ * than 2^29 cells per z-plane always take the staged path.  FRAME grids: 1 = the per-face
  * antidiffusive kernel and a separate donor pass, otherwise the tiled kernel that forms
  * the donor field and the velocities together (results are identical). */
 int mpd_set_path(mpd_grid g, int path);

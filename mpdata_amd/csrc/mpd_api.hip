@@ -624,7 +624,7 @@ extern "C" int mpd_field_stats(mpd_grid g, double* mass, double* cmin, double* c
 extern "C" int mpd_set_path(mpd_grid g, int path)
 {
     GRID_OR_FAIL(g);
-    if (path < 0 || path > 2) return fail(MPD_ERR_ARG, "mpd_set_path: %d not in 0..2", path);
+    if (path < 0 || path > 3) return fail(MPD_ERR_ARG, "mpd_set_path: %d not in 0..3", path);
     g->path = path;
     return MPD_OK;
 }

@@ -688,6 +688,465 @@ k_donor_faces(int nx, int ny, int nz, int k0, const double* __restrict__ U,
     cout[t] = r;
 }
 
+// ------------------------------------------------------ one kernel per pass ---
+//
+// F12: a whole corrective pass in one kernel -- [donor field c1 = donor(c^n)],
+// antidiffusive velocities A of c1, limiter ratios, limited donor pass -- so that c1 and
+// A never leave the chip.  The price is recomputing them on a wider halo: the ratios on
+// ring 1 of the tile need A on the faces of ring-1 cells, which needs c1 on ring 2,
+// which needs c^n (and the velocities) on ring 3.
+//
+// Geometry: a TX x TY tile with a three-cell ring (the region).  Each thread owns one
+// tile column and at most one ring column, numbered so that the expensive ring work
+// falls on as few waves as possible: ring 1 (c1, A on three faces, ratios), then the
+// ring-2 cells left of and below ring 1 (c1 and the one face ring 1 needs), the other
+// ring-2 cells (c1), ring 3 (loads only).
+//
+// Plane pipeline, iteration p (two barriers):
+//   phase 0  publish c^n(p), U(p), V(p); prefetch the next planes
+//   phase 1  c1(p) on the tile and rings 1-2; publish it with W(p-1)
+//   phase 2  A of plane k = p-1 (x/y faces of k, z-face k|k+1) on the tile and ring 1;
+//            ratios of plane m = p-2 on the tile and ring 1; the limited donor pass of
+//            plane o = p-3 on the tile.
+// LDS slots per quantity follow from how far a fast wave can run ahead (one phase):
+// c^n 4 (read for the ratios two iterations after it is published), U and V 3, c1 4,
+// W 1, A (x, y) 2, ratios 2.
+template <int TX, int TY>
+struct Geo3 {
+    static constexpr int RX = TX + 6, RY = TY + 6, R = RX * RY;
+    static constexpr int NT = TX * TY;
+    static constexpr int H1 = 2 * (TX + 2) + 2 * TY;      // ring 1
+    static constexpr int H2A = TY + 2, H2B = TX + 2;      // ring-2 cells with one face
+    static constexpr int H2 = 2 * (TX + 4) + 2 * (TY + 2);  // ring 2
+    static constexpr int H3 = 2 * (TX + 6) + 2 * (TY + 4);  // ring 3
+    static constexpr int NH = H1 + H2 + H3;
+    static_assert(NH <= NT, "every thread owns at most one ring cell");
+    static_assert(NT % 64 == 0, "whole waves");
+    __device__ static int role(int item)
+    {
+        return (int)((threadIdx.x + 64u * (unsigned)(item % (NT / 64))) % (unsigned)NT);
+    }
+    // Region cell (rx, ry) of ring cell h and its kind: 1 ring 1, 2 ring 2 left of ring 1
+    // (x-face), 3 ring 2 below ring 1 (y-face), 4 the other ring-2 cells, 5 ring 3.
+    __device__ static int halo(int h, int& rx, int& ry)
+    {
+        if (h < H1) {
+            if (h < 2 * (TX + 2)) {
+                ry = h < TX + 2 ? 2 : TY + 3;
+                rx = 2 + h % (TX + 2);
+            } else {
+                const int r = h - 2 * (TX + 2);
+                ry = 3 + r / 2;
+                rx = (r & 1) ? TX + 3 : 2;
+            }
+            return 1;
+        }
+        h -= H1;
+        if (h < H2A) {
+            rx = 1;
+            ry = 2 + h;
+            return 2;
+        }
+        h -= H2A;
+        if (h < H2B) {
+            ry = 1;
+            rx = 2 + h;
+            return 3;
+        }
+        h -= H2B;
+        if (h < H2 - H2A - H2B) {
+            if (h < 4) {
+                rx = (h & 1) ? TX + 4 : 1;
+                ry = (h & 2) ? TY + 4 : 1;
+            } else if (h < 4 + TY + 2) {
+                rx = TX + 4;
+                ry = 2 + (h - 4);
+            } else {
+                ry = TY + 4;
+                rx = 2 + (h - 4 - (TY + 2));
+            }
+            return 4;
+        }
+        h -= H2 - H2A - H2B;
+        if (h < 2 * RX) {
+            ry = h < RX ? 0 : RY - 1;
+            rx = h % RX;
+        } else {
+            const int r = h - 2 * RX;
+            ry = 1 + r / 2;
+            rx = (r & 1) ? RX - 1 : 0;
+        }
+        return 5;
+    }
+};
+
+template <int TX, int TY>
+__device__ __forceinline__ unsigned column3(int nx, int ny, int bx, int by, int cx, int cy)
+{
+    const int gx = wrapi(bx * TX + cx - 3, nx), gy = wrapi(by * TY + cy - 3, ny);
+    return (unsigned)(gy * nx + gx) * 8u;
+}
+
+// Rolling per-column state of F12 (a tile column or a ring column).
+struct Col3 {
+    unsigned colb;
+    double cn_m3 = 0, cn_m2 = 0, cn_m1 = 0, cn_0 = 0, cn_p1 = 0, ncn = 0;  // c^n(p-3 .. p+2)
+    double w_m2 = 0, w_m1 = 0, w_0 = 0, nw = 0;                           // W(p-2 .. p+1)
+    double u_m1 = 0, u_0 = 0, nu = 0, v_m1 = 0, v_0 = 0, nv = 0;          // U, V(p-1 .. p+1)
+    double fld = 0, nf = 0;                                               // pass input (p, p+1)
+    double wr_m = 0, wu_m = 0;        // W of the x / y neighbour at plane p-2
+    double c1_m4 = 0, c1_m3 = 0, c1_m2 = 0, c1_m1 = 0, c1_0 = 0;         // c1(p-4 .. p)
+    double ax_m3 = 0, ay_m3 = 0, ax_m2 = 0, ay_m2 = 0;                   // own A_x, A_y
+    double az_m4 = 0, az_m3 = 0, az_m2 = 0;                              // own A_z
+    double axl_m3 = 0, ayd_m3 = 0;    // A_x of the x-, A_y of the y- neighbour, plane p-3
+    double bu_m4 = 0, bd_m4 = 0, bu_m3 = 0, bd_m3 = 0;                   // own ratios
+};
+
+template <int TX, int TY, bool DONOR, bool NT>
+__device__ __forceinline__ void f12_tile(const FusedArgs& a, int item)
+{
+    using G = Geo3<TX, TY>;
+    constexpr int RX = G::RX, R = G::R;
+    __shared__ double s_cn[4][R], s_u[3][R], s_v[3][R], s_w[R], s_c1[4][R], s_ax[2][R],
+        s_ay[2][R], s_bu[2][R], s_bd[2][R];
+
+    const int t = G::role(item);
+    int bx, by, bz;
+    tile_of(a, item, bx, by, bz);
+    const int rx = 3 + t % TX, ry = 3 + t / TX;
+    const int L = ry * RX + rx;
+    const bool store = bx * TX + rx - 3 < a.nx && by * TY + ry - 3 < a.ny;
+    int hx = 0, hy = 0, kind = 0;
+    if (t < G::NH) kind = G::halo(t, hx, hy);
+    const int HL = hy * RX + hx;
+    const bool hl = kind != 0;                           // a ring cell: loads, publishes
+    const bool h2 = kind >= 1 && kind <= 4;              // rings 1-2: c1, W
+    const bool hax = kind == 1 || kind == 2, hay = kind == 1 || kind == 3, h1 = kind == 1;
+    Col3 T, H;
+    T.colb = column3<TX, TY>(a.nx, a.ny, bx, by, rx, ry);
+    H.colb = column3<TX, TY>(a.nx, a.ny, bx, by, hx, hy);
+
+    const int z0 = bz * a.zc;
+    const int z1 = min(z0 + a.zc, a.nz);
+    const int pstart = z0 - 2, pend = z1 + 2;
+    // planes p+2, p+1 (loads), p-2 (emission of the next output plane), p-3 (stores)
+    ZCursor zp2(pstart + 2, a.nz, a.np), zp1(pstart + 1, a.nz, a.np),
+        zm2(pstart - 2, a.nz, a.np), zm3(pstart - 3, a.nz, a.np);
+    auto first_loads = [&](Col3& C) __attribute__((always_inline)) {
+        ZCursor zm1(pstart - 1, a.nz, a.np), z0c(pstart, a.nz, a.np);
+        if (DONOR) {
+            C.cn_m1 = ld(a.cn + zm1.off, C.colb);
+            C.cn_0 = ld(a.cn + z0c.off, C.colb);
+        } else {
+            C.cn_m1 = ld(a.cn + zm1.off, C.colb);
+            C.cn_0 = ld(a.cn + z0c.off, C.colb);
+            C.nf = ld(a.cin + z0c.off, C.colb);
+        }
+        C.ncn = ld(a.cn + zp1.off, C.colb);
+        C.w_m1 = ld(a.W + zm1.off, C.colb);
+        C.nw = ld(a.W + z0c.off, C.colb);
+        C.nu = ld(a.U + z0c.off, C.colb);
+        C.nv = ld(a.V + z0c.off, C.colb);
+    };
+    first_loads(T);
+    first_loads(H);
+    double nq = 0;  // emission of plane p-2, used for the output plane of the next iteration
+
+    // the plane loop, unrolled by hand (four planes per trip, then one at a time): across
+    // the copies the per-column registers are renamed instead of copied (the barriers
+    // inside keep the compiler from unrolling a loop of unknown trip count itself)
+    auto plane = [&](const int pl) __attribute__((always_inline)) {
+        const int p = __builtin_amdgcn_readfirstlane(pl);
+        keep32(T.colb);
+        keep32(H.colb);
+        const int p8 = p + 8;  // p >= -2: non-negative slot arithmetic
+        const int s4 = p8 & 3, s3 = p8 % 3;
+        // ---- phase 0: this plane's inputs, the next planes' loads
+        __builtin_amdgcn_s_setprio(2);
+        const double qo = nq;
+        if (a.q) nq = ld(a.q + zm2.off, T.colb);
+        auto advance = [&](Col3& C) __attribute__((always_inline)) {
+            C.cn_p1 = C.ncn;
+            C.u_0 = C.nu;
+            C.v_0 = C.nv;
+            C.w_0 = C.nw;
+            C.fld = C.nf;
+            unsigned c2 = C.colb;
+            keep32(c2);
+            C.ncn = ld(a.cn + zp2.off, c2);
+            C.nu = ld(a.U + zp1.off, c2);
+            C.nv = ld(a.V + zp1.off, c2);
+            C.nw = ld(a.W + zp1.off, c2);
+            if (!DONOR) C.nf = ld(a.cin + zp1.off, c2);
+        };
+        advance(T);
+        advance(H);
+        s_cn[s4][L] = T.cn_0;
+        s_u[s3][L] = T.u_0;
+        s_v[s3][L] = T.v_0;
+        if (hl) {
+            s_cn[s4][HL] = H.cn_0;
+            s_u[s3][HL] = H.u_0;
+            s_v[s3][HL] = H.v_0;
+        }
+        lds_barrier();
+        __builtin_amdgcn_s_setprio(0);
+        // ---- phase 1: c1 of plane p
+        const bool do_c1 = p <= z1 + 1;
+        if (do_c1) {
+            const double* cn = s_cn[s4];
+            const double* up = s_u[s3];
+            const double* vp = s_v[s3];
+            T.c1_0 = DONOR ? donor(T.cn_0, cn[L - 1], cn[L + 1], cn[L - RX], cn[L + RX], T.cn_m1,
+                                   T.cn_p1, up[L - 1], T.u_0, vp[L - RX], T.v_0, T.w_m1, T.w_0)
+                           : T.fld;
+            s_c1[s4][L] = T.c1_0;
+            if (h2) {
+                H.c1_0 = DONOR ? donor(H.cn_0, cn[HL - 1], cn[HL + 1], cn[HL - RX], cn[HL + RX],
+                                       H.cn_m1, H.cn_p1, up[HL - 1], H.u_0, vp[HL - RX], H.v_0,
+                                       H.w_m1, H.w_0)
+                               : H.fld;
+                s_c1[s4][HL] = H.c1_0;
+            }
+        }
+        s_w[L] = T.w_m1;
+        if (h2) s_w[HL] = H.w_m1;
+        lds_barrier();
+        // ---- phase 2
+        const int k = p - 1, m = p - 2, o = p - 3;
+        const int km4 = (p8 - 1) & 3, km3 = (p8 - 1) % 3, kq4 = s4, kq3 = s3;  // planes k, k+1
+        const int kb4 = (p8 - 2) & 3;                                          // plane k-1
+        const double wkr = s_w[L + 1], wku = s_w[L + RX];
+        // (ring cells that compute no face are not read at their neighbours' positions,
+        // some of which lie outside the region)
+        const double hwkr = s_w[hax ? HL + 1 : L], hwku = s_w[hay ? HL + RX : L];
+        double ax_k = 0, ay_k = 0, az_k = 0, hax_k = 0, hay_k = 0, haz_k = 0;
+        if (k >= z0 - 2 && k <= z1) {
+            const double* cm = s_c1[kb4];
+            const double* cc = s_c1[km4];
+            const double* cq = s_c1[kq4];
+            const double* uk = s_u[km3];
+            const double* uq = s_u[kq3];
+            const double* vk = s_v[km3];
+            const double* vq = s_v[kq3];
+            // antidiffusive velocities of the faces above/right of cell M of plane k
+            // (the formulas and operand order of f1_tile)
+            auto faces = [&](int M, const Col3& C, double wr, double wu, bool fx, bool fy,
+                             bool fz, double& ox, double& oy, double& oz)
+                             __attribute__((always_inline)) {
+                const double c = C.c1_m1;
+                if (fx) {  // x-face (uu.c:81-87)
+                    double av = C.u_m1;
+                    double vr = vk[M + 1 - RX], vru = vk[M + 1], vu = C.v_m1, vcc = vk[M - RX];
+                    double wr_ = C.wr_m, wrf = wr, wf = C.w_m1, wc = C.w_m2;
+                    double cr = cc[M + 1], cru = cc[M + 1 + RX], cu = cc[M + RX];
+                    double cd = cc[M - RX], crd = cc[M + 1 - RX];
+                    double crf = cq[M + 1], cf = C.c1_0, cb = C.c1_m2, crb = cm[M + 1];
+#ifdef MPD_FAST_DIV
+                    const double v1 = vr + vcc, v2 = vru + vu, w1 = wr_ + wc, w2 = wrf + wf;
+                    const double b1 = cru + cu, b2 = cd + crd, e1 = crf + cf, e2 = cb + crb;
+                    double sv = 0.25 * (v1 + v2);
+                    double sw = 0.25 * (w1 + w2);
+                    double div = 0.25 * av * ((uk[M + 1] - uk[M - 1]) + (v1 - v2) + (w1 - w2));
+                    ox = antidiff(div, av, cr, c, sv, b1 - b2, b1 + b2, sw, e1 - e2, e1 + e2);
+#else
+                    double sv = 0.25 * (vr + vru + vu + vcc);
+                    double sw = 0.25 * (wr_ + wrf + wf + wc);
+                    double div = 0.25 * av * (uk[M + 1] - uk[M - 1] + vr - vru - vu + vcc + wr_ -
+                                              wrf - wf + wc);
+                    ox = antidiff(div, av, cr, c, sv, cru + cu - cd - crd, cru + cu + cd + crd,
+                                  sw, crf + cf - cb - crb, crf + cf + cb + crb);
+#endif
+                }
+                if (fy) {  // y-face (vv.c:85-92)
+                    double av = C.v_m1;
+                    double uu = uk[M - 1 + RX], uru = uk[M + RX], ur = C.u_m1, ucc = uk[M - 1];
+                    double ulu = uk[M - 2 + RX], ul = uk[M - 2];
+                    double wu_ = C.wu_m, wuf = wu, wf = C.w_m1, wc = C.w_m2;
+                    double cu = cc[M + RX], cru = cc[M + 1 + RX], cr = cc[M + 1];
+                    double cl = cc[M - 1], clu = cc[M - 1 + RX];
+                    double cuf = cq[M + RX], cf = C.c1_0, cb = C.c1_m2, cub = cm[M + RX];
+#ifdef MPD_FAST_DIV
+                    const double u1 = uu + ucc, w1 = wu_ + wc, w2 = wuf + wf;
+                    const double b1 = cru + cr, b2 = cl + clu, e1 = cuf + cf, e2 = cb + cub;
+                    double su = 0.25 * (u1 + (uru + ur));
+                    double sw = 0.25 * (w1 + w2);
+                    double div =
+                        0.25 * av * ((vk[M + RX] - vk[M - RX]) + (w1 - w2) + (u1 - (ulu + ul)));
+                    oy = antidiff(div, av, cu, c, su, b1 - b2, b1 + b2, sw, e1 - e2, e1 + e2);
+#else
+                    double su = 0.25 * (uu + uru + ur + ucc);
+                    double sw = 0.25 * (wu_ + wuf + wf + wc);
+                    double div = 0.25 * av * (vk[M + RX] - vk[M - RX] + wu_ - wuf - wf + wc + uu -
+                                              ulu - ul + ucc);
+                    oy = antidiff(div, av, cu, c, su, cru + cr - cl - clu, cru + cr + cl + clu,
+                                  sw, cuf + cf - cb - cub, cuf + cf + cb + cub);
+#endif
+                }
+                if (fz) {  // z-face (ww.c:248-255)
+                    double av = C.w_m1;
+                    double ur = C.u_m1, urf = C.u_0, uf = uq[M - 1], ucc = uk[M - 1];
+                    double vf = vq[M - RX], vuf = C.v_0, vu = C.v_m1, vcc = vk[M - RX];
+                    double cf = C.c1_0, crf = cq[M + 1], cr = cc[M + 1], cl = cc[M - 1];
+                    double clf = cq[M - 1], cuf = cq[M + RX], cu = cc[M + RX], cd = cc[M - RX];
+                    double cdf = cq[M - RX];
+#ifdef MPD_FAST_DIV
+                    const double u1 = uf + ucc, u2 = urf + ur, v1 = vf + vcc, v2 = vuf + vu;
+                    const double b1 = crf + cr, b2 = cl + clf, e1 = cuf + cu, e2 = cd + cdf;
+                    double su = 0.25 * (u1 + u2);
+                    double sv = 0.25 * (v1 + v2);
+                    double div = 0.25 * av * ((C.w_0 - C.w_m2) + (u1 - u2) + (v1 - v2));
+                    oz = antidiff(div, av, cf, c, su, b1 - b2, b1 + b2, sv, e1 - e2, e1 + e2);
+#else
+                    double su = 0.25 * (ur + urf + uf + ucc);
+                    double sv = 0.25 * (vf + vuf + vu + vcc);
+                    double div = 0.25 * av * (C.w_0 - C.w_m2 + uf - urf - ur + ucc + vf - vuf -
+                                              vu + vcc);
+                    oz = antidiff(div, av, cf, c, su, crf + cr - cl - clf, crf + cr + cl + clf,
+                                  sv, cuf + cu - cd - cdf, cuf + cu + cd + cdf);
+#endif
+                }
+            };
+            faces(L, T, wkr, wku, true, true, true, ax_k, ay_k, az_k);
+            if (hax || hay) faces(HL, H, hwkr, hwku, hax, hay, h1, hax_k, hay_k, haz_k);
+            const int ka = (p8 - 1) & 1;
+            s_ax[ka][L] = ax_k;
+            s_ay[ka][L] = ay_k;
+            if (hax) s_ax[ka][HL] = hax_k;
+            if (hay) s_ay[ka][HL] = hay_k;
+        }
+        // limiter ratios of plane m (see cpu_ref.c), clamped to at most 1
+        double bu_m = 0, bd_m = 0;
+        double axl_m = 0, ayd_m = 0;
+        if (m >= z0 - 1 && m <= z1) {
+            const int mb4 = (p8 - 2) & 3, ma = (p8 - 2) & 1;
+            const double* cnm = s_cn[mb4];
+            const double* c1m = s_c1[mb4];
+            const double* axm = s_ax[ma];
+            const double* aym = s_ay[ma];
+            auto ratios = [&](int M, const Col3& C, double& bu, double& bd, double& axl,
+                              double& ayd) __attribute__((always_inline)) {
+                const double c = C.c1_m2;
+                const double xl = c1m[M - 1], xr = c1m[M + 1], yd = c1m[M - RX], yu = c1m[M + RX];
+                const double hi =
+                    mx2(mx2(mx2(mx2(C.cn_m2, c), mx2(cnm[M - 1], xl)), mx2(mx2(cnm[M + 1], xr),
+                                                                          mx2(cnm[M - RX], yd))),
+                        mx2(mx2(mx2(cnm[M + RX], yu), mx2(C.cn_m3, C.c1_m3)),
+                            mx2(C.cn_m1, C.c1_m1)));
+                const double lo =
+                    mn2(mn2(mn2(mn2(C.cn_m2, c), mn2(cnm[M - 1], xl)), mn2(mn2(cnm[M + 1], xr),
+                                                                          mn2(cnm[M - RX], yd))),
+                        mn2(mn2(mn2(cnm[M + RX], yu), mn2(C.cn_m3, C.c1_m3)),
+                            mn2(C.cn_m1, C.c1_m1)));
+                axl = axm[M - 1];
+                ayd = aym[M - RX];
+                const double fxl = 0.5 * F2(xl, c, axl), fxr = 0.5 * F2(c, xr, C.ax_m2);
+                const double fyl = 0.5 * F2(yd, c, ayd), fyr = 0.5 * F2(c, yu, C.ay_m2);
+                const double fzl = 0.5 * F2(C.c1_m3, c, C.az_m3), fzr = 0.5 * F2(c, C.c1_m1, C.az_m2);
+                beta(hi, lo, c, fxl, fxr, fyl, fyr, fzl, fzr, bu, bd);
+                bu = mn2(bu, 1.0);
+                bd = mn2(bd, 1.0);
+            };
+            ratios(L, T, bu_m, bd_m, axl_m, ayd_m);
+            s_bu[ma][L] = bu_m;
+            s_bd[ma][L] = bd_m;
+            if (h1) {
+                double hbu, hbd, dx, dy;
+                ratios(HL, H, hbu, hbd, dx, dy);
+                s_bu[ma][HL] = hbu;
+                s_bd[ma][HL] = hbd;
+            }
+        }
+        // limited donor pass of plane o on the tile
+        if (o >= z0 && o < z1) {
+            const int ob4 = (p8 - 3) & 3, oa = (p8 - 3) & 1;
+            const double* bu = s_bu[oa];
+            const double* bd = s_bd[oa];
+            const double* cq = s_c1[ob4];
+            const double b0u = T.bu_m3, b0d = T.bd_m3;
+            const double uxr = limit_face_clamped(T.ax_m3, b0u, b0d, bu[L + 1], bd[L + 1]);
+            const double uxl = limit_face_clamped(T.axl_m3, bu[L - 1], bd[L - 1], b0u, b0d);
+            const double vyr = limit_face_clamped(T.ay_m3, b0u, b0d, bu[L + RX], bd[L + RX]);
+            const double vyl = limit_face_clamped(T.ayd_m3, bu[L - RX], bd[L - RX], b0u, b0d);
+            const double wzr = limit_face_clamped(T.az_m3, b0u, b0d, bu_m, bd_m);
+            const double wzl = limit_face_clamped(T.az_m4, T.bu_m4, T.bd_m4, b0u, b0d);
+            double r = donor(T.c1_m3, cq[L - 1], cq[L + 1], cq[L - RX], cq[L + RX], T.c1_m4,
+                             T.c1_m2, uxl, uxr, vyl, vyr, wzl, wzr);
+            if (store) {
+                if (a.q) r = r + qo;
+                put<NT>(a.cout + zm3.off, T.colb, r);
+                if (a.Lx) {
+                    put<NT>(a.Lx + zm3.off, T.colb, uxr);
+                    put<NT>(a.Ly + zm3.off, T.colb, vyr);
+                    put<NT>(a.Lz + zm3.off, T.colb, wzr);
+                }
+            }
+        }
+        // rotate the per-column registers
+        auto rotate = [&](Col3& C, double wkr_, double wku_, double ax, double ay, double az)
+                          __attribute__((always_inline)) {
+            C.cn_m3 = C.cn_m2;
+            C.cn_m2 = C.cn_m1;
+            C.cn_m1 = C.cn_0;
+            C.cn_0 = C.cn_p1;
+            C.w_m2 = C.w_m1;
+            C.w_m1 = C.w_0;
+            C.wr_m = wkr_;
+            C.wu_m = wku_;
+            C.u_m1 = C.u_0;
+            C.v_m1 = C.v_0;
+            C.c1_m4 = C.c1_m3;
+            C.c1_m3 = C.c1_m2;
+            C.c1_m2 = C.c1_m1;
+            C.c1_m1 = C.c1_0;
+            C.ax_m3 = C.ax_m2;
+            C.ay_m3 = C.ay_m2;
+            C.ax_m2 = ax;
+            C.ay_m2 = ay;
+            C.az_m4 = C.az_m3;
+            C.az_m3 = C.az_m2;
+            C.az_m2 = az;
+        };
+        rotate(T, wkr, wku, ax_k, ay_k, az_k);
+        rotate(H, hwkr, hwku, hax_k, hay_k, haz_k);
+        T.axl_m3 = axl_m;
+        T.ayd_m3 = ayd_m;
+        T.bu_m4 = T.bu_m3;
+        T.bd_m4 = T.bd_m3;
+        T.bu_m3 = bu_m;
+        T.bd_m3 = bd_m;
+        zp2.next();
+        zp1.next();
+        zm2.next();
+        zm3.next();
+    };
+    int pl = pstart;
+    for (; pl + 3 <= pend; pl += 4) {
+        plane(pl);
+        plane(pl + 1);
+        plane(pl + 2);
+        plane(pl + 3);
+    }
+    for (; pl <= pend; pl++) plane(pl);
+}
+
+template <int TX, int TY, bool DONOR, bool NT>
+__global__ void __launch_bounds__((Geo3<TX, TY>::NT)) k_f12(FusedArgs a, Work w)
+{
+    for (int it = blockIdx.x; it < w.nitems; it += gridDim.x) {
+        int item = item_of(w, it);
+        FusedArgs x = a;
+        if (w.ninst) {
+            ConstInt* item0 = (ConstInt*)w.item0;
+            int i = find_inst(item0, w.ninst, item);
+            __builtin_memcpy(&x, (const FusedArgs*)((ConstArgs*)w.table + i), sizeof x);
+            item -= item0[i];
+        }
+        f12_tile<TX, TY, DONOR, NT>(x, item);
+        lds_barrier();
+    }
+}
+
 // ---------------------------------------------------------------- dispatch ---
 
 // Launches a kernel; with ev set, between the start event ev[0] and the stop event
@@ -739,6 +1198,29 @@ int launch_any(const FusedArgs& a, const Work& w, int grid, int tx, int ty, bool
     MPD_FUSED_TILES(MPD_TILE)
 #undef MPD_TILE
     return fail(MPD_ERR_ARG, "tile %dx%d is not one of the compiled tile shapes", tx, ty);
+}
+
+// Tile shapes of the one-kernel-per-pass path (LDS: 23 planes of the tile + 3-cell ring).
+constexpr int F12_TX = 32, F12_TY = 16;
+
+int launch_f12(const FusedArgs& a, const Work& w, int grid, bool donor, hipStream_t s,
+               const hipEvent_t* ev)
+{
+    constexpr int TX = F12_TX, TY = F12_TY;
+    const dim3 blk(Geo3<TX, TY>::NT);
+    if (donor) {
+        if (a.nt)
+            launch_k(k_f12<TX, TY, true, true>, dim3(grid), blk, s, ev, a, w);
+        else
+            launch_k(k_f12<TX, TY, true, false>, dim3(grid), blk, s, ev, a, w);
+    } else {
+        if (a.nt)
+            launch_k(k_f12<TX, TY, false, true>, dim3(grid), blk, s, ev, a, w);
+        else
+            launch_k(k_f12<TX, TY, false, false>, dim3(grid), blk, s, ev, a, w);
+    }
+    MPD_HIP(hipGetLastError());
+    return MPD_OK;
 }
 
 // single grid: one work item per workgroup
@@ -885,6 +1367,37 @@ int MPD_CAT(periodic_fused_step, MPD_ARITH)(mpd_grid_s* g)
     double* out = g->c[1 - g->cur].p;
     const double* prev[3] = {g->face[0].p, g->face[1].p, g->face[2].p};
     const double* field = cin;
+    if (g->nonosc && g->path == 3) {
+        // one kernel per corrective pass; the limited faces of pass p (read by pass p+1)
+        // alternate between the two face sets
+        const int zc = g->zc ? g->zc : choose_zc(g, F12_TX, F12_TY);
+        a.zc = zc;
+        fused_geometry(a, F12_TX, F12_TY);
+        const int nb = fused_items(a);
+        const Work w{nullptr, nullptr, 0, nb, nb % 8 == 0 ? 1 : 0};
+        for (int p = 1; p < g->iord; p++) {
+            const bool last = (p == g->iord - 1);
+            mpd::DevBuf* lf = (p & 1) ? g->fb : g->fa;
+            a.cn = cin;
+            a.cin = field;
+            a.U = prev[0];
+            a.V = prev[1];
+            a.W = prev[2];
+            a.q = last ? q : nullptr;
+            a.cout = last ? out : g->cp[p & 1].p;
+            a.Lx = last ? nullptr : lf[0].p;
+            a.Ly = last ? nullptr : lf[1].p;
+            a.Lz = last ? nullptr : lf[2].p;
+            hipEvent_t ev[2];
+            const hipEvent_t* use;
+            if ((rc = span(g, ev, &use))) return rc;
+            if ((rc = launch_f12(a, w, nb, p == 1, s, use))) return rc;
+            for (int d = 0; d < 3; d++) prev[d] = lf[d].p;
+            field = a.cout;
+        }
+        g->cur = 1 - g->cur;
+        return MPD_OK;
+    }
     for (int p = 1; p < g->iord; p++) {
         const bool last = (p == g->iord - 1);
         double* dst = last ? out : g->cp[p & 1].p;

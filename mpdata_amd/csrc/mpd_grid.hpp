@@ -9,7 +9,7 @@ struct mpd_grid_s {
     int boundary = MPD_PERIODIC;
     int iord = 2;
     int nonosc = 0;
-    int path = 0;  // 0 auto, 1 staged, 2 fused
+    int path = 0;  // 0 auto, 1 staged, 2 fused (two kernels per pass), 3 one kernel per pass
     int arith = MPD_ARITH_FAST;
     int adj_scatter = 0;  // adjoint: 0 deterministic gathers, 1 atomic scatter
     int adj_tiled = 1;    // adjoint limiter stages: 1 one LDS-tiled kernel, 0 per-cell passes
