@@ -26,6 +26,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "mpd_formulas.hpp"
 #include "mpd_fused.hpp"
@@ -855,7 +856,11 @@ __device__ __forceinline__ void f12_tile(const FusedArgs& a, int item)
     // the plane loop, unrolled by hand (four planes per trip, then one at a time): across
     // the copies the per-column registers are renamed instead of copied (the barriers
     // inside keep the compiler from unrolling a loop of unknown trip count itself)
-    auto plane = [&](const int pl) __attribute__((always_inline)) {
+    // steady: every stage is active (planes z0 + 3 .. z1 + 1), so the body has no stage
+    // guards and the compiler may move each stage's LDS reads ahead of the previous
+    // stage's arithmetic
+    auto plane = [&](const int pl, auto steady) __attribute__((always_inline)) {
+        constexpr bool S = decltype(steady)::value;
         const int p = __builtin_amdgcn_readfirstlane(pl);
         keep32(T.colb);
         keep32(H.colb);
@@ -892,7 +897,7 @@ __device__ __forceinline__ void f12_tile(const FusedArgs& a, int item)
         lds_barrier();
         __builtin_amdgcn_s_setprio(0);
         // ---- phase 1: c1 of plane p
-        const bool do_c1 = p <= z1 + 1;
+        const bool do_c1 = S || p <= z1 + 1;
         if (do_c1) {
             const double* cn = s_cn[s4];
             const double* up = s_u[s3];
@@ -912,157 +917,151 @@ __device__ __forceinline__ void f12_tile(const FusedArgs& a, int item)
         s_w[L] = T.w_m1;
         if (h2) s_w[HL] = H.w_m1;
         lds_barrier();
-        // ---- phase 2
+        // ---- phase 2: the tile's three stages first (one block in steady planes: they
+        // read only LDS planes published before this phase), then the ring cells' work
         const int k = p - 1, m = p - 2, o = p - 3;
+        const bool doA = S || (k >= z0 - 2 && k <= z1);
+        const bool doB = S || (m >= z0 - 1 && m <= z1);
+        const bool doO = S || (o >= z0 && o < z1);
         const int km4 = (p8 - 1) & 3, km3 = (p8 - 1) % 3, kq4 = s4, kq3 = s3;  // planes k, k+1
         const int kb4 = (p8 - 2) & 3;                                          // plane k-1
+        const int ka = (p8 - 1) & 1;
         const double wkr = s_w[L + 1], wku = s_w[L + RX];
         // (ring cells that compute no face are not read at their neighbours' positions,
         // some of which lie outside the region)
         const double hwkr = s_w[hax ? HL + 1 : L], hwku = s_w[hay ? HL + RX : L];
+        const double* cm = s_c1[kb4];
+        const double* cc = s_c1[km4];
+        const double* cq = s_c1[kq4];
+        const double* uk = s_u[km3];
+        const double* uq = s_u[kq3];
+        const double* vk = s_v[km3];
+        const double* vq = s_v[kq3];
+        // antidiffusive velocities of the faces above/right of cell M of plane k
+        // (the formulas and operand order of f1_tile)
+        auto faces = [&](int M, const Col3& C, double wr, double wu, bool fx, bool fy,
+                         bool fz, double& ox, double& oy, double& oz)
+                         __attribute__((always_inline)) {
+            const double c = C.c1_m1;
+            if (fx) {  // x-face (uu.c:81-87)
+                double av = C.u_m1;
+                double vr = vk[M + 1 - RX], vru = vk[M + 1], vu = C.v_m1, vcc = vk[M - RX];
+                double wr_ = C.wr_m, wrf = wr, wf = C.w_m1, wc = C.w_m2;
+                double cr = cc[M + 1], cru = cc[M + 1 + RX], cu = cc[M + RX];
+                double cd = cc[M - RX], crd = cc[M + 1 - RX];
+                double crf = cq[M + 1], cf = C.c1_0, cb = C.c1_m2, crb = cm[M + 1];
+#ifdef MPD_FAST_DIV
+                const double v1 = vr + vcc, v2 = vru + vu, w1 = wr_ + wc, w2 = wrf + wf;
+                const double b1 = cru + cu, b2 = cd + crd, e1 = crf + cf, e2 = cb + crb;
+                double sv = 0.25 * (v1 + v2);
+                double sw = 0.25 * (w1 + w2);
+                double div = 0.25 * av * ((uk[M + 1] - uk[M - 1]) + (v1 - v2) + (w1 - w2));
+                ox = antidiff(div, av, cr, c, sv, b1 - b2, b1 + b2, sw, e1 - e2, e1 + e2);
+#else
+                double sv = 0.25 * (vr + vru + vu + vcc);
+                double sw = 0.25 * (wr_ + wrf + wf + wc);
+                double div = 0.25 * av * (uk[M + 1] - uk[M - 1] + vr - vru - vu + vcc + wr_ -
+                                          wrf - wf + wc);
+                ox = antidiff(div, av, cr, c, sv, cru + cu - cd - crd, cru + cu + cd + crd,
+                              sw, crf + cf - cb - crb, crf + cf + cb + crb);
+#endif
+            }
+            if (fy) {  // y-face (vv.c:85-92)
+                double av = C.v_m1;
+                double uu = uk[M - 1 + RX], uru = uk[M + RX], ur = C.u_m1, ucc = uk[M - 1];
+                double ulu = uk[M - 2 + RX], ul = uk[M - 2];
+                double wu_ = C.wu_m, wuf = wu, wf = C.w_m1, wc = C.w_m2;
+                double cu = cc[M + RX], cru = cc[M + 1 + RX], cr = cc[M + 1];
+                double cl = cc[M - 1], clu = cc[M - 1 + RX];
+                double cuf = cq[M + RX], cf = C.c1_0, cb = C.c1_m2, cub = cm[M + RX];
+#ifdef MPD_FAST_DIV
+                const double u1 = uu + ucc, w1 = wu_ + wc, w2 = wuf + wf;
+                const double b1 = cru + cr, b2 = cl + clu, e1 = cuf + cf, e2 = cb + cub;
+                double su = 0.25 * (u1 + (uru + ur));
+                double sw = 0.25 * (w1 + w2);
+                double div =
+                    0.25 * av * ((vk[M + RX] - vk[M - RX]) + (w1 - w2) + (u1 - (ulu + ul)));
+                oy = antidiff(div, av, cu, c, su, b1 - b2, b1 + b2, sw, e1 - e2, e1 + e2);
+#else
+                double su = 0.25 * (uu + uru + ur + ucc);
+                double sw = 0.25 * (wu_ + wuf + wf + wc);
+                double div = 0.25 * av * (vk[M + RX] - vk[M - RX] + wu_ - wuf - wf + wc + uu -
+                                          ulu - ul + ucc);
+                oy = antidiff(div, av, cu, c, su, cru + cr - cl - clu, cru + cr + cl + clu,
+                              sw, cuf + cf - cb - cub, cuf + cf + cb + cub);
+#endif
+            }
+            if (fz) {  // z-face (ww.c:248-255)
+                double av = C.w_m1;
+                double ur = C.u_m1, urf = C.u_0, uf = uq[M - 1], ucc = uk[M - 1];
+                double vf = vq[M - RX], vuf = C.v_0, vu = C.v_m1, vcc = vk[M - RX];
+                double cf = C.c1_0, crf = cq[M + 1], cr = cc[M + 1], cl = cc[M - 1];
+                double clf = cq[M - 1], cuf = cq[M + RX], cu = cc[M + RX], cd = cc[M - RX];
+                double cdf = cq[M - RX];
+#ifdef MPD_FAST_DIV
+                const double u1 = uf + ucc, u2 = urf + ur, v1 = vf + vcc, v2 = vuf + vu;
+                const double b1 = crf + cr, b2 = cl + clf, e1 = cuf + cu, e2 = cd + cdf;
+                double su = 0.25 * (u1 + u2);
+                double sv = 0.25 * (v1 + v2);
+                double div = 0.25 * av * ((C.w_0 - C.w_m2) + (u1 - u2) + (v1 - v2));
+                oz = antidiff(div, av, cf, c, su, b1 - b2, b1 + b2, sv, e1 - e2, e1 + e2);
+#else
+                double su = 0.25 * (ur + urf + uf + ucc);
+                double sv = 0.25 * (vf + vuf + vu + vcc);
+                double div = 0.25 * av * (C.w_0 - C.w_m2 + uf - urf - ur + ucc + vf - vuf -
+                                          vu + vcc);
+                oz = antidiff(div, av, cf, c, su, crf + cr - cl - clf, crf + cr + cl + clf,
+                              sv, cuf + cu - cd - cdf, cuf + cu + cd + cdf);
+#endif
+            }
+        };
+        // limiter ratios of plane m (see cpu_ref.c), clamped to at most 1
+        const int mb4 = (p8 - 2) & 3, ma = (p8 - 2) & 1;
+        const double* cnm = s_cn[mb4];
+        const double* c1m = s_c1[mb4];
+        const double* axm = s_ax[ma];
+        const double* aym = s_ay[ma];
+        auto ratios = [&](int M, const Col3& C, double& bu, double& bd, double& axl,
+                          double& ayd) __attribute__((always_inline)) {
+            const double c = C.c1_m2;
+            const double xl = c1m[M - 1], xr = c1m[M + 1], yd = c1m[M - RX], yu = c1m[M + RX];
+            const double hi =
+                mx2(mx2(mx2(mx2(C.cn_m2, c), mx2(cnm[M - 1], xl)), mx2(mx2(cnm[M + 1], xr),
+                                                                      mx2(cnm[M - RX], yd))),
+                    mx2(mx2(mx2(cnm[M + RX], yu), mx2(C.cn_m3, C.c1_m3)),
+                        mx2(C.cn_m1, C.c1_m1)));
+            const double lo =
+                mn2(mn2(mn2(mn2(C.cn_m2, c), mn2(cnm[M - 1], xl)), mn2(mn2(cnm[M + 1], xr),
+                                                                      mn2(cnm[M - RX], yd))),
+                    mn2(mn2(mn2(cnm[M + RX], yu), mn2(C.cn_m3, C.c1_m3)),
+                        mn2(C.cn_m1, C.c1_m1)));
+            axl = axm[M - 1];
+            ayd = aym[M - RX];
+            const double fxl = 0.5 * F2(xl, c, axl), fxr = 0.5 * F2(c, xr, C.ax_m2);
+            const double fyl = 0.5 * F2(yd, c, ayd), fyr = 0.5 * F2(c, yu, C.ay_m2);
+            const double fzl = 0.5 * F2(C.c1_m3, c, C.az_m3), fzr = 0.5 * F2(c, C.c1_m1, C.az_m2);
+            beta(hi, lo, c, fxl, fxr, fyl, fyr, fzl, fzr, bu, bd);
+            bu = mn2(bu, 1.0);
+            bd = mn2(bd, 1.0);
+        };
         double ax_k = 0, ay_k = 0, az_k = 0, hax_k = 0, hay_k = 0, haz_k = 0;
-        if (k >= z0 - 2 && k <= z1) {
-            const double* cm = s_c1[kb4];
-            const double* cc = s_c1[km4];
-            const double* cq = s_c1[kq4];
-            const double* uk = s_u[km3];
-            const double* uq = s_u[kq3];
-            const double* vk = s_v[km3];
-            const double* vq = s_v[kq3];
-            // antidiffusive velocities of the faces above/right of cell M of plane k
-            // (the formulas and operand order of f1_tile)
-            auto faces = [&](int M, const Col3& C, double wr, double wu, bool fx, bool fy,
-                             bool fz, double& ox, double& oy, double& oz)
-                             __attribute__((always_inline)) {
-                const double c = C.c1_m1;
-                if (fx) {  // x-face (uu.c:81-87)
-                    double av = C.u_m1;
-                    double vr = vk[M + 1 - RX], vru = vk[M + 1], vu = C.v_m1, vcc = vk[M - RX];
-                    double wr_ = C.wr_m, wrf = wr, wf = C.w_m1, wc = C.w_m2;
-                    double cr = cc[M + 1], cru = cc[M + 1 + RX], cu = cc[M + RX];
-                    double cd = cc[M - RX], crd = cc[M + 1 - RX];
-                    double crf = cq[M + 1], cf = C.c1_0, cb = C.c1_m2, crb = cm[M + 1];
-#ifdef MPD_FAST_DIV
-                    const double v1 = vr + vcc, v2 = vru + vu, w1 = wr_ + wc, w2 = wrf + wf;
-                    const double b1 = cru + cu, b2 = cd + crd, e1 = crf + cf, e2 = cb + crb;
-                    double sv = 0.25 * (v1 + v2);
-                    double sw = 0.25 * (w1 + w2);
-                    double div = 0.25 * av * ((uk[M + 1] - uk[M - 1]) + (v1 - v2) + (w1 - w2));
-                    ox = antidiff(div, av, cr, c, sv, b1 - b2, b1 + b2, sw, e1 - e2, e1 + e2);
-#else
-                    double sv = 0.25 * (vr + vru + vu + vcc);
-                    double sw = 0.25 * (wr_ + wrf + wf + wc);
-                    double div = 0.25 * av * (uk[M + 1] - uk[M - 1] + vr - vru - vu + vcc + wr_ -
-                                              wrf - wf + wc);
-                    ox = antidiff(div, av, cr, c, sv, cru + cu - cd - crd, cru + cu + cd + crd,
-                                  sw, crf + cf - cb - crb, crf + cf + cb + crb);
-#endif
-                }
-                if (fy) {  // y-face (vv.c:85-92)
-                    double av = C.v_m1;
-                    double uu = uk[M - 1 + RX], uru = uk[M + RX], ur = C.u_m1, ucc = uk[M - 1];
-                    double ulu = uk[M - 2 + RX], ul = uk[M - 2];
-                    double wu_ = C.wu_m, wuf = wu, wf = C.w_m1, wc = C.w_m2;
-                    double cu = cc[M + RX], cru = cc[M + 1 + RX], cr = cc[M + 1];
-                    double cl = cc[M - 1], clu = cc[M - 1 + RX];
-                    double cuf = cq[M + RX], cf = C.c1_0, cb = C.c1_m2, cub = cm[M + RX];
-#ifdef MPD_FAST_DIV
-                    const double u1 = uu + ucc, w1 = wu_ + wc, w2 = wuf + wf;
-                    const double b1 = cru + cr, b2 = cl + clu, e1 = cuf + cf, e2 = cb + cub;
-                    double su = 0.25 * (u1 + (uru + ur));
-                    double sw = 0.25 * (w1 + w2);
-                    double div =
-                        0.25 * av * ((vk[M + RX] - vk[M - RX]) + (w1 - w2) + (u1 - (ulu + ul)));
-                    oy = antidiff(div, av, cu, c, su, b1 - b2, b1 + b2, sw, e1 - e2, e1 + e2);
-#else
-                    double su = 0.25 * (uu + uru + ur + ucc);
-                    double sw = 0.25 * (wu_ + wuf + wf + wc);
-                    double div = 0.25 * av * (vk[M + RX] - vk[M - RX] + wu_ - wuf - wf + wc + uu -
-                                              ulu - ul + ucc);
-                    oy = antidiff(div, av, cu, c, su, cru + cr - cl - clu, cru + cr + cl + clu,
-                                  sw, cuf + cf - cb - cub, cuf + cf + cb + cub);
-#endif
-                }
-                if (fz) {  // z-face (ww.c:248-255)
-                    double av = C.w_m1;
-                    double ur = C.u_m1, urf = C.u_0, uf = uq[M - 1], ucc = uk[M - 1];
-                    double vf = vq[M - RX], vuf = C.v_0, vu = C.v_m1, vcc = vk[M - RX];
-                    double cf = C.c1_0, crf = cq[M + 1], cr = cc[M + 1], cl = cc[M - 1];
-                    double clf = cq[M - 1], cuf = cq[M + RX], cu = cc[M + RX], cd = cc[M - RX];
-                    double cdf = cq[M - RX];
-#ifdef MPD_FAST_DIV
-                    const double u1 = uf + ucc, u2 = urf + ur, v1 = vf + vcc, v2 = vuf + vu;
-                    const double b1 = crf + cr, b2 = cl + clf, e1 = cuf + cu, e2 = cd + cdf;
-                    double su = 0.25 * (u1 + u2);
-                    double sv = 0.25 * (v1 + v2);
-                    double div = 0.25 * av * ((C.w_0 - C.w_m2) + (u1 - u2) + (v1 - v2));
-                    oz = antidiff(div, av, cf, c, su, b1 - b2, b1 + b2, sv, e1 - e2, e1 + e2);
-#else
-                    double su = 0.25 * (ur + urf + uf + ucc);
-                    double sv = 0.25 * (vf + vuf + vu + vcc);
-                    double div = 0.25 * av * (C.w_0 - C.w_m2 + uf - urf - ur + ucc + vf - vuf -
-                                              vu + vcc);
-                    oz = antidiff(div, av, cf, c, su, crf + cr - cl - clf, crf + cr + cl + clf,
-                                  sv, cuf + cu - cd - cdf, cuf + cu + cd + cdf);
-#endif
-                }
-            };
+        double bu_m = 0, bd_m = 0, axl_m = 0, ayd_m = 0;
+        if (doA) {
             faces(L, T, wkr, wku, true, true, true, ax_k, ay_k, az_k);
-            if (hax || hay) faces(HL, H, hwkr, hwku, hax, hay, h1, hax_k, hay_k, haz_k);
-            const int ka = (p8 - 1) & 1;
             s_ax[ka][L] = ax_k;
             s_ay[ka][L] = ay_k;
-            if (hax) s_ax[ka][HL] = hax_k;
-            if (hay) s_ay[ka][HL] = hay_k;
         }
-        // limiter ratios of plane m (see cpu_ref.c), clamped to at most 1
-        double bu_m = 0, bd_m = 0;
-        double axl_m = 0, ayd_m = 0;
-        if (m >= z0 - 1 && m <= z1) {
-            const int mb4 = (p8 - 2) & 3, ma = (p8 - 2) & 1;
-            const double* cnm = s_cn[mb4];
-            const double* c1m = s_c1[mb4];
-            const double* axm = s_ax[ma];
-            const double* aym = s_ay[ma];
-            auto ratios = [&](int M, const Col3& C, double& bu, double& bd, double& axl,
-                              double& ayd) __attribute__((always_inline)) {
-                const double c = C.c1_m2;
-                const double xl = c1m[M - 1], xr = c1m[M + 1], yd = c1m[M - RX], yu = c1m[M + RX];
-                const double hi =
-                    mx2(mx2(mx2(mx2(C.cn_m2, c), mx2(cnm[M - 1], xl)), mx2(mx2(cnm[M + 1], xr),
-                                                                          mx2(cnm[M - RX], yd))),
-                        mx2(mx2(mx2(cnm[M + RX], yu), mx2(C.cn_m3, C.c1_m3)),
-                            mx2(C.cn_m1, C.c1_m1)));
-                const double lo =
-                    mn2(mn2(mn2(mn2(C.cn_m2, c), mn2(cnm[M - 1], xl)), mn2(mn2(cnm[M + 1], xr),
-                                                                          mn2(cnm[M - RX], yd))),
-                        mn2(mn2(mn2(cnm[M + RX], yu), mn2(C.cn_m3, C.c1_m3)),
-                            mn2(C.cn_m1, C.c1_m1)));
-                axl = axm[M - 1];
-                ayd = aym[M - RX];
-                const double fxl = 0.5 * F2(xl, c, axl), fxr = 0.5 * F2(c, xr, C.ax_m2);
-                const double fyl = 0.5 * F2(yd, c, ayd), fyr = 0.5 * F2(c, yu, C.ay_m2);
-                const double fzl = 0.5 * F2(C.c1_m3, c, C.az_m3), fzr = 0.5 * F2(c, C.c1_m1, C.az_m2);
-                beta(hi, lo, c, fxl, fxr, fyl, fyr, fzl, fzr, bu, bd);
-                bu = mn2(bu, 1.0);
-                bd = mn2(bd, 1.0);
-            };
+        if (doB) {
             ratios(L, T, bu_m, bd_m, axl_m, ayd_m);
             s_bu[ma][L] = bu_m;
             s_bd[ma][L] = bd_m;
-            if (h1) {
-                double hbu, hbd, dx, dy;
-                ratios(HL, H, hbu, hbd, dx, dy);
-                s_bu[ma][HL] = hbu;
-                s_bd[ma][HL] = hbd;
-            }
         }
         // limited donor pass of plane o on the tile
-        if (o >= z0 && o < z1) {
+        if (doO) {
             const int ob4 = (p8 - 3) & 3, oa = (p8 - 3) & 1;
             const double* bu = s_bu[oa];
             const double* bd = s_bd[oa];
-            const double* cq = s_c1[ob4];
+            const double* co = s_c1[ob4];
             const double b0u = T.bu_m3, b0d = T.bd_m3;
             const double uxr = limit_face_clamped(T.ax_m3, b0u, b0d, bu[L + 1], bd[L + 1]);
             const double uxl = limit_face_clamped(T.axl_m3, bu[L - 1], bd[L - 1], b0u, b0d);
@@ -1070,7 +1069,7 @@ __device__ __forceinline__ void f12_tile(const FusedArgs& a, int item)
             const double vyl = limit_face_clamped(T.ayd_m3, bu[L - RX], bd[L - RX], b0u, b0d);
             const double wzr = limit_face_clamped(T.az_m3, b0u, b0d, bu_m, bd_m);
             const double wzl = limit_face_clamped(T.az_m4, T.bu_m4, T.bd_m4, b0u, b0d);
-            double r = donor(T.c1_m3, cq[L - 1], cq[L + 1], cq[L - RX], cq[L + RX], T.c1_m4,
+            double r = donor(T.c1_m3, co[L - 1], co[L + 1], co[L - RX], co[L + RX], T.c1_m4,
                              T.c1_m2, uxl, uxr, vyl, vyr, wzl, wzr);
             if (store) {
                 if (a.q) r = r + qo;
@@ -1081,6 +1080,17 @@ __device__ __forceinline__ void f12_tile(const FusedArgs& a, int item)
                     put<NT>(a.Lz + zm3.off, T.colb, wzr);
                 }
             }
+        }
+        if (doA && (hax || hay)) {
+            faces(HL, H, hwkr, hwku, hax, hay, h1, hax_k, hay_k, haz_k);
+            if (hax) s_ax[ka][HL] = hax_k;
+            if (hay) s_ay[ka][HL] = hay_k;
+        }
+        if (doB && h1) {
+            double hbu, hbd, dx, dy;
+            ratios(HL, H, hbu, hbd, dx, dy);
+            s_bu[ma][HL] = hbu;
+            s_bd[ma][HL] = hbd;
         }
         // rotate the per-column registers
         auto rotate = [&](Col3& C, double wkr_, double wku_, double ax, double ay, double az)
@@ -1120,14 +1130,19 @@ __device__ __forceinline__ void f12_tile(const FusedArgs& a, int item)
         zm2.next();
         zm3.next();
     };
+    using Steady = std::integral_constant<bool, true>;
+    using Edge = std::integral_constant<bool, false>;
     int pl = pstart;
-    for (; pl + 3 <= pend; pl += 4) {
-        plane(pl);
-        plane(pl + 1);
-        plane(pl + 2);
-        plane(pl + 3);
+    const int s0 = z0 + 3, s1 = z1 + 1;  // steady planes
+    for (; pl < s0 && pl <= pend; pl++) plane(pl, Edge());
+    for (; pl + 3 <= s1; pl += 4) {
+        plane(pl, Steady());
+        plane(pl + 1, Steady());
+        plane(pl + 2, Steady());
+        plane(pl + 3, Steady());
     }
-    for (; pl <= pend; pl++) plane(pl);
+    for (; pl <= s1; pl++) plane(pl, Steady());
+    for (; pl <= pend; pl++) plane(pl, Edge());
 }
 
 template <int TX, int TY, bool DONOR, bool NT>
