@@ -55,8 +55,11 @@ def face_courant_from_cell_winds(u, v, w, dt=1.0):
 
 
 def tuned_config(g):
-    """The launch shapes a grid steps with: the antidiffusive kernel's tile and z-chunk,
-    and the limiter kernel's (its own when the tuner chose one)."""
+    """The launch shapes a grid steps with: one kernel per corrective pass (32x16 tiles,
+    its z-chunk), or the antidiffusive kernel's tile and z-chunk and the limiter kernel's
+    (its own when the tuner chose one)."""
+    if g.path() == 3:
+        return {"kernel": "one per corrective pass", "tile": [32, 16], "zc": g.pass_zchunk()}
     (tx, ty, zc), (lx, ly, lz) = g.tiles()
     out = {"tile": [tx, ty], "zc": zc}
     if lx:
@@ -116,6 +119,15 @@ def step_kernels(M, iord, kms, path=0):
         return [{"name": "staged", "avg_ms": t, "io_bytes_per_cell": None, "io_GB/s": None}
                 for t in kms]
     names, own = [], []
+    if len(kms) == iord - 1:  # one kernel per corrective pass
+        for p in range(1, iord):
+            last = p == iord - 1
+            names.append(f"k_f12 (pass {p})")
+            # reads c^n, the pass input (later passes), the three faces; writes the
+            # result (and the limited faces when another pass follows)
+            own.append((32 if p == 1 else 40) + (8 if last else 32))
+        return [{"name": nm, "avg_ms": t, "io_bytes_per_cell": b}
+                for nm, t, b in zip(names, kms, own)]
     for p in range(1, iord):
         names += [f"k_f1 (pass {p})", f"k_f2 (pass {p})"]
         last = p == iord - 1
@@ -198,6 +210,13 @@ def main():
     ap.add_argument("--ring2d", type=int, default=128,
                     help="edge of the periodic 2-D ring leg, 1024 steps (0: skip)")
     ap.add_argument("--legs", type=int, default=1, help="0: the headline only")
+    ap.add_argument("--only", default="",
+                    help="run one workload alone and print its object (for profiling): "
+                         "headline, ring3d, large, ensemble, ring2d, dropin, adjoint")
+    ap.add_argument("--profile", action="store_true",
+                    help="skip the exact-arithmetic re-runs and CPU checks (the exact and "
+                         "fast kernels share names, which would mix in a profiler's "
+                         "statistics)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU (every box has one); the "
@@ -216,8 +235,20 @@ def main():
                           "GB/s": nbytes / ms / 1e6}))
         return
 
+    if args.profile:
+        args.no_cpu = True
+    if args.only and args.only != "headline":
+        legs = {"ring3d": lambda: ring3d_leg(M, args.ring3d, args.arith, check=not args.profile),
+                "large": lambda: large_leg(M, args.large, 2, args.arith, check=not args.profile),
+                "ensemble": lambda: ensemble_leg(M, args.ensemble),
+                "ring2d": lambda: ring2d_leg(M, args.ring2d),
+                "dropin": lambda: dropin_leg(M, args.dropin),
+                "adjoint": lambda: adjoint_leg(M, args.adjoint, nsteps=args.adjoint_steps,
+                                               checks=bool(args.adjoint_checks))}
+        print(json.dumps({args.only: legs[args.only]()}))
+        return
     line = headline(M, args)
-    if args.legs:
+    if args.legs and not args.only:
         if args.ring3d:
             line["ring3d"] = ring3d_leg(M, args.ring3d, args.arith)
         if args.ensemble:
@@ -268,12 +299,14 @@ def headline(M, args):
     # the whole trajectory (warm-up, timed and kernel-timing steps; the autotuner restores
     # the state it found) again in exact arithmetic, which equals the oracle's bit for bit
     taken = args.warmup + args.steps + 2
-    fast = g.field()
-    g.set_arith(M.ARITH_EXACT)
-    g.set_field(c0)
-    g.step(taken)
-    err_run = relerr(fast, g.field())
-    del fast
+    err_run = None
+    if not args.profile:
+        fast = g.field()
+        g.set_arith(M.ARITH_EXACT)
+        g.set_field(c0)
+        g.step(taken)
+        err_run = relerr(fast, g.field())
+        del fast
     line = {
         "metric": "cell-updates/s at <=1e-12 rel err (FP64 MPDATA step, 1 GPU)",
         "value": value,
@@ -299,7 +332,7 @@ def headline(M, args):
         "wall_s_timed": wall,
         "mass_rel_drift": abs(mass - mass0) / mass0, "min": cmin,
     }
-    par = {f"steps_{taken}_vs_exact_kernels": {"rel_err": err_run}}
+    par = {f"steps_{taken}_vs_exact_kernels": {"rel_err": err_run}} if err_run is not None else {}
     if not args.no_cpu and args.cpu_steps > 0:
         ks = args.cpu_steps
         g.set_arith(args.arith)
@@ -348,7 +381,7 @@ def headline(M, args):
     return line
 
 
-def ring3d_leg(M, n, arith, steps=128):
+def ring3d_leg(M, n, arith, steps=128, check=True):
     """The cache-resident rotation case: n^3 periodic, solid-body rotation about (1,1,1)
     at max Courant 0.4, Gaussian blob sigma 8, iord 2 nonoscillatory; autotuned, 64
     warm-up and `steps` timed steps, the step's roofline, and the run against the
@@ -369,6 +402,10 @@ def ring3d_leg(M, n, arith, steps=128):
     ms = ev_ms / steps
     tfile = os.path.join(ROOT, "profiles", f"traffic_rotation{n}_iord2.json")
     roof = roofline(cells, kms, step_kernels(M, 2, kms), tfile if arith == 1 else None, tuned)
+    if not check:
+        g.close()
+        return {"ms_per_step": ms, "cell_updates_per_s": cells / (ms * 1e-3), "autotuned": tuned,
+                "roofline": roof}
     # parity on the timed grid (its tuned launch shapes), from the initial field
     g.set_field(case["c"])
     g.step(1)
@@ -496,7 +533,7 @@ def ensemble_leg(M, n):
             f"rel_err_smallest_vs_oracle_after_{taken}_steps": err}
 
 
-def large_leg(M, n, iord, arith):
+def large_leg(M, n, iord, arith, check=True):
     """Roofline of the single-grid step at n^3 (the size the 70%-of-HBM target names); one
     step against the oracle, and the whole run against the exact-arithmetic kernels."""
     import mpdata_amd.configs as K
@@ -509,13 +546,17 @@ def large_leg(M, n, iord, arith):
     c0 = g.field()
     g.autotune(2)
     g.step(1)
-    err1 = relerr(g.field(), O.per_step(n, n, n, U, V, W, iord, True, c0))
+    err1 = relerr(g.field(), O.per_step(n, n, n, U, V, W, iord, True, c0)) if check else None
     del U, V, W
-    g.step(1)
-    ms = g.step_timed(20) / 20
+    g.step(40)  # warm-up after the CPU check (clocks)
+    ms = g.step_timed(40) / 40
     kms = g.kernel_times(4)
     tuned = tuned_config(g)
-    taken = 2 + 20 + 4
+    taken = 1 + 40 + 40 + 4
+    if not check:
+        g.close()
+        return {"autotuned": tuned, "ms_per_step": ms, "cell_updates_per_s": n ** 3 / (ms * 1e-3),
+                "kernel_ms": kms}
     fast = g.field()
     g.set_arith(M.ARITH_EXACT)
     g.set_field(c0)

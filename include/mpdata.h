@@ -100,8 +100,9 @@ int mpd_sentinel_check(mpd_grid g, long long* first_bad_step);
  * kernel per stage, intermediates in HBM), 2 = fused (two kernels per corrective pass:
  * antidiffusive velocities, then limiter and limited donor pass), 3 = one kernel per
  * corrective pass (nonoscillatory grids; the donor field and the antidiffusive
- * velocities stay on chip; its z-chunk is the one mpd_set_zchunk sets); grids with more
- * than 2^29 cells per z-plane always take the staged path.  FRAME grids: 1 = the per-face
+ * velocities stay on chip); 0 picks 3 for nonoscillatory grids of at least 2^22 cells
+ * and 2 otherwise; grids with more than 2^29 cells per z-plane always take the staged
+ * path.  FRAME grids: 1 = the per-face
  * antidiffusive kernel and a separate donor pass, otherwise the tiled kernel that forms
  * the donor field and the velocities together (results are identical). */
 int mpd_set_path(mpd_grid g, int path);
@@ -126,6 +127,15 @@ int mpd_get_tiles(mpd_grid g, int* v);
 int mpd_set_graph(mpd_grid g, int on);
 /* Planes per workgroup of the fused kernels (0 = chosen from the grid size). */
 int mpd_set_zchunk(mpd_grid g, int zc);
+/* Planes per workgroup of the one-kernel-per-pass kernel (0 = chosen from the grid
+ * size: the longest chunk that still gives every CU a workgroup). */
+int mpd_set_pass_zchunk(mpd_grid g, int zc);
+/* The z-chunk the one-kernel-per-pass kernel uses (the set value or the chosen one). */
+int mpd_get_pass_zchunk(mpd_grid g, int* zc);
+/* The kernel path the grid's steps take: 1 staged (or donor only), 2 two kernels per
+ * corrective pass, 3 one kernel per corrective pass (automatic for nonoscillatory
+ * PERIODIC grids of at least 2^22 cells).  FRAME grids: 1 per-face kernels, 2 tiled. */
+int mpd_get_path(mpd_grid g, int* path);
 /* Times every tile shape x z-chunk length on the grid itself (one warm-up step and nsteps
  * timed steps each, each kernel between events of its own), then times whole steps of the
  * leading pairings (the two fastest shapes of each of the two fused kernels, and the

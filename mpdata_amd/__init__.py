@@ -83,6 +83,9 @@ SIGNATURES = {
     "mpd_set_path": (C.c_int, [C.c_void_p, C.c_int]),
     "mpd_set_tile": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "mpd_set_zchunk": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpd_set_pass_zchunk": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpd_get_path": (C.c_int, [C.c_void_p, PI]),
+    "mpd_get_pass_zchunk": (C.c_int, [C.c_void_p, PI]),
     "mpd_set_limiter_tile": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "mpd_get_tiles": (C.c_int, [C.c_void_p, PI]),
     "mpd_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
@@ -302,6 +305,22 @@ class Grid:
 
     def set_tile(self, tx, ty):
         _check(lib().mpd_set_tile(self.h, tx, ty))
+
+    def set_pass_zchunk(self, zc):
+        """z-chunk of the one-kernel-per-pass kernel (0: chosen from the size)."""
+        _check(lib().mpd_set_pass_zchunk(self.h, zc))
+
+    def pass_zchunk(self):
+        """The z-chunk the one-kernel-per-pass kernel uses."""
+        v = C.c_int(0)
+        _check(lib().mpd_get_pass_zchunk(self.h, C.byref(v)))
+        return v.value
+
+    def path(self):
+        """The kernel path the steps take (1 staged, 2 two kernels per pass, 3 one)."""
+        v = C.c_int(0)
+        _check(lib().mpd_get_path(self.h, C.byref(v)))
+        return v.value
 
     def set_zchunk(self, zc):
         _check(lib().mpd_set_zchunk(self.h, zc))

@@ -397,8 +397,9 @@ static bool graphable(const mpd_grid_s* g)
 // or copy nodes).
 static int graph_steps(mpd_grid_s* g)
 {
-    const mpd_grid_s::GraphKey key{g->tile_x,  g->tile_y,  g->zc,    g->tile2_x, g->tile2_y,
-                                   g->zc2,     g->arith,   g->path,  g->has_q ? g->q.p : nullptr};
+    const mpd_grid_s::GraphKey key{g->tile_x, g->tile_y, g->zc,    g->tile2_x, g->tile2_y,
+                                   g->zc2,    g->zc3,    g->arith, g->path,
+                                   g->has_q ? g->q.p : nullptr};
     const int par = g->cur;
     hipStream_t s = stream();
     if (!g->gexec[par] || !(g->gkey[par] == key)) {
@@ -673,6 +674,39 @@ extern "C" int mpd_set_zchunk(mpd_grid g, int zc)
     GRID_OR_FAIL(g);
     if (zc < 0) return fail(MPD_ERR_ARG, "mpd_set_zchunk: negative z-chunk length");
     g->zc = zc;
+    return MPD_OK;
+}
+
+extern "C" int mpd_set_pass_zchunk(mpd_grid g, int zc)
+{
+    GRID_OR_FAIL(g);
+    if (zc < 0) return fail(MPD_ERR_ARG, "mpd_set_pass_zchunk: negative z-chunk length");
+    g->zc3 = zc;
+    return MPD_OK;
+}
+
+namespace mpd {
+int pass_zchunk(const mpd_grid_s* g);
+}
+
+extern "C" int mpd_get_pass_zchunk(mpd_grid g, int* zc)
+{
+    GRID_OR_FAIL(g);
+    if (!zc) return fail(MPD_ERR_ARG, "mpd_get_pass_zchunk: NULL");
+    *zc = pass_zchunk(g);
+    return MPD_OK;
+}
+
+extern "C" int mpd_get_path(mpd_grid g, int* path)
+{
+    GRID_OR_FAIL(g);
+    if (!path) return fail(MPD_ERR_ARG, "mpd_get_path: NULL");
+    if (g->boundary == MPD_FRAME)
+        *path = g->path == 1 || g->iord < 2 ? 1 : 2;
+    else if (g->path == 1 || g->iord < 2 || !fused_plane_fits(g->nx, g->ny))
+        *path = 1;
+    else
+        *path = one_kernel_passes(g) ? 3 : 2;
     return MPD_OK;
 }
 

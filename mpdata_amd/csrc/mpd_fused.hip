@@ -1294,7 +1294,34 @@ int choose_zc(const mpd_grid_s* g, int tx, int ty)
     return zc;
 }
 
+// z-chunk of the one-kernel-per-pass kernel: the longest chunk (halving from the whole
+// column) that still gives every CU a workgroup -- one workgroup fills a CU's LDS, and
+// each chunk recomputes five planes of pipeline start (measured at 256^3: 0.45 ms with
+// 128-plane chunks, 0.46 with 64; at 512x512x256 256 and 128 within 1 %).
+int choose_zc_f12(const mpd_grid_s* g)
+{
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+            cus = n;
+        else
+            cus = 256;
+    }
+    const long long cols =
+        (long long)((g->nx + F12_TX - 1) / F12_TX) * ((g->ny + F12_TY - 1) / F12_TY);
+    int zc = g->nz;
+    while (zc > 16 && cols * ((g->nz + zc - 1) / zc) < cus) zc = (zc + 1) / 2;
+    return zc;
+}
+
 }  // namespace
+
+#ifdef MPD_FAST_DIV
+// (one definition: the fast build's)
+int pass_zchunk(const mpd_grid_s* g) { return g->zc3 ? g->zc3 : choose_zc_f12(g); }
+#endif
 
 int MPD_CAT(fused_batch_launch, MPD_ARITH)(int tx, int ty, bool f1, bool donor,
                                            const FusedArgs* table, const int* item0, int ninst,
@@ -1382,10 +1409,10 @@ int MPD_CAT(periodic_fused_step, MPD_ARITH)(mpd_grid_s* g)
     double* out = g->c[1 - g->cur].p;
     const double* prev[3] = {g->face[0].p, g->face[1].p, g->face[2].p};
     const double* field = cin;
-    if (g->nonosc && g->path == 3) {
+    if (one_kernel_passes(g)) {
         // one kernel per corrective pass; the limited faces of pass p (read by pass p+1)
         // alternate between the two face sets
-        const int zc = g->zc ? g->zc : choose_zc(g, F12_TX, F12_TY);
+        const int zc = g->zc3 ? g->zc3 : choose_zc_f12(g);
         a.zc = zc;
         fused_geometry(a, F12_TX, F12_TY);
         const int nb = fused_items(a);

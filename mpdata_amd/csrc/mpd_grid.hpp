@@ -19,6 +19,7 @@ struct mpd_grid_s {
     // tile shape and z-chunk of the limiter kernel F2 (0: those of F1); the two kernels of
     // a pass share no on-chip state, so each runs in the launch shape that suits it
     int tile2_x = 0, tile2_y = 0, zc2 = 0;
+    int zc3 = 0;  // z-chunk of the one-kernel-per-pass kernel (0: chosen from the size)
     size_t ncell = 0;
     size_t nface[3] = {0, 0, 0};  // elements per face family (layout of the mode)
 
@@ -51,12 +52,13 @@ struct mpd_grid_s {
     // time loop captured as a HIP graph (PERIODIC fused path): a block of steps from each
     // field parity, with the launch configuration it was captured with
     struct GraphKey {
-        int tx, ty, zc, tx2, ty2, zc2, arith, path;
+        int tx, ty, zc, tx2, ty2, zc2, zc3, arith, path;
         const double* q;
         bool operator==(const GraphKey& o) const
         {
             return tx == o.tx && ty == o.ty && zc == o.zc && tx2 == o.tx2 && ty2 == o.ty2 &&
-                   zc2 == o.zc2 && arith == o.arith && path == o.path && q == o.q;
+                   zc2 == o.zc2 && zc3 == o.zc3 && arith == o.arith && path == o.path &&
+                   q == o.q;
         }
     };
     // launch stream of the fused step: the library's stream unless a caller supplies its
@@ -94,6 +96,14 @@ int field_stats(mpd_grid_s* g, double* mass, double* cmin, double* cmax);
 int sentinel_check_launch(mpd_grid_s* g);
 // Records a timing event after a kernel launch when g->trace is set.
 void trace_mark(mpd_grid_s* g);
+// Whether a PERIODIC grid's corrective passes run as one kernel each: path 3, or the
+// automatic path on nonoscillatory grids too large for the Infinity Cache (measured on
+// MI355X: 256^3 0.52 -> 0.45 ms, 512x512x256 iord 3 4.29 -> 3.55 ms per step; at 128^3
+// the two-kernel path is faster, 0.067 against 0.072 ms).
+inline bool one_kernel_passes(const mpd_grid_s* g)
+{
+    return g->nonosc && (g->path == 3 || (g->path == 0 && g->ncell >= (size_t(1) << 22)));
+}
 // The stream a grid's fused step launches on.
 inline hipStream_t launch_stream(const mpd_grid_s* g)
 {

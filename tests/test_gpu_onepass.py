@@ -56,7 +56,7 @@ def test_one_kernel_per_pass_matches_oracle(dims, arith):
                 g = M.Grid(nx, ny, nz, M.PERIODIC, iord=iord, nonosc=True)
                 g.set_path(3)
                 g.set_arith(arith)
-                g.set_zchunk(zc)
+                g.set_pass_zchunk(zc)
                 g.set_courant(U, V, W)
                 g.set_field(c)
                 if with_q:
@@ -74,3 +74,37 @@ def test_one_kernel_per_pass_matches_oracle(dims, arith):
                         e = relerr(got, want)
                         assert e <= TOL, f"step {s}: rel err {e:.3e}"
                 g.close()
+
+
+def test_automatic_path_selection_and_agreement():
+    """The automatic path takes one kernel per pass for nonoscillatory grids of at least
+    2^22 cells and two kernels otherwise; at such a size the two paths agree bit for bit
+    in exact arithmetic, and the fast one-kernel path with the chosen z-chunk stays
+    within 1e-12 of them."""
+    nx, ny, nz = 256, 128, 128  # 2^22 cells
+    U, V, W, c, q = case(nx, ny, nz, 5)
+    out = {}
+    for path in (0, 2, 3):
+        for arith in (M.ARITH_EXACT, M.ARITH_FAST):
+            print(f"case auto-path check path={path} arith={arith}", flush=True)
+            g = M.Grid(nx, ny, nz, M.PERIODIC, iord=3, nonosc=True)
+            g.set_path(path)
+            g.set_arith(arith)
+            g.set_courant(U, V, W)
+            g.set_field(c)
+            g.set_emission(q)
+            assert g.path() == (3 if path == 0 else path)
+            if g.path() == 3:
+                assert 16 <= g.pass_zchunk() <= nz
+            g.step(2)
+            out[(path, arith)] = g.field()
+            g.close()
+    np.testing.assert_array_equal(out[(0, M.ARITH_EXACT)], out[(2, M.ARITH_EXACT)])
+    np.testing.assert_array_equal(out[(3, M.ARITH_EXACT)], out[(2, M.ARITH_EXACT)])
+    assert relerr(out[(0, M.ARITH_FAST)], out[(2, M.ARITH_EXACT)]) <= 2 * TOL
+    for dims, nonosc, want in (((128, 128, 128), True, 2), ((256, 128, 128), False, 2)):
+        g = M.Grid(*dims, M.PERIODIC, iord=2, nonosc=nonosc)
+        assert g.path() == want
+        g.set_path(3)
+        assert g.path() == (3 if nonosc else 2)
+        g.close()
