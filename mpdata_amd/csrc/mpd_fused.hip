@@ -803,7 +803,10 @@ struct Col3 {
     double bu_m4 = 0, bd_m4 = 0, bu_m3 = 0, bd_m3 = 0;                   // own ratios
 };
 
-template <int TX, int TY, bool DONOR, bool NT>
+// SAVE (first pass of an iord-2 step only): also stores the adjoint's forward record of
+// every tile cell -- c1, the unlimited faces A, the unclamped limiter ratios and the bound
+// codes (see FusedArgs) -- the values the two-kernel storing step records, bit for bit.
+template <int TX, int TY, bool DONOR, bool NT, bool SAVE>
 __device__ __forceinline__ void f12_tile(const FusedArgs& a, int item)
 {
     using G = Geo3<TX, TY>;
@@ -833,6 +836,7 @@ __device__ __forceinline__ void f12_tile(const FusedArgs& a, int item)
     // planes p+2, p+1 (loads), p-2 (emission of the next output plane), p-3 (stores)
     ZCursor zp2(pstart + 2, a.nz, a.np), zp1(pstart + 1, a.nz, a.np),
         zm2(pstart - 2, a.nz, a.np), zm3(pstart - 3, a.nz, a.np);
+    ZCursor zp0(pstart, a.nz, a.np), zm1(pstart - 1, a.nz, a.np);  // records (SAVE)
     auto first_loads = [&](Col3& C) __attribute__((always_inline)) {
         ZCursor zm1(pstart - 1, a.nz, a.np), z0c(pstart, a.nz, a.np);
         if (DONOR) {
@@ -906,6 +910,8 @@ __device__ __forceinline__ void f12_tile(const FusedArgs& a, int item)
                                    T.cn_p1, up[L - 1], T.u_0, vp[L - RX], T.v_0, T.w_m1, T.w_0)
                            : T.fld;
             s_c1[s4][L] = T.c1_0;
+            if constexpr (SAVE)
+                if (store && p >= z0 && p < z1) put<NT>(a.c1out + zp0.off, T.colb, T.c1_0);
             if (h2) {
                 H.c1_0 = DONOR ? donor(H.cn_0, cn[HL - 1], cn[HL + 1], cn[HL - RX], cn[HL + RX],
                                        H.cn_m1, H.cn_p1, up[HL - 1], H.u_0, vp[HL - RX], H.v_0,
@@ -1022,7 +1028,7 @@ __device__ __forceinline__ void f12_tile(const FusedArgs& a, int item)
         const double* axm = s_ax[ma];
         const double* aym = s_ay[ma];
         auto ratios = [&](int M, const Col3& C, double& bu, double& bd, double& axl,
-                          double& ayd) __attribute__((always_inline)) {
+                          double& ayd, bool rec) __attribute__((always_inline)) {
             const double c = C.c1_m2;
             const double xl = c1m[M - 1], xr = c1m[M + 1], yd = c1m[M - RX], yu = c1m[M + RX];
             const double hi =
@@ -1041,6 +1047,31 @@ __device__ __forceinline__ void f12_tile(const FusedArgs& a, int item)
             const double fyl = 0.5 * F2(yd, c, ayd), fyr = 0.5 * F2(c, yu, C.ay_m2);
             const double fzl = 0.5 * F2(C.c1_m3, c, C.az_m3), fzr = 0.5 * F2(c, C.c1_m1, C.az_m2);
             beta(hi, lo, c, fxl, fxr, fyl, fyr, fzl, fzr, bu, bd);
+            if constexpr (SAVE) {
+                if (rec && store && m >= z0 && m < z1) {
+                    // which element holds each bound: element 2q + s is the c^n (s = 0) or
+                    // pass-input (s = 1) value of neighbour q in {self, x-, x+, y-, y+, z-,
+                    // z+}; of the elements equal to the bound the last one counts
+                    const double nbn[7] = {C.cn_m2,     cnm[M - 1], cnm[M + 1], cnm[M - RX],
+                                           cnm[M + RX], C.cn_m3,    C.cn_m1};
+                    const double nbp[7] = {c, xl, xr, yd, yu, C.c1_m3, C.c1_m1};
+                    int hc = 0, lc = 0;
+#pragma unroll
+                    for (int q = 0; q < 7; q++) {
+                        hc = nbn[q] == hi ? 2 * q : hc;
+                        lc = nbn[q] == lo ? 2 * q : lc;
+                        hc = nbp[q] == hi ? 2 * q + 1 : hc;
+                        lc = nbp[q] == lo ? 2 * q + 1 : lc;
+                    }
+                    put<NT>(a.bu + zm2.off, T.colb, bu);
+                    put<NT>(a.bd + zm2.off, T.colb, bd);
+                    int* cp_ = (int*)((char*)(a.code + zm2.off) + T.colb / 2);
+                    if constexpr (NT)
+                        __builtin_nontemporal_store(hc | (lc << 8), cp_);
+                    else
+                        *cp_ = hc | (lc << 8);
+                }
+            }
             bu = mn2(bu, 1.0);
             bd = mn2(bd, 1.0);
         };
@@ -1050,9 +1081,15 @@ __device__ __forceinline__ void f12_tile(const FusedArgs& a, int item)
             faces(L, T, wkr, wku, true, true, true, ax_k, ay_k, az_k);
             s_ax[ka][L] = ax_k;
             s_ay[ka][L] = ay_k;
+            if constexpr (SAVE)
+                if (store && k >= z0 && k < z1) {
+                    put<NT>(a.Ax + zm1.off, T.colb, ax_k);
+                    put<NT>(a.Ay + zm1.off, T.colb, ay_k);
+                    put<NT>(a.Az + zm1.off, T.colb, az_k);
+                }
         }
         if (doB) {
-            ratios(L, T, bu_m, bd_m, axl_m, ayd_m);
+            ratios(L, T, bu_m, bd_m, axl_m, ayd_m, true);
             s_bu[ma][L] = bu_m;
             s_bd[ma][L] = bd_m;
         }
@@ -1088,7 +1125,7 @@ __device__ __forceinline__ void f12_tile(const FusedArgs& a, int item)
         }
         if (doB && h1) {
             double hbu, hbd, dx, dy;
-            ratios(HL, H, hbu, hbd, dx, dy);
+            ratios(HL, H, hbu, hbd, dx, dy, false);
             s_bu[ma][HL] = hbu;
             s_bd[ma][HL] = hbd;
         }
@@ -1129,6 +1166,8 @@ __device__ __forceinline__ void f12_tile(const FusedArgs& a, int item)
         zp1.next();
         zm2.next();
         zm3.next();
+        zp0.next();
+        zm1.next();
     };
     using Steady = std::integral_constant<bool, true>;
     using Edge = std::integral_constant<bool, false>;
@@ -1145,7 +1184,7 @@ __device__ __forceinline__ void f12_tile(const FusedArgs& a, int item)
     for (; pl <= pend; pl++) plane(pl, Edge());
 }
 
-template <int TX, int TY, bool DONOR, bool NT>
+template <int TX, int TY, bool DONOR, bool NT, bool SAVE>
 __global__ void __launch_bounds__((Geo3<TX, TY>::NT)) k_f12(FusedArgs a, Work w)
 {
     for (int it = blockIdx.x; it < w.nitems; it += gridDim.x) {
@@ -1157,7 +1196,7 @@ __global__ void __launch_bounds__((Geo3<TX, TY>::NT)) k_f12(FusedArgs a, Work w)
             __builtin_memcpy(&x, (const FusedArgs*)((ConstArgs*)w.table + i), sizeof x);
             item -= item0[i];
         }
-        f12_tile<TX, TY, DONOR, NT>(x, item);
+        f12_tile<TX, TY, DONOR, NT, SAVE>(x, item);
         lds_barrier();
     }
 }
@@ -1223,16 +1262,22 @@ int launch_f12(const FusedArgs& a, const Work& w, int grid, bool donor, hipStrea
 {
     constexpr int TX = F12_TX, TY = F12_TY;
     const dim3 blk(Geo3<TX, TY>::NT);
-    if (donor) {
+    if (a.bu) {  // the adjoint's record (first pass only)
+        if (!donor) return fail(MPD_ERR_UNSUPPORTED, "records of the first pass only");
         if (a.nt)
-            launch_k(k_f12<TX, TY, true, true>, dim3(grid), blk, s, ev, a, w);
+            launch_k(k_f12<TX, TY, true, true, true>, dim3(grid), blk, s, ev, a, w);
         else
-            launch_k(k_f12<TX, TY, true, false>, dim3(grid), blk, s, ev, a, w);
+            launch_k(k_f12<TX, TY, true, false, true>, dim3(grid), blk, s, ev, a, w);
+    } else if (donor) {
+        if (a.nt)
+            launch_k(k_f12<TX, TY, true, true, false>, dim3(grid), blk, s, ev, a, w);
+        else
+            launch_k(k_f12<TX, TY, true, false, false>, dim3(grid), blk, s, ev, a, w);
     } else {
         if (a.nt)
-            launch_k(k_f12<TX, TY, false, true>, dim3(grid), blk, s, ev, a, w);
+            launch_k(k_f12<TX, TY, false, true, false>, dim3(grid), blk, s, ev, a, w);
         else
-            launch_k(k_f12<TX, TY, false, false>, dim3(grid), blk, s, ev, a, w);
+            launch_k(k_f12<TX, TY, false, false, false>, dim3(grid), blk, s, ev, a, w);
     }
     MPD_HIP(hipGetLastError());
     return MPD_OK;
@@ -1370,6 +1415,20 @@ int MPD_CAT(fused_store_step, MPD_ARITH)(mpd_grid_s* g, const double* cin, doubl
     a.Ax = A[0];
     a.Ay = A[1];
     a.Az = A[2];
+    if (one_kernel_passes(g)) {
+        // the grid's steps take one kernel per pass: so does its record step
+        a.cn = cin;
+        a.q = g->has_q ? g->q.p : nullptr;
+        a.cout = cout;
+        a.bu = bu;
+        a.bd = bd;
+        a.code = code;
+        a.zc = g->zc3 ? g->zc3 : choose_zc_f12(g);
+        fused_geometry(a, F12_TX, F12_TY);
+        const int nb = fused_items(a);
+        const Work w{nullptr, nullptr, 0, nb, nb % 8 == 0 ? 1 : 0};
+        return launch_f12(a, w, nb, true, s, nullptr);
+    }
     if ((rc = launch(a, tx, ty, true, true, s, nullptr))) return rc;
     const double* q = g->has_q ? g->q.p : nullptr;
     if (g->nonosc) {

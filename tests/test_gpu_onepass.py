@@ -108,3 +108,41 @@ def test_automatic_path_selection_and_agreement():
         g.set_path(3)
         assert g.path() == (3 if nonosc else 2)
         g.close()
+
+
+@pytest.mark.parametrize("dims", [(40, 20, 10), (33, 17, 5), (7, 6, 4)])
+def test_one_kernel_records_give_the_same_adjoint(dims):
+    """The adjoint's forward records (c1, the unlimited faces, the unclamped ratios, the
+    bound codes) taken from the one-kernel storing pass give the adjoint step and the
+    checkpointed receptor gradient of the two-kernel records bit for bit, with several
+    z-chunks (records on chunk boundaries)."""
+    nx, ny, nz = dims
+    U, V, W, c, q = case(nx, ny, nz, 900 + nx)
+    rng = np.random.default_rng(nx)
+    lam = rng.standard_normal(c.size)
+    rec = rng.choice(c.size, size=min(10, c.size), replace=False).astype(np.int32)
+    nsteps, every = 12, 3
+    obs = rng.uniform(0, 1, (nsteps // every, rec.size))
+    out = []
+    for path, zc in ((2, 0), (3, 1), (3, 3), (3, 0)):
+        print(f"case records dims={dims} path={path} zc={zc}", flush=True)
+        g = M.Grid(nx, ny, nz, M.PERIODIC, iord=2, nonosc=True)
+        g.set_arith(M.ARITH_EXACT)
+        g.set_path(path)
+        g.set_pass_zchunk(zc)
+        g.set_courant(U, V, W)
+        g.set_emission(q)
+        lq = np.zeros(c.size)
+        lin = g.adjoint_step(c, lam, lq)
+        g.set_field(c)
+        J, g0, gq, _ = g.receptor_gradient(nsteps, 4, rec, every, obs)
+        g.close()
+        out.append((lin, lq, J, g0, gq))
+    for o in out[1:]:
+        np.testing.assert_array_equal(o[0], out[0][0])
+        np.testing.assert_array_equal(o[1], out[0][1])
+        assert o[2] == out[0][2]
+        np.testing.assert_array_equal(o[3], out[0][3])
+        np.testing.assert_array_equal(o[4], out[0][4])
+    want = O.adj_step(nx, ny, nz, U, V, W, True, c, lam)
+    assert relerr(out[0][0], want) <= 1e-12
