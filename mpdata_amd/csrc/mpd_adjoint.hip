@@ -428,7 +428,7 @@ k_adj_lim_t(LimArgs a)
 {
     using G = tile::GeoB<TX, TY>;
     using tile::ld;
-    using tile::st;
+    using tile::st_nt;  // streaming: the adjoint's arrays far exceed the caches at its sizes
     constexpr int RX = G::RX, R = G::R;
     __shared__ double s_lo[R], s_bu[R], s_bd[R], s_c[3][R], s_a0[3][R], s_a1[3][R];
     __shared__ double s_iu[2][R], s_id[2][R], s_li[2][R], s_lx[2][R];
@@ -638,7 +638,7 @@ k_adj_lim_t(LimArgs a)
             s_li[s2][L] = li;
             s_lx[s2][L] = lx;
         }
-        if (store && p >= z0 && p < z1 && a.lq) st(a.lq + zp0.off, colb, lq_p + P.lo[1]);
+        if (store && p >= z0 && p < z1 && a.lq) st_nt(a.lq + zp0.off, colb, lq_p + P.lo[1]);
         tile::lds_barrier();
         // stage 2 on the tile cell of plane k = p - 1 (k_adj_limit3, same operation order)
         const int k = p - 1;
@@ -721,11 +721,11 @@ k_adj_lim_t(LimArgs a)
                 }
             }
             if (store) {
-                st(a.lin + zm1.off, colb, s0);
-                st(a.l1 + zm1.off, colb, dacc_m + (s1 + acc));
-                st(a.lA0 + zm1.off, colb, lAf[0]);
-                st(a.lA1 + zm1.off, colb, lAf[1]);
-                st(a.lA2 + zm1.off, colb, lAf[2]);
+                st_nt(a.lin + zm1.off, colb, s0);
+                st_nt(a.l1 + zm1.off, colb, dacc_m + (s1 + acc));
+                st_nt(a.lA0 + zm1.off, colb, lAf[0]);
+                st_nt(a.lA1 + zm1.off, colb, lAf[1]);
+                st_nt(a.lA2 + zm1.off, colb, lAf[2]);
             }
         }
         // shift the tile cell's history by one plane: plane p - 1 becomes p - 2 (its LDS
@@ -860,7 +860,7 @@ __global__ void __launch_bounds__((tile::GeoB<TX, TY>::NTH)) k_adj_anti_t(AntiAr
 {
     using G = tile::GeoB<TX, TY>;
     using tile::ld;
-    using tile::st;
+    using tile::st_nt;  // streaming: the adjoint's arrays far exceed the caches at its sizes
     constexpr int RX = G::RX, R = G::R;
     __shared__ double s_c[3][R], s_u[2][R], s_v[2][R], s_w[2][R];
     __shared__ double s_k[14][R];
@@ -1017,7 +1017,7 @@ __global__ void __launch_bounds__((tile::GeoB<TX, TY>::NTH)) k_adj_anti_t(AntiAr
             accc += kz.km + zu1 + zd1 + zu2 + zd2;
             accn += (kx.ku2 + s_k[3][L - 1]) + (ky.ku2 + s_k[8][L - RX]) + kz.kp + zu1 + zd1 +
                     zu2 + zd2;
-            if (store && p - 1 >= z0 && p - 1 < z1) st(a.l1 + zm1.off, colb, l1m + accm);
+            if (store && p - 1 >= z0 && p - 1 < z1) st_nt(a.l1 + zm1.off, colb, l1m + accm);
             accm = accc;
             accc = accn;
             accn = 0.0;
