@@ -502,9 +502,11 @@ def ensemble_leg(M, n):
     ms_per = b.step_timed(10, M.BATCH_PER_INSTANCE) / 10
     taken = 2 + 100 + 10 + 1 + 10
     mass_ok = all(np.isfinite(g.stats()[0]) for g in b.instances[:4])
-    # every mode gives the bits of a single grid (default launch shape), and the smallest
-    # instance stays within the tolerance of the oracle over the whole run
-    same, err = True, None
+    # against a single grid of each size stepped alone (its own path: two kernels per pass
+    # at these sizes; the batched graph takes one, so the fast arithmetic agrees to
+    # rounding), and the smallest instance against the oracle over the whole run
+    worst, err = 0.0, None
+    path = b.path()
     for i in firsts:
         (U, V, W), c = start[i]
         got = b.instances[i].field()
@@ -512,7 +514,7 @@ def ensemble_leg(M, n):
         g.set_courant(U, V, W)
         g.set_field(c)
         g.step(taken)
-        same = same and bool(np.array_equal(got, g.field()))
+        worst = max(worst, relerr(got, g.field()))
         g.close()
         if dims[i] == min(dims, key=lambda d: d[0] * d[1] * d[2]):
             err = relerr(got, O.per_run(*dims[i], U, V, W, 2, True, c, taken))
@@ -528,8 +530,9 @@ def ensemble_leg(M, n):
             "graph_cell_updates_per_s": cells / (ms_graph * 1e-3),
             "per_instance_cell_updates_per_s": cells / (ms_per * 1e-3),
             "speedup_graph_vs_per_instance": ms_per / ms_graph, "finite": mass_ok,
-            f"bitwise_vs_single_grids_after_{taken}_steps":
-                {"instances": firsts, "equal": same},
+            "batched_path": "one kernel per corrective pass" if path == 3 else "two kernels per pass",
+            f"rel_err_vs_single_grids_after_{taken}_steps":
+                {"instances": firsts, "max": worst},
             f"rel_err_smallest_vs_oracle_after_{taken}_steps": err}
 
 

@@ -256,6 +256,15 @@ int mpd_batch_set_limiter_tile(mpd_batch b, int tile_x, int tile_y, int zc);
 /* v[0..3]: tile_x, tile_y, zc, persistent of the first kernel; v[4..6]: the limiter
  * kernel's tile_x, tile_y, zc (0 = the first kernel's). */
 int mpd_batch_get_tiles(mpd_batch b, int* v);
+/* Kernel path of the batched launches: 0 automatic (one kernel per corrective pass when
+ * the ensemble has 2^22 cells or more, as for single grids), 2 two kernels per pass, 3
+ * one kernel per pass.  Per-instance launches follow each instance's own path.  The
+ * results agree to rounding (bit for bit in exact arithmetic). */
+int mpd_batch_set_path(mpd_batch b, int path);
+/* The path the batched launches take (2 or 3). */
+int mpd_batch_get_path(mpd_batch b, int* path);
+/* z-chunk of the one-kernel batched launches (0: min(nz, 64) per instance). */
+int mpd_batch_set_pass_zchunk(mpd_batch b, int zc);
 /* Times every tile shape x z-chunk x grid kind over nsteps steps on the batch itself
  * (fields restored afterwards), applies and returns the fastest (any output may be
  * NULL).  Cached per process; also in the file named by MPD_TUNE_CACHE when set. */

@@ -111,6 +111,9 @@ SIGNATURES = {
     "mpd_batch_set_tile": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "mpd_batch_set_limiter_tile": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "mpd_batch_get_tiles": (C.c_int, [C.c_void_p, PI]),
+    "mpd_batch_set_path": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpd_batch_get_path": (C.c_int, [C.c_void_p, PI]),
+    "mpd_batch_set_pass_zchunk": (C.c_int, [C.c_void_p, C.c_int]),
     "mpd_batch_autotune": (C.c_int, [C.c_void_p, C.c_int, PI, PI, PI, PI]),
     "mpd_batch_step": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "mpd_batch_step_timed": (C.c_int, [C.c_void_p, C.c_int, C.c_int, PD]),
@@ -457,6 +460,18 @@ class Batch:
         v = (C.c_int * 7)()
         _check(lib().mpd_batch_get_tiles(self.h, v))
         return tuple(v[0:4]), tuple(v[4:7])
+
+    def set_path(self, path):
+        """0 automatic, 2 two kernels per corrective pass, 3 one kernel per pass."""
+        _check(lib().mpd_batch_set_path(self.h, path))
+
+    def path(self):
+        v = C.c_int(0)
+        _check(lib().mpd_batch_get_path(self.h, C.byref(v)))
+        return v.value
+
+    def set_pass_zchunk(self, zc):
+        _check(lib().mpd_batch_set_pass_zchunk(self.h, zc))
 
     def autotune(self, nsteps=2):
         v = [C.c_int() for _ in range(4)]

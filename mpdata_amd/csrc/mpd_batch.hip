@@ -11,6 +11,8 @@
 //                           over every (instance, tile, z-chunk) work item.
 //   MPD_BATCH_GRAPH         the MPD_BATCH_FUSED time loop captured once in a hipGraph
 //                           and replayed (no per-step host launch work).
+// The batched launches run one kernel per corrective pass (k_f12) when the ensemble is
+// larger than the Infinity Cache, else the two kernels F1, F2.
 // Tile shape, z-chunk and grid size come from the built-in autotuner (timed trials on
 // the batch itself), or are set explicitly.
 #include <algorithm>
@@ -35,6 +37,8 @@ struct mpd_batch_s {
     int ninst = 0, iord = 2, nonosc = 1, arith = MPD_ARITH_FAST;
     int tx = 32, ty = 8, zc = 16, persistent = 1;
     int tx2 = 0, ty2 = 0, zc2 = 0;  // the limiter kernel's launch shape (0: F1's)
+    int path = 0;  // 0 automatic, 2 two kernels per pass, 3 one kernel per pass
+    int zc3 = 0;   // z-chunk of the one-kernel launches (0: min(nz, 64) per instance)
     int cur = 0;
     long long cells = 0;
     std::vector<mpd_grid_s*> g;
@@ -64,6 +68,14 @@ struct mpd_batch_s {
 
 namespace {
 
+// Whether the batched launches run one kernel per corrective pass: path 3, or the
+// automatic path when the ensemble's arrays exceed the Infinity Cache (2^22 cells), as
+// for single grids.  Per-instance launches follow each instance's own path.
+bool one_kernel(const mpd_batch_s* b)
+{
+    return b->nonosc && (b->path == 3 || (b->path == 0 && b->cells >= (1LL << 22)));
+}
+
 int table_index(const mpd_batch_s* b, int parity, int pass, int f2, int inst)
 {
     return ((parity * (b->iord - 1) + (pass - 1)) * 2 + f2) * b->ninst + inst;
@@ -73,7 +85,8 @@ int table_index(const mpd_batch_s* b, int parity, int pass, int f2, int inst)
 long long config_key(const mpd_batch_s* b)
 {
     long long k = ((b->tx * 131LL + b->ty) * 131 + b->zc) * 4 + b->persistent * 2 + b->arith;
-    return ((k * 131 + b->tx2) * 131 + b->ty2) * 131 + b->zc2;
+    k = ((k * 131 + b->tx2) * 131 + b->ty2) * 131 + b->zc2;
+    return (k * 2 + (one_kernel(b) ? 1 : 0)) * 1031 + b->zc3;
 }
 
 // Launch shape of the first kernel (f2 = false) or of the limiter kernel (f2 = true).
@@ -96,9 +109,49 @@ int build_tables(mpd_batch_s* b)
     int items = 0, items2 = 0;
     int tx2, ty2, zc2;
     shape_of(b, true, tx2, ty2, zc2);
+    const bool one = one_kernel(b);
     for (int i = 0; i < b->ninst; i++) {
         mpd_grid_s* g = b->g[i];
         if ((rc = ensure_fused_scratch(g))) return rc;
+        if (one) {
+            // one descriptor per pass in the F1 slot: the kernel's own tiling, the buffer
+            // roles of the single-grid one-kernel step
+            FusedArgs geo{};
+            geo.nx = g->nx;
+            geo.ny = g->ny;
+            geo.nz = g->nz;
+            geo.np = (long long)g->nx * g->ny;
+            geo.nt = fused_streaming((size_t)b->cells);
+            geo.zc = std::min(b->zc3 ? b->zc3 : 64, g->nz);
+            fused_geometry(geo, F12_TILE_X, F12_TILE_Y);
+            b->hitem0[i] = items;
+            items += fused_items(geo);
+            for (int par = 0; par < 2; par++) {
+                const double* cin = g->c[par].p;
+                double* out = g->c[1 - par].p;
+                const double* prev[3] = {g->face[0].p, g->face[1].p, g->face[2].p};
+                const double* field = cin;
+                for (int p = 1; p <= npass; p++) {
+                    const bool last = (p == npass);
+                    mpd::DevBuf* lf = (p & 1) ? g->fb : g->fa;
+                    FusedArgs f = geo;
+                    f.cn = cin;
+                    f.cin = field;
+                    f.U = prev[0];
+                    f.V = prev[1];
+                    f.W = prev[2];
+                    f.q = (last && g->has_q) ? g->q.p : nullptr;
+                    f.cout = last ? out : g->cp[p & 1].p;
+                    f.Lx = last ? nullptr : lf[0].p;
+                    f.Ly = last ? nullptr : lf[1].p;
+                    f.Lz = last ? nullptr : lf[2].p;
+                    b->htab[table_index(b, par, p, 0, i)] = f;
+                    for (int d = 0; d < 3; d++) prev[d] = lf[d].p;
+                    field = f.cout;
+                }
+            }
+            continue;
+        }
         FusedArgs geo{};
         // streaming stores when the whole batch's arrays exceed the caches
         geo.nt = fused_streaming((size_t)b->cells);
@@ -195,6 +248,16 @@ int launch_step(mpd_batch_s* b, int par)
     hipStream_t s = stream();
     int rc;
     auto launch = b->arith == MPD_ARITH_FAST ? fused_batch_launch_fast : fused_batch_launch_exact;
+    if (one_kernel(b)) {
+        auto launch1 = b->arith == MPD_ARITH_FAST ? fused_batch_launch_f12_fast
+                                                  : fused_batch_launch_f12_exact;
+        const bool nt = fused_streaming((size_t)b->cells);
+        for (int p = 1; p < b->iord; p++)
+            if ((rc = launch1(b->dtab + table_index(b, par, p, 0, 0), b->ditem0, b->ninst,
+                              b->nitems, p == 1, nt, s)))
+                return rc;
+        return MPD_OK;
+    }
     for (int p = 1; p < b->iord; p++) {
         const FusedArgs* t1 = b->dtab + table_index(b, par, p, 0, 0);
         const FusedArgs* t2 = b->dtab + table_index(b, par, p, 1, 0);
@@ -395,6 +458,31 @@ extern "C" int mpd_batch_set_limiter_tile(mpd_batch b, int tx, int ty, int zc)
     return MPD_OK;
 }
 
+extern "C" int mpd_batch_set_path(mpd_batch b, int path)
+{
+    BATCH_OR_FAIL(b);
+    if (path != 0 && path != 2 && path != 3)
+        return fail(MPD_ERR_ARG, "mpd_batch_set_path: %d is not 0, 2 or 3", path);
+    b->path = path;
+    return MPD_OK;
+}
+
+extern "C" int mpd_batch_get_path(mpd_batch b, int* path)
+{
+    BATCH_OR_FAIL(b);
+    if (!path) return fail(MPD_ERR_ARG, "mpd_batch_get_path: NULL");
+    *path = one_kernel(b) ? 3 : 2;
+    return MPD_OK;
+}
+
+extern "C" int mpd_batch_set_pass_zchunk(mpd_batch b, int zc)
+{
+    BATCH_OR_FAIL(b);
+    if (zc < 0) return fail(MPD_ERR_ARG, "mpd_batch_set_pass_zchunk: negative");
+    b->zc3 = zc;
+    return MPD_OK;
+}
+
 extern "C" int mpd_batch_get_tiles(mpd_batch b, int* v)
 {
     BATCH_OR_FAIL(b);
@@ -476,6 +564,15 @@ extern "C" int mpd_batch_autotune(mpd_batch b, int nsteps, int* tx, int* ty, int
     const std::string key = "batch2:" + opts;
     int rc;
     int c[7];
+    if (one_kernel(b)) {
+        // the one-kernel launches have a single tile shape; nothing to tune
+        if (tx) *tx = b->tx;
+        if (ty) *ty = b->ty;
+        if (zc) *zc = b->zc;
+        if (persistent) *persistent = b->persistent;
+        b->tuned = true;
+        return MPD_OK;
+    }
     if (!tune_lookup(key, 7, c)) {
         if (tune_lookup("batch:" + opts, 4, c)) {
             c[4] = c[5] = c[6] = 0;

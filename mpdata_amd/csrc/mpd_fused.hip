@@ -1255,7 +1255,7 @@ int launch_any(const FusedArgs& a, const Work& w, int grid, int tx, int ty, bool
 }
 
 // Tile shapes of the one-kernel-per-pass path (LDS: 23 planes of the tile + 3-cell ring).
-constexpr int F12_TX = 32, F12_TY = 16;
+constexpr int F12_TX = F12_TILE_X, F12_TY = F12_TILE_Y;
 
 int launch_f12(const FusedArgs& a, const Work& w, int grid, bool donor, hipStream_t s,
                const hipEvent_t* ev)
@@ -1376,6 +1376,16 @@ int MPD_CAT(fused_batch_launch, MPD_ARITH)(int tx, int ty, bool f1, bool donor,
     FusedArgs none{};
     none.nt = nt;
     return launch_any(none, w, grid, tx, ty, f1, donor, s);
+}
+
+int MPD_CAT(fused_batch_launch_f12, MPD_ARITH)(const FusedArgs* table, const int* item0,
+                                               int ninst, int nitems, bool donor, bool nt,
+                                               hipStream_t s)
+{
+    Work w{table, item0, ninst, nitems, nitems % 8 == 0 ? 1 : 0};
+    FusedArgs none{};
+    none.nt = nt;
+    return launch_f12(none, w, nitems, donor, s, nullptr);
 }
 
 int MPD_CAT(fused_resident_blocks, MPD_ARITH)(int tx, int ty, bool f1)

@@ -118,6 +118,51 @@ def test_batch_modes_agree_and_match_oracle(iord, arith):
             assert relerr(got, want) <= 1e-12
 
 
+@pytest.mark.parametrize("iord", [2, 3])
+@pytest.mark.parametrize("arith", [M.ARITH_EXACT, M.ARITH_FAST])
+def test_one_kernel_batched_launches(iord, arith):
+    """Batched launches with one kernel per corrective pass (forced on a small batch):
+    fused launches and graph replays, several z-chunks and emission on some instances,
+    against per-instance launches and the oracle -- bit for bit in exact arithmetic,
+    1e-12 in fast arithmetic (the two paths contract differently)."""
+    rng = np.random.default_rng(iord)
+    qs = [rng.uniform(0, 1e-3, nx * ny * nz) if i % 2 else None
+          for i, (nx, ny, nz) in enumerate(DIMS)]
+    results = []
+    for mode, zc in ((M.BATCH_PER_INSTANCE, 0), (M.BATCH_FUSED, 0), (M.BATCH_FUSED, 3),
+                     (M.BATCH_GRAPH, 5), (M.BATCH_GRAPH, 0)):
+        print(f"case one-kernel batch iord={iord} arith={arith} mode={mode} zc={zc}",
+              flush=True)
+        b, init = make_batch(iord, arith)
+        for g, q in zip(b.instances, qs):
+            if q is not None:
+                g.set_emission(q)
+        b.set_path(3)
+        b.set_pass_zchunk(zc)
+        assert b.path() == 3
+        b.step(3, mode)
+        if mode == M.BATCH_GRAPH:
+            b.step(2, mode)  # a second replay from the other parity
+            b.step(2, M.BATCH_FUSED)
+        else:
+            b.step(4, mode)
+        results.append([g.field() for g in b.instances])
+        b.close()
+    for r in results[1:]:
+        for a, c in zip(r, results[0]):
+            if arith == M.ARITH_EXACT:
+                np.testing.assert_array_equal(a, c)
+            else:
+                assert relerr(a, c) <= 1e-12
+    _, init = make_batch(iord, arith)
+    for (nx, ny, nz), (U, V, W, c), q, got in zip(DIMS, init, qs, results[1]):
+        want = O.per_run(nx, ny, nz, U, V, W, iord, True, c, 7, q)
+        if arith == M.ARITH_EXACT:
+            np.testing.assert_array_equal(got, want)
+        else:
+            assert relerr(got, want) <= 1e-12
+
+
 def test_graph_replays_match_eager():
     ref, _ = make_batch(2, M.ARITH_FAST)
     gb, _ = make_batch(2, M.ARITH_FAST)
