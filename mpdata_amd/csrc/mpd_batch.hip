@@ -252,9 +252,11 @@ int launch_step(mpd_batch_s* b, int par)
         auto launch1 = b->arith == MPD_ARITH_FAST ? fused_batch_launch_f12_fast
                                                   : fused_batch_launch_f12_exact;
         const bool nt = fused_streaming((size_t)b->cells);
+        bool any_q = false;
+        for (int i = 0; i < b->ninst; i++) any_q = any_q || b->g[i]->has_q;
         for (int p = 1; p < b->iord; p++)
             if ((rc = launch1(b->dtab + table_index(b, par, p, 0, 0), b->ditem0, b->ninst,
-                              b->nitems, p == 1, nt, s)))
+                              b->nitems, p == 1, any_q && p == b->iord - 1, nt, s)))
                 return rc;
         return MPD_OK;
     }

@@ -697,11 +697,15 @@ k_donor_faces(int nx, int ny, int nz, int k0, const double* __restrict__ U,
 // ring 1 of the tile need A on the faces of ring-1 cells, which needs c1 on ring 2,
 // which needs c^n (and the velocities) on ring 3.
 //
-// Geometry: a TX x TY tile with a three-cell ring (the region).  Each thread owns one
-// tile column and at most one ring column, numbered so that the expensive ring work
-// falls on as few waves as possible: ring 1 (c1, A on three faces, ratios), then the
-// ring-2 cells left of and below ring 1 (c1 and the one face ring 1 needs), the other
-// ring-2 cells (c1), ring 3 (loads only).
+// Geometry: a TX x TY tile with a three-cell ring (the region).  The workgroup's waves
+// split the columns by role: TX*TY/64 tile waves own one tile column per thread; four
+// ring waves own the ring columns, each lane at most one cell of rings 1-2 (the primary:
+// c1, and on ring 1 and the ring-2 cells left of / below ring 1 the faces the tile needs)
+// and at most one cell of ring 3 (the secondary: loads and publishes only).  Ring 1 and
+// ring 2 are dealt evenly over the four ring waves, so every SIMD (three waves: two tile
+// waves and one ring wave) carries the same work per plane, and no thread holds the
+// rolling state of two computed columns -- which keeps a wave within the 168 registers
+// that three waves per SIMD allow (the LDS planes below hold a CU to one workgroup).
 //
 // Plane pipeline, iteration p (two barriers):
 //   phase 0  publish c^n(p), U(p), V(p); prefetch the next planes
@@ -721,11 +725,25 @@ struct Geo3 {
     static constexpr int H2 = 2 * (TX + 4) + 2 * (TY + 2);  // ring 2
     static constexpr int H3 = 2 * (TX + 6) + 2 * (TY + 4);  // ring 3
     static constexpr int NH = H1 + H2 + H3;
-    static_assert(NH <= NT, "every thread owns at most one ring cell");
-    static_assert(NT % 64 == 0, "whole waves");
-    __device__ static int role(int item)
+    // ring waves and the cells per ring wave of ring 1, ring 2 (primary) and ring 3
+    static constexpr int NRW = 4;
+    static constexpr int Q1 = (H1 + NRW - 1) / NRW, Q2 = (H2 + NRW - 1) / NRW;
+    static constexpr int Q3 = (H3 + NRW - 1) / NRW;
+    static constexpr int NTH = NT + 64 * NRW;  // threads of the workgroup
+    static_assert(Q1 + Q2 <= 64 && Q3 <= 64, "a ring lane owns at most one cell of each kind");
+    static_assert(NT % 64 == 0, "whole tile waves");
+    // Ring cell (halo numbering below) of ring lane l as primary / secondary, or -1.
+    __device__ static int primary(int l)
     {
-        return (int)((threadIdx.x + 64u * (unsigned)(item % (NT / 64))) % (unsigned)NT);
+        const int r = l >> 6, j = l & 63;
+        if (j < Q1) return r * Q1 + j < H1 ? r * Q1 + j : -1;
+        if (j < Q1 + Q2) return r * Q2 + j - Q1 < H2 ? H1 + r * Q2 + j - Q1 : -1;
+        return -1;
+    }
+    __device__ static int secondary(int l)
+    {
+        const int r = l >> 6, j = l & 63;
+        return j < Q3 && r * Q3 + j < H3 ? H1 + H2 + r * Q3 + j : -1;
     }
     // Region cell (rx, ry) of ring cell h and its kind: 1 ring 1, 2 ring 2 left of ring 1
     // (x-face), 3 ring 2 below ring 1 (y-face), 4 the other ring-2 cells, 5 ring 3.
@@ -781,6 +799,12 @@ struct Geo3 {
     }
 };
 
+// The LDS planes of F12 (one region per slot).
+template <int R>
+struct F12Lds {
+    double cn[4][R], u[3][R], v[3][R], w[R], c1[4][R], ax[2][R], ay[2][R], bu[2][R], bd[2][R];
+};
+
 template <int TX, int TY>
 __device__ __forceinline__ unsigned column3(int nx, int ny, int bx, int by, int cx, int cy)
 {
@@ -803,32 +827,45 @@ struct Col3 {
     double bu_m4 = 0, bd_m4 = 0, bu_m3 = 0, bd_m3 = 0;                   // own ratios
 };
 
-// SAVE (first pass of an iord-2 step only): also stores the adjoint's forward record of
-// every tile cell -- c1, the unlimited faces A, the unclamped limiter ratios and the bound
-// codes (see FusedArgs) -- the values the two-kernel storing step records, bit for bit.
-template <int TX, int TY, bool DONOR, bool NT, bool SAVE>
-__device__ __forceinline__ void f12_tile(const FusedArgs& a, int item)
+// One role's share of a work item.  TILE: lane = tile cell (all stages, the stores).
+// !TILE: lane = ring lane, with its primary column (rings 1-2: c1; the faces and ratios
+// the tile reads) and its secondary column (ring 3: c^n, U, V for the ring-2 donor field).
+// SAVE (first pass of an iord-2 step only): the tile also stores the adjoint's forward
+// record of every tile cell -- c1, the unlimited faces A, the unclamped limiter ratios and
+// the bound codes (see FusedArgs) -- the values the two-kernel storing step records, bit
+// for bit.
+template <int TX, int TY, bool DONOR, bool NT, bool SAVE, bool EMIT, bool TILE>
+__device__ __forceinline__ void f12_part(const FusedArgs& a, int item, int lane,
+                                         F12Lds<Geo3<TX, TY>::R>& sm)
 {
     using G = Geo3<TX, TY>;
-    constexpr int RX = G::RX, R = G::R;
-    __shared__ double s_cn[4][R], s_u[3][R], s_v[3][R], s_w[R], s_c1[4][R], s_ax[2][R],
-        s_ay[2][R], s_bu[2][R], s_bd[2][R];
-
-    const int t = G::role(item);
+    constexpr int RX = G::RX;
     int bx, by, bz;
     tile_of(a, item, bx, by, bz);
-    const int rx = 3 + t % TX, ry = 3 + t / TX;
-    const int L = ry * RX + rx;
-    const bool store = bx * TX + rx - 3 < a.nx && by * TY + ry - 3 < a.ny;
-    int hx = 0, hy = 0, kind = 0;
-    if (t < G::NH) kind = G::halo(t, hx, hy);
-    const int HL = hy * RX + hx;
-    const bool hl = kind != 0;                           // a ring cell: loads, publishes
-    const bool h2 = kind >= 1 && kind <= 4;              // rings 1-2: c1, W
-    const bool hax = kind == 1 || kind == 2, hay = kind == 1 || kind == 3, h1 = kind == 1;
-    Col3 T, H;
+    // the primary column (rx, ry) and its kind (0: none; the tile's cells count as kind 1
+    // with every stage); the secondary (sx, sy) of a ring lane, if any
+    int rx = 3, ry = 3, sx = 3, sy = 3, kind = 1;
+    bool sec = false;
+    if constexpr (TILE) {
+        rx = 3 + lane % TX;
+        ry = 3 + lane / TX;
+    } else {
+        const int hp = G::primary(lane), hs = G::secondary(lane);
+        kind = hp >= 0 ? G::halo(hp, rx, ry) : 0;
+        if (hs >= 0) {
+            G::halo(hs, sx, sy);
+            sec = true;
+        }
+    }
+    const int L = ry * RX + rx, LS = sy * RX + sx;
+    const bool act = TILE || kind != 0;                   // loads, c1, publishes
+    const bool fxk = TILE || kind == 1 || kind == 2;      // the face right of the cell
+    const bool fyk = TILE || kind == 1 || kind == 3;      // the face above it
+    const bool r1 = TILE || kind == 1;                    // z-face and ratios
+    const bool store = TILE && bx * TX + rx - 3 < a.nx && by * TY + ry - 3 < a.ny;
+    Col3 T, S;
     T.colb = column3<TX, TY>(a.nx, a.ny, bx, by, rx, ry);
-    H.colb = column3<TX, TY>(a.nx, a.ny, bx, by, hx, hy);
+    S.colb = column3<TX, TY>(a.nx, a.ny, bx, by, sx, sy);
 
     const int z0 = bz * a.zc;
     const int z1 = min(z0 + a.zc, a.nz);
@@ -837,25 +874,30 @@ __device__ __forceinline__ void f12_tile(const FusedArgs& a, int item)
     ZCursor zp2(pstart + 2, a.nz, a.np), zp1(pstart + 1, a.nz, a.np),
         zm2(pstart - 2, a.nz, a.np), zm3(pstart - 3, a.nz, a.np);
     ZCursor zp0(pstart, a.nz, a.np), zm1(pstart - 1, a.nz, a.np);  // records (SAVE)
-    auto first_loads = [&](Col3& C) __attribute__((always_inline)) {
-        ZCursor zm1(pstart - 1, a.nz, a.np), z0c(pstart, a.nz, a.np);
-        if (DONOR) {
-            C.cn_m1 = ld(a.cn + zm1.off, C.colb);
-            C.cn_0 = ld(a.cn + z0c.off, C.colb);
-        } else {
-            C.cn_m1 = ld(a.cn + zm1.off, C.colb);
-            C.cn_0 = ld(a.cn + z0c.off, C.colb);
-            C.nf = ld(a.cin + z0c.off, C.colb);
+    // Every lane loads, idle ring lanes at a valid column of the region: a load under a
+    // divergent condition makes the compiler copy its value out at the join, which waits
+    // for the load -- once per plane, a full memory round trip.
+    {
+        ZCursor zb(pstart - 1, a.nz, a.np), zc(pstart, a.nz, a.np);
+        T.cn_m1 = ld(a.cn + zb.off, T.colb);
+        T.cn_0 = ld(a.cn + zc.off, T.colb);
+        if (!DONOR) T.nf = ld(a.cin + zc.off, T.colb);
+        T.ncn = ld(a.cn + zp1.off, T.colb);
+        T.w_m1 = ld(a.W + zb.off, T.colb);
+        T.nw = ld(a.W + zc.off, T.colb);
+        T.nu = ld(a.U + zc.off, T.colb);
+        T.nv = ld(a.V + zc.off, T.colb);
+        if constexpr (!TILE) {
+            S.cn_0 = ld(a.cn + zc.off, S.colb);
+            S.ncn = ld(a.cn + zp1.off, S.colb);
+            S.nu = ld(a.U + zc.off, S.colb);
+            S.nv = ld(a.V + zc.off, S.colb);
         }
-        C.ncn = ld(a.cn + zp1.off, C.colb);
-        C.w_m1 = ld(a.W + zm1.off, C.colb);
-        C.nw = ld(a.W + z0c.off, C.colb);
-        C.nu = ld(a.U + z0c.off, C.colb);
-        C.nv = ld(a.V + z0c.off, C.colb);
-    };
-    first_loads(T);
-    first_loads(H);
+    }
     double nq = 0;  // emission of plane p-2, used for the output plane of the next iteration
+    // (an instance of a batch without emission reads a plane of its field instead: an
+    // unconditional load, whatever the item)
+    const double* qb = a.q ? a.q : a.cn;
 
     // the plane loop, unrolled by hand (four planes per trip, then one at a time): across
     // the copies the per-column registers are renamed instead of copied (the barriers
@@ -864,85 +906,91 @@ __device__ __forceinline__ void f12_tile(const FusedArgs& a, int item)
     // guards and the compiler may move each stage's LDS reads ahead of the previous
     // stage's arithmetic
     auto plane = [&](const int pl, auto steady) __attribute__((always_inline)) {
-        constexpr bool S = decltype(steady)::value;
+        constexpr bool ST = decltype(steady)::value;
         const int p = __builtin_amdgcn_readfirstlane(pl);
         keep32(T.colb);
-        keep32(H.colb);
+        if constexpr (!TILE) keep32(S.colb);
         const int p8 = p + 8;  // p >= -2: non-negative slot arithmetic
         const int s4 = p8 & 3, s3 = p8 % 3;
         // ---- phase 0: this plane's inputs, the next planes' loads
         __builtin_amdgcn_s_setprio(2);
         const double qo = nq;
-        if (a.q) nq = ld(a.q + zm2.off, T.colb);
-        auto advance = [&](Col3& C) __attribute__((always_inline)) {
-            C.cn_p1 = C.ncn;
-            C.u_0 = C.nu;
-            C.v_0 = C.nv;
-            C.w_0 = C.nw;
-            C.fld = C.nf;
-            unsigned c2 = C.colb;
+        if constexpr (TILE && EMIT) nq = ld(qb + zm2.off, T.colb);
+        T.cn_p1 = T.ncn;
+        T.u_0 = T.nu;
+        T.v_0 = T.nv;
+        T.w_0 = T.nw;
+        T.fld = T.nf;
+        {
+            unsigned c2 = T.colb;
             keep32(c2);
-            C.ncn = ld(a.cn + zp2.off, c2);
-            C.nu = ld(a.U + zp1.off, c2);
-            C.nv = ld(a.V + zp1.off, c2);
-            C.nw = ld(a.W + zp1.off, c2);
-            if (!DONOR) C.nf = ld(a.cin + zp1.off, c2);
-        };
-        advance(T);
-        advance(H);
-        s_cn[s4][L] = T.cn_0;
-        s_u[s3][L] = T.u_0;
-        s_v[s3][L] = T.v_0;
-        if (hl) {
-            s_cn[s4][HL] = H.cn_0;
-            s_u[s3][HL] = H.u_0;
-            s_v[s3][HL] = H.v_0;
+            T.ncn = ld(a.cn + zp2.off, c2);
+            T.nu = ld(a.U + zp1.off, c2);
+            T.nv = ld(a.V + zp1.off, c2);
+            T.nw = ld(a.W + zp1.off, c2);
+            if (!DONOR) T.nf = ld(a.cin + zp1.off, c2);
         }
-        lds_barrier();
-        __builtin_amdgcn_s_setprio(0);
-        // ---- phase 1: c1 of plane p
-        const bool do_c1 = S || p <= z1 + 1;
-        if (do_c1) {
-            const double* cn = s_cn[s4];
-            const double* up = s_u[s3];
-            const double* vp = s_v[s3];
-            T.c1_0 = DONOR ? donor(T.cn_0, cn[L - 1], cn[L + 1], cn[L - RX], cn[L + RX], T.cn_m1,
-                                   T.cn_p1, up[L - 1], T.u_0, vp[L - RX], T.v_0, T.w_m1, T.w_0)
-                           : T.fld;
-            s_c1[s4][L] = T.c1_0;
-            if constexpr (SAVE)
-                if (store && p >= z0 && p < z1) put<NT>(a.c1out + zp0.off, T.colb, T.c1_0);
-            if (h2) {
-                H.c1_0 = DONOR ? donor(H.cn_0, cn[HL - 1], cn[HL + 1], cn[HL - RX], cn[HL + RX],
-                                       H.cn_m1, H.cn_p1, up[HL - 1], H.u_0, vp[HL - RX], H.v_0,
-                                       H.w_m1, H.w_0)
-                               : H.fld;
-                s_c1[s4][HL] = H.c1_0;
+        if (act) {
+            sm.cn[s4][L] = T.cn_0;
+            sm.u[s3][L] = T.u_0;
+            sm.v[s3][L] = T.v_0;
+        }
+        if constexpr (!TILE) {
+            S.u_0 = S.nu;
+            S.v_0 = S.nv;
+            unsigned c2 = S.colb;
+            keep32(c2);
+            const double cn = S.cn_0;
+            S.cn_0 = S.ncn;  // (c^n(p+1): the secondary keeps no plane below p)
+            S.ncn = ld(a.cn + zp2.off, c2);
+            S.nu = ld(a.U + zp1.off, c2);
+            S.nv = ld(a.V + zp1.off, c2);
+            if (sec) {
+                sm.cn[s4][LS] = cn;
+                sm.u[s3][LS] = S.u_0;
+                sm.v[s3][LS] = S.v_0;
             }
         }
-        s_w[L] = T.w_m1;
-        if (h2) s_w[HL] = H.w_m1;
         lds_barrier();
-        // ---- phase 2: the tile's three stages first (one block in steady planes: they
-        // read only LDS planes published before this phase), then the ring cells' work
+        // the ring waves' work gates the next plane of every wave: they take precedence
+        // over the (older) tile waves, which fill the cycles the ring waves leave
+        __builtin_amdgcn_s_setprio(TILE ? 0 : 1);
+        // ---- phase 1: c1 of plane p
+        const bool do_c1 = ST || p <= z1 + 1;
+        if (act) {
+            if (do_c1) {
+                const double* cn = sm.cn[s4];
+                const double* up = sm.u[s3];
+                const double* vp = sm.v[s3];
+                T.c1_0 = DONOR ? donor(T.cn_0, cn[L - 1], cn[L + 1], cn[L - RX], cn[L + RX],
+                                       T.cn_m1, T.cn_p1, up[L - 1], T.u_0, vp[L - RX], T.v_0,
+                                       T.w_m1, T.w_0)
+                               : T.fld;
+                sm.c1[s4][L] = T.c1_0;
+                if constexpr (SAVE && TILE)
+                    if (store && p >= z0 && p < z1) put<NT>(a.c1out + zp0.off, T.colb, T.c1_0);
+            }
+            sm.w[L] = T.w_m1;
+        }
+        lds_barrier();
+        // ---- phase 2: faces, ratios and (tile) the limited donor pass
         const int k = p - 1, m = p - 2, o = p - 3;
-        const bool doA = S || (k >= z0 - 2 && k <= z1);
-        const bool doB = S || (m >= z0 - 1 && m <= z1);
-        const bool doO = S || (o >= z0 && o < z1);
+        const bool doA = ST || (k >= z0 - 2 && k <= z1);
+        const bool doB = ST || (m >= z0 - 1 && m <= z1);
+        const bool doO = ST || (o >= z0 && o < z1);
         const int km4 = (p8 - 1) & 3, km3 = (p8 - 1) % 3, kq4 = s4, kq3 = s3;  // planes k, k+1
         const int kb4 = (p8 - 2) & 3;                                          // plane k-1
         const int ka = (p8 - 1) & 1;
-        const double wkr = s_w[L + 1], wku = s_w[L + RX];
         // (ring cells that compute no face are not read at their neighbours' positions,
         // some of which lie outside the region)
-        const double hwkr = s_w[hax ? HL + 1 : L], hwku = s_w[hay ? HL + RX : L];
-        const double* cm = s_c1[kb4];
-        const double* cc = s_c1[km4];
-        const double* cq = s_c1[kq4];
-        const double* uk = s_u[km3];
-        const double* uq = s_u[kq3];
-        const double* vk = s_v[km3];
-        const double* vq = s_v[kq3];
+        const double wkr = sm.w[fxk ? L + 1 : L], wku = sm.w[fyk ? L + RX : L];
+        const double* cm = sm.c1[kb4];
+        const double* cc = sm.c1[km4];
+        const double* cq = sm.c1[kq4];
+        const double* uk = sm.u[km3];
+        const double* uq = sm.u[kq3];
+        const double* vk = sm.v[km3];
+        const double* vq = sm.v[kq3];
         // antidiffusive velocities of the faces above/right of cell M of plane k
         // (the formulas and operand order of f1_tile)
         auto faces = [&](int M, const Col3& C, double wr, double wu, bool fx, bool fy,
@@ -1023,12 +1071,12 @@ __device__ __forceinline__ void f12_tile(const FusedArgs& a, int item)
         };
         // limiter ratios of plane m (see cpu_ref.c), clamped to at most 1
         const int mb4 = (p8 - 2) & 3, ma = (p8 - 2) & 1;
-        const double* cnm = s_cn[mb4];
-        const double* c1m = s_c1[mb4];
-        const double* axm = s_ax[ma];
-        const double* aym = s_ay[ma];
+        const double* cnm = sm.cn[mb4];
+        const double* c1m = sm.c1[mb4];
+        const double* axm = sm.ax[ma];
+        const double* aym = sm.ay[ma];
         auto ratios = [&](int M, const Col3& C, double& bu, double& bd, double& axl,
-                          double& ayd, bool rec) __attribute__((always_inline)) {
+                          double& ayd) __attribute__((always_inline)) {
             const double c = C.c1_m2;
             const double xl = c1m[M - 1], xr = c1m[M + 1], yd = c1m[M - RX], yu = c1m[M + RX];
             const double hi =
@@ -1047,8 +1095,8 @@ __device__ __forceinline__ void f12_tile(const FusedArgs& a, int item)
             const double fyl = 0.5 * F2(yd, c, ayd), fyr = 0.5 * F2(c, yu, C.ay_m2);
             const double fzl = 0.5 * F2(C.c1_m3, c, C.az_m3), fzr = 0.5 * F2(c, C.c1_m1, C.az_m2);
             beta(hi, lo, c, fxl, fxr, fyl, fyr, fzl, fzr, bu, bd);
-            if constexpr (SAVE) {
-                if (rec && store && m >= z0 && m < z1) {
+            if constexpr (SAVE && TILE) {
+                if (store && m >= z0 && m < z1) {
                     // which element holds each bound: element 2q + s is the c^n (s = 0) or
                     // pass-input (s = 1) value of neighbour q in {self, x-, x+, y-, y+, z-,
                     // z+}; of the elements equal to the bound the last one counts
@@ -1075,93 +1123,81 @@ __device__ __forceinline__ void f12_tile(const FusedArgs& a, int item)
             bu = mn2(bu, 1.0);
             bd = mn2(bd, 1.0);
         };
-        double ax_k = 0, ay_k = 0, az_k = 0, hax_k = 0, hay_k = 0, haz_k = 0;
+        double ax_k = 0, ay_k = 0, az_k = 0;
         double bu_m = 0, bd_m = 0, axl_m = 0, ayd_m = 0;
-        if (doA) {
-            faces(L, T, wkr, wku, true, true, true, ax_k, ay_k, az_k);
-            s_ax[ka][L] = ax_k;
-            s_ay[ka][L] = ay_k;
-            if constexpr (SAVE)
+        if (doA && (fxk || fyk)) {
+            faces(L, T, wkr, wku, fxk, fyk, r1, ax_k, ay_k, az_k);
+            if (fxk) sm.ax[ka][L] = ax_k;
+            if (fyk) sm.ay[ka][L] = ay_k;
+            if constexpr (SAVE && TILE)
                 if (store && k >= z0 && k < z1) {
                     put<NT>(a.Ax + zm1.off, T.colb, ax_k);
                     put<NT>(a.Ay + zm1.off, T.colb, ay_k);
                     put<NT>(a.Az + zm1.off, T.colb, az_k);
                 }
         }
-        if (doB) {
-            ratios(L, T, bu_m, bd_m, axl_m, ayd_m, true);
-            s_bu[ma][L] = bu_m;
-            s_bd[ma][L] = bd_m;
+        if (doB && r1) {
+            ratios(L, T, bu_m, bd_m, axl_m, ayd_m);
+            sm.bu[ma][L] = bu_m;
+            sm.bd[ma][L] = bd_m;
         }
         // limited donor pass of plane o on the tile
-        if (doO) {
-            const int ob4 = (p8 - 3) & 3, oa = (p8 - 3) & 1;
-            const double* bu = s_bu[oa];
-            const double* bd = s_bd[oa];
-            const double* co = s_c1[ob4];
-            const double b0u = T.bu_m3, b0d = T.bd_m3;
-            const double uxr = limit_face_clamped(T.ax_m3, b0u, b0d, bu[L + 1], bd[L + 1]);
-            const double uxl = limit_face_clamped(T.axl_m3, bu[L - 1], bd[L - 1], b0u, b0d);
-            const double vyr = limit_face_clamped(T.ay_m3, b0u, b0d, bu[L + RX], bd[L + RX]);
-            const double vyl = limit_face_clamped(T.ayd_m3, bu[L - RX], bd[L - RX], b0u, b0d);
-            const double wzr = limit_face_clamped(T.az_m3, b0u, b0d, bu_m, bd_m);
-            const double wzl = limit_face_clamped(T.az_m4, T.bu_m4, T.bd_m4, b0u, b0d);
-            double r = donor(T.c1_m3, co[L - 1], co[L + 1], co[L - RX], co[L + RX], T.c1_m4,
-                             T.c1_m2, uxl, uxr, vyl, vyr, wzl, wzr);
-            if (store) {
-                if (a.q) r = r + qo;
-                put<NT>(a.cout + zm3.off, T.colb, r);
-                if (a.Lx) {
-                    put<NT>(a.Lx + zm3.off, T.colb, uxr);
-                    put<NT>(a.Ly + zm3.off, T.colb, vyr);
-                    put<NT>(a.Lz + zm3.off, T.colb, wzr);
+        if constexpr (TILE) {
+            if (doO) {
+                const int ob4 = (p8 - 3) & 3, oa = (p8 - 3) & 1;
+                const double* bu = sm.bu[oa];
+                const double* bd = sm.bd[oa];
+                const double* co = sm.c1[ob4];
+                const double b0u = T.bu_m3, b0d = T.bd_m3;
+                const double uxr = limit_face_clamped(T.ax_m3, b0u, b0d, bu[L + 1], bd[L + 1]);
+                const double uxl = limit_face_clamped(T.axl_m3, bu[L - 1], bd[L - 1], b0u, b0d);
+                const double vyr = limit_face_clamped(T.ay_m3, b0u, b0d, bu[L + RX], bd[L + RX]);
+                const double vyl = limit_face_clamped(T.ayd_m3, bu[L - RX], bd[L - RX], b0u, b0d);
+                const double wzr = limit_face_clamped(T.az_m3, b0u, b0d, bu_m, bd_m);
+                const double wzl = limit_face_clamped(T.az_m4, T.bu_m4, T.bd_m4, b0u, b0d);
+                double r = donor(T.c1_m3, co[L - 1], co[L + 1], co[L - RX], co[L + RX], T.c1_m4,
+                                 T.c1_m2, uxl, uxr, vyl, vyr, wzl, wzr);
+                if (store) {
+                    if (EMIT && a.q) r = r + qo;
+                    put<NT>(a.cout + zm3.off, T.colb, r);
+                    if (a.Lx) {
+                        put<NT>(a.Lx + zm3.off, T.colb, uxr);
+                        put<NT>(a.Ly + zm3.off, T.colb, vyr);
+                        put<NT>(a.Lz + zm3.off, T.colb, wzr);
+                    }
                 }
             }
         }
-        if (doA && (hax || hay)) {
-            faces(HL, H, hwkr, hwku, hax, hay, h1, hax_k, hay_k, haz_k);
-            if (hax) s_ax[ka][HL] = hax_k;
-            if (hay) s_ay[ka][HL] = hay_k;
-        }
-        if (doB && h1) {
-            double hbu, hbd, dx, dy;
-            ratios(HL, H, hbu, hbd, dx, dy, false);
-            s_bu[ma][HL] = hbu;
-            s_bd[ma][HL] = hbd;
-        }
         // rotate the per-column registers
-        auto rotate = [&](Col3& C, double wkr_, double wku_, double ax, double ay, double az)
-                          __attribute__((always_inline)) {
-            C.cn_m3 = C.cn_m2;
-            C.cn_m2 = C.cn_m1;
-            C.cn_m1 = C.cn_0;
-            C.cn_0 = C.cn_p1;
-            C.w_m2 = C.w_m1;
-            C.w_m1 = C.w_0;
-            C.wr_m = wkr_;
-            C.wu_m = wku_;
-            C.u_m1 = C.u_0;
-            C.v_m1 = C.v_0;
-            C.c1_m4 = C.c1_m3;
-            C.c1_m3 = C.c1_m2;
-            C.c1_m2 = C.c1_m1;
-            C.c1_m1 = C.c1_0;
-            C.ax_m3 = C.ax_m2;
-            C.ay_m3 = C.ay_m2;
-            C.ax_m2 = ax;
-            C.ay_m2 = ay;
-            C.az_m4 = C.az_m3;
-            C.az_m3 = C.az_m2;
-            C.az_m2 = az;
-        };
-        rotate(T, wkr, wku, ax_k, ay_k, az_k);
-        rotate(H, hwkr, hwku, hax_k, hay_k, haz_k);
-        T.axl_m3 = axl_m;
-        T.ayd_m3 = ayd_m;
-        T.bu_m4 = T.bu_m3;
-        T.bd_m4 = T.bd_m3;
-        T.bu_m3 = bu_m;
-        T.bd_m3 = bd_m;
+        T.cn_m3 = T.cn_m2;
+        T.cn_m2 = T.cn_m1;
+        T.cn_m1 = T.cn_0;
+        T.cn_0 = T.cn_p1;
+        T.w_m2 = T.w_m1;
+        T.w_m1 = T.w_0;
+        T.wr_m = wkr;
+        T.wu_m = wku;
+        T.u_m1 = T.u_0;
+        T.v_m1 = T.v_0;
+        T.c1_m4 = T.c1_m3;
+        T.c1_m3 = T.c1_m2;
+        T.c1_m2 = T.c1_m1;
+        T.c1_m1 = T.c1_0;
+        T.ax_m3 = T.ax_m2;
+        T.ay_m3 = T.ay_m2;
+        T.ax_m2 = ax_k;
+        T.ay_m2 = ay_k;
+        T.az_m4 = T.az_m3;
+        T.az_m3 = T.az_m2;
+        T.az_m2 = az_k;
+        if constexpr (TILE) {
+            T.axl_m3 = axl_m;
+            T.ayd_m3 = ayd_m;
+            T.bu_m4 = T.bu_m3;
+            T.bd_m4 = T.bd_m3;
+            T.bu_m3 = bu_m;
+            T.bd_m3 = bd_m;
+        }
         zp2.next();
         zp1.next();
         zm2.next();
@@ -1184,9 +1220,14 @@ __device__ __forceinline__ void f12_tile(const FusedArgs& a, int item)
     for (; pl <= pend; pl++) plane(pl, Edge());
 }
 
-template <int TX, int TY, bool DONOR, bool NT, bool SAVE>
-__global__ void __launch_bounds__((Geo3<TX, TY>::NT)) k_f12(FusedArgs a, Work w)
+// Tile waves first, then the ring waves; both roles take the same planes and barriers.
+// EMIT: the pass adds the emission increment (a.q set; the last pass of a step).
+template <int TX, int TY, bool DONOR, bool NT, bool SAVE, bool EMIT>
+__global__ void __launch_bounds__((Geo3<TX, TY>::NTH)) k_f12(FusedArgs a, Work w)
 {
+    using G = Geo3<TX, TY>;
+    __shared__ F12Lds<G::R> sm;
+    const bool tile = __builtin_amdgcn_readfirstlane(threadIdx.x) < G::NT;
     for (int it = blockIdx.x; it < w.nitems; it += gridDim.x) {
         int item = item_of(w, it);
         FusedArgs x = a;
@@ -1196,7 +1237,10 @@ __global__ void __launch_bounds__((Geo3<TX, TY>::NT)) k_f12(FusedArgs a, Work w)
             __builtin_memcpy(&x, (const FusedArgs*)((ConstArgs*)w.table + i), sizeof x);
             item -= item0[i];
         }
-        f12_tile<TX, TY, DONOR, NT, SAVE>(x, item);
+        if (tile)
+            f12_part<TX, TY, DONOR, NT, SAVE, EMIT, true>(x, item, threadIdx.x, sm);
+        else
+            f12_part<TX, TY, DONOR, NT, SAVE, EMIT, false>(x, item, threadIdx.x - G::NT, sm);
         lds_barrier();
     }
 }
@@ -1257,27 +1301,37 @@ int launch_any(const FusedArgs& a, const Work& w, int grid, int tx, int ty, bool
 // Tile shapes of the one-kernel-per-pass path (LDS: 23 planes of the tile + 3-cell ring).
 constexpr int F12_TX = F12_TILE_X, F12_TY = F12_TILE_Y;
 
-int launch_f12(const FusedArgs& a, const Work& w, int grid, bool donor, hipStream_t s,
-               const hipEvent_t* ev)
+template <bool DONOR, bool SAVE>
+void launch_f12_as(const FusedArgs& a, const Work& w, int grid, bool emit, hipStream_t s,
+                   const hipEvent_t* ev)
 {
     constexpr int TX = F12_TX, TY = F12_TY;
-    const dim3 blk(Geo3<TX, TY>::NT);
+    const dim3 blk(Geo3<TX, TY>::NTH);
+    if (a.nt) {
+        if (emit)
+            launch_k(k_f12<TX, TY, DONOR, true, SAVE, true>, dim3(grid), blk, s, ev, a, w);
+        else
+            launch_k(k_f12<TX, TY, DONOR, true, SAVE, false>, dim3(grid), blk, s, ev, a, w);
+    } else {
+        if (emit)
+            launch_k(k_f12<TX, TY, DONOR, false, SAVE, true>, dim3(grid), blk, s, ev, a, w);
+        else
+            launch_k(k_f12<TX, TY, DONOR, false, SAVE, false>, dim3(grid), blk, s, ev, a, w);
+    }
+}
+
+// emit: the pass adds an emission increment (for a batch: some instance's descriptor has
+// one; the others' are null)
+int launch_f12(const FusedArgs& a, const Work& w, int grid, bool donor, bool emit, hipStream_t s,
+               const hipEvent_t* ev)
+{
     if (a.bu) {  // the adjoint's record (first pass only)
         if (!donor) return fail(MPD_ERR_UNSUPPORTED, "records of the first pass only");
-        if (a.nt)
-            launch_k(k_f12<TX, TY, true, true, true>, dim3(grid), blk, s, ev, a, w);
-        else
-            launch_k(k_f12<TX, TY, true, false, true>, dim3(grid), blk, s, ev, a, w);
+        launch_f12_as<true, true>(a, w, grid, emit, s, ev);
     } else if (donor) {
-        if (a.nt)
-            launch_k(k_f12<TX, TY, true, true, false>, dim3(grid), blk, s, ev, a, w);
-        else
-            launch_k(k_f12<TX, TY, true, false, false>, dim3(grid), blk, s, ev, a, w);
+        launch_f12_as<true, false>(a, w, grid, emit, s, ev);
     } else {
-        if (a.nt)
-            launch_k(k_f12<TX, TY, false, true, false>, dim3(grid), blk, s, ev, a, w);
-        else
-            launch_k(k_f12<TX, TY, false, false, false>, dim3(grid), blk, s, ev, a, w);
+        launch_f12_as<false, false>(a, w, grid, emit, s, ev);
     }
     MPD_HIP(hipGetLastError());
     return MPD_OK;
@@ -1379,13 +1433,13 @@ int MPD_CAT(fused_batch_launch, MPD_ARITH)(int tx, int ty, bool f1, bool donor,
 }
 
 int MPD_CAT(fused_batch_launch_f12, MPD_ARITH)(const FusedArgs* table, const int* item0,
-                                               int ninst, int nitems, bool donor, bool nt,
-                                               hipStream_t s)
+                                               int ninst, int nitems, bool donor, bool emit,
+                                               bool nt, hipStream_t s)
 {
     Work w{table, item0, ninst, nitems, nitems % 8 == 0 ? 1 : 0};
     FusedArgs none{};
     none.nt = nt;
-    return launch_f12(none, w, nitems, donor, s, nullptr);
+    return launch_f12(none, w, nitems, donor, emit, s, nullptr);
 }
 
 int MPD_CAT(fused_resident_blocks, MPD_ARITH)(int tx, int ty, bool f1)
@@ -1437,7 +1491,7 @@ int MPD_CAT(fused_store_step, MPD_ARITH)(mpd_grid_s* g, const double* cin, doubl
         fused_geometry(a, F12_TX, F12_TY);
         const int nb = fused_items(a);
         const Work w{nullptr, nullptr, 0, nb, nb % 8 == 0 ? 1 : 0};
-        return launch_f12(a, w, nb, true, s, nullptr);
+        return launch_f12(a, w, nb, true, a.q != nullptr, s, nullptr);
     }
     if ((rc = launch(a, tx, ty, true, true, s, nullptr))) return rc;
     const double* q = g->has_q ? g->q.p : nullptr;
@@ -1502,7 +1556,7 @@ int MPD_CAT(periodic_fused_step, MPD_ARITH)(mpd_grid_s* g)
             hipEvent_t ev[2];
             const hipEvent_t* use;
             if ((rc = span(g, ev, &use))) return rc;
-            if ((rc = launch_f12(a, w, nb, p == 1, s, use))) return rc;
+            if ((rc = launch_f12(a, w, nb, p == 1, a.q != nullptr, s, use))) return rc;
             for (int d = 0; d < 3; d++) prev[d] = lf[d].p;
             field = a.cout;
         }

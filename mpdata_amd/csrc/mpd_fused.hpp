@@ -64,13 +64,14 @@ int fused_batch_launch_exact(int tx, int ty, bool f1, bool donor, const FusedArg
 int fused_batch_launch_fast(int tx, int ty, bool f1, bool donor, const FusedArgs* table,
                             const int* item0, int ninst, int nitems, int grid, bool nt,
                             hipStream_t s);
-// One batched launch of the one-kernel-per-pass kernel (donor: the first pass) over a
-// device table of instance descriptors in 32 x 16 tiles (F12_TILE_X x F12_TILE_Y).
+// One batched launch of the one-kernel-per-pass kernel (donor: the first pass; emit: some
+// instance's descriptor carries an emission increment) over a device table of instance
+// descriptors in 32 x 16 tiles (F12_TILE_X x F12_TILE_Y).
 constexpr int F12_TILE_X = 32, F12_TILE_Y = 16;
 int fused_batch_launch_f12_exact(const FusedArgs* table, const int* item0, int ninst,
-                                 int nitems, bool donor, bool nt, hipStream_t s);
+                                 int nitems, bool donor, bool emit, bool nt, hipStream_t s);
 int fused_batch_launch_f12_fast(const FusedArgs* table, const int* item0, int ninst,
-                                int nitems, bool donor, bool nt, hipStream_t s);
+                                int nitems, bool donor, bool emit, bool nt, hipStream_t s);
 // One exact-arithmetic iord-2 step cin -> cout of a PERIODIC grid that also stores the
 // forward's intermediates for the adjoint: the donor field c1, the antidiffusive faces
 // A[3] and, with the limiter, its data (see FusedArgs; null pointers when unlimited).
