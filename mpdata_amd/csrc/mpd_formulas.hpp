@@ -30,19 +30,17 @@ __device__ __forceinline__ double ab(double a) { return __builtin_fabs(a); }
 
 #ifdef MPD_FAST_DIV
 // n / d for d >= DBL_MIN (every denominator here is a sum of non-negative cell values
-// plus MINV, or a positive flux sum plus MINV): hardware reciprocal estimate r0 of
-// relative error e0, one Newton-Raphson step r <- r + r(1 - d r) (error e0^2), then one
-// residual correction of the quotient q <- q + r(n - d q) (error e0^4, below one ulp
-// for any e0 < 2^-13.5; v_rcp_f64 is far more accurate than that).  Error: about one
-// ulp of the IEEE quotient.
+// plus MINV, or a positive flux sum plus MINV): hardware reciprocal estimate r of
+// relative error e0 (v_rcp_f64: about 2^-26 on gfx950), the quotient q = n r, then one
+// residual correction q <- q + r(n - d q), whose error is e0^2 plus the rounding of the
+// two fused steps.  Measured on 4M quotients spanning the kernels' magnitudes: at most
+// 18 ulp, 0.6 ulp on average, of the IEEE quotient (a Newton step on r first would make
+// it exact there, for two more instructions per division).
 __device__ __forceinline__ double dv(double n, double d)
 {
     double r = __builtin_amdgcn_rcp(d);
-    double e = __builtin_fma(-d, r, 1.0);
-    r = __builtin_fma(r, e, r);
     double q = n * r;
-    double res = __builtin_fma(-d, q, n);
-    return __builtin_fma(r, res, q);
+    return __builtin_fma(r, __builtin_fma(-d, q, n), q);
 }
 #else
 __device__ __forceinline__ double dv(double n, double d) { return n / d; }
