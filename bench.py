@@ -59,7 +59,8 @@ def tuned_config(g):
     its z-chunk), or the antidiffusive kernel's tile and z-chunk and the limiter kernel's
     (its own when the tuner chose one)."""
     if g.path() == 3:
-        return {"kernel": "one per corrective pass", "tile": [32, 16], "zc": g.pass_zchunk()}
+        return {"kernel": "one per corrective pass", "tile": [32, 16], "zc": g.pass_zchunk(),
+                "waves": "8 tile + 4 ring"}
     (tx, ty, zc), (lx, ly, lz) = g.tiles()
     out = {"tile": [tx, ty], "zc": zc}
     if lx:
@@ -210,6 +211,9 @@ def main():
     ap.add_argument("--ring2d", type=int, default=128,
                     help="edge of the periodic 2-D ring leg, 1024 steps (0: skip)")
     ap.add_argument("--legs", type=int, default=1, help="0: the headline only")
+    ap.add_argument("--path", type=int, default=0, choices=(0, 2, 3),
+                    help="step path of the headline and large grids: 0 automatic, 2 two "
+                         "kernels per corrective pass, 3 one kernel per pass")
     ap.add_argument("--only", default="",
                     help="run one workload alone and print its object (for profiling): "
                          "headline, ring3d, large, ensemble, ring2d, dropin, adjoint")
@@ -239,7 +243,8 @@ def main():
         args.no_cpu = True
     if args.only and args.only != "headline":
         legs = {"ring3d": lambda: ring3d_leg(M, args.ring3d, args.arith, check=not args.profile),
-                "large": lambda: large_leg(M, args.large, 2, args.arith, check=not args.profile),
+                "large": lambda: large_leg(M, args.large, 2, args.arith, check=not args.profile,
+                                           path=args.path),
                 "ensemble": lambda: ensemble_leg(M, args.ensemble),
                 "ring2d": lambda: ring2d_leg(M, args.ring2d),
                 "dropin": lambda: dropin_leg(M, args.dropin),
@@ -254,7 +259,7 @@ def main():
         if args.ensemble:
             line["ensemble"] = ensemble_leg(M, args.ensemble)
         if args.large:
-            line["large"] = large_leg(M, args.large, 2, args.arith)
+            line["large"] = large_leg(M, args.large, 2, args.arith, path=args.path)
         if args.ring2d:
             line["ring2d"] = ring2d_leg(M, args.ring2d)
         if args.dropin:
@@ -278,6 +283,7 @@ def headline(M, args):
     cells = nx * ny * nz
     g = M.Grid(*shape, M.PERIODIC, iord=3, nonosc=True)
     g.set_arith(args.arith)
+    g.set_path(args.path)
     K.deform(g, shape, seed=3)
     c0 = g.field()
     mass0 = c0.sum()
@@ -536,7 +542,7 @@ def ensemble_leg(M, n):
             f"rel_err_smallest_vs_oracle_after_{taken}_steps": err}
 
 
-def large_leg(M, n, iord, arith, check=True):
+def large_leg(M, n, iord, arith, check=True, path=0):
     """Roofline of the single-grid step at n^3 (the size the 70%-of-HBM target names); one
     step against the oracle, and the whole run against the exact-arithmetic kernels."""
     import mpdata_amd.configs as K
@@ -544,6 +550,7 @@ def large_leg(M, n, iord, arith, check=True):
 
     g = M.Grid(n, n, n, M.PERIODIC, iord=iord, nonosc=True)
     g.set_arith(arith)
+    g.set_path(path)
     K.ring3d(g, n)
     U, V, W = g.get_courant()
     c0 = g.field()
