@@ -212,8 +212,8 @@ def main():
                     help="edge of the periodic 2-D ring leg, 1024 steps (0: skip)")
     ap.add_argument("--legs", type=int, default=1, help="0: the headline only")
     ap.add_argument("--path", type=int, default=0, choices=(0, 2, 3),
-                    help="step path of the headline and large grids: 0 automatic, 2 two "
-                         "kernels per corrective pass, 3 one kernel per pass")
+                    help="step path of the headline, ring3d and large grids: 0 automatic, 2 "
+                         "two kernels per corrective pass, 3 one kernel per pass")
     ap.add_argument("--only", default="",
                     help="run one workload alone and print its object (for profiling): "
                          "headline, ring3d, large, ensemble, ring2d, dropin, adjoint")
@@ -242,7 +242,8 @@ def main():
     if args.profile:
         args.no_cpu = True
     if args.only and args.only != "headline":
-        legs = {"ring3d": lambda: ring3d_leg(M, args.ring3d, args.arith, check=not args.profile),
+        legs = {"ring3d": lambda: ring3d_leg(M, args.ring3d, args.arith, check=not args.profile,
+                                             path=args.path),
                 "large": lambda: large_leg(M, args.large, 2, args.arith, check=not args.profile,
                                            path=args.path),
                 "ensemble": lambda: ensemble_leg(M, args.ensemble),
@@ -255,7 +256,7 @@ def main():
     line = headline(M, args)
     if args.legs and not args.only:
         if args.ring3d:
-            line["ring3d"] = ring3d_leg(M, args.ring3d, args.arith)
+            line["ring3d"] = ring3d_leg(M, args.ring3d, args.arith, path=args.path)
         if args.ensemble:
             line["ensemble"] = ensemble_leg(M, args.ensemble)
         if args.large:
@@ -387,7 +388,7 @@ def headline(M, args):
     return line
 
 
-def ring3d_leg(M, n, arith, steps=128, check=True):
+def ring3d_leg(M, n, arith, steps=128, check=True, path=0):
     """The cache-resident rotation case: n^3 periodic, solid-body rotation about (1,1,1)
     at max Courant 0.4, Gaussian blob sigma 8, iord 2 nonoscillatory; autotuned, 64
     warm-up and `steps` timed steps, the step's roofline, and the run against the
@@ -398,6 +399,7 @@ def ring3d_leg(M, n, arith, steps=128, check=True):
     cells = n ** 3
     g = M.Grid(n, n, n, M.PERIODIC, iord=2, nonosc=True)
     g.set_arith(arith)
+    g.set_path(path)
     g.set_courant(case["U"], case["V"], case["W"])
     g.set_field(case["c"])
     g.autotune(2)

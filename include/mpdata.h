@@ -100,8 +100,7 @@ int mpd_sentinel_check(mpd_grid g, long long* first_bad_step);
  * kernel per stage, intermediates in HBM), 2 = fused (two kernels per corrective pass:
  * antidiffusive velocities, then limiter and limited donor pass), 3 = one kernel per
  * corrective pass (nonoscillatory grids; the donor field and the antidiffusive
- * velocities stay on chip); 0 picks 3 for nonoscillatory grids of at least 2^22 cells
- * and 2 otherwise; grids with more than 2^29 cells per z-plane always take the staged
+ * velocities stay on chip); 0 picks 3 for nonoscillatory grids and 2 otherwise; grids with more than 2^29 cells per z-plane always take the staged
  * path.  FRAME grids: 1 = the per-face
  * antidiffusive kernel and a separate donor pass, otherwise the tiled kernel that forms
  * the donor field and the velocities together (results are identical). */
@@ -134,7 +133,7 @@ int mpd_set_pass_zchunk(mpd_grid g, int zc);
 int mpd_get_pass_zchunk(mpd_grid g, int* zc);
 /* The kernel path the grid's steps take: 1 staged (or donor only), 2 two kernels per
  * corrective pass, 3 one kernel per corrective pass (automatic for nonoscillatory
- * PERIODIC grids of at least 2^22 cells).  FRAME grids: 1 per-face kernels, 2 tiled. */
+ * PERIODIC grids).  FRAME grids: 1 per-face kernels, 2 tiled. */
 int mpd_get_path(mpd_grid g, int* path);
 /* Times every tile shape x z-chunk length on the grid itself (one warm-up step and nsteps
  * timed steps each, each kernel between events of its own), then times whole steps of the
@@ -144,7 +143,9 @@ int mpd_get_path(mpd_grid g, int* path);
  * restored, and the trial steps neither capture graphs nor count for the step sentinel,
  * so the grid's state is unchanged.  Results are cached per process by shape and
  * options, and in the file named by the environment variable MPD_TUNE_CACHE when set.
- * PERIODIC grids with iord >= 2 on the fused path only (MPD_ERR_UNSUPPORTED otherwise). */
+ * PERIODIC grids with iord >= 2 on the fused path only (MPD_ERR_UNSUPPORTED otherwise);
+ * a grid whose steps take one kernel per corrective pass (mpd_get_path 3) has nothing to
+ * tune: it returns MPD_OK with the two-kernel settings unchanged. */
 int mpd_autotune(mpd_grid g, int nsteps, int* tile_x, int* tile_y, int* zc);
 
 /* One PERIODIC step c_out = A(c_in) (+ q) on caller-owned DEVICE arrays that allocates

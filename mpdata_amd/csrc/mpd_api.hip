@@ -727,6 +727,14 @@ extern "C" int mpd_autotune(mpd_grid g, int nsteps, int* tile_x, int* tile_y, in
         return fail(MPD_ERR_UNSUPPORTED, "mpd_autotune: tunes the fused PERIODIC path only");
     if (!g->has_winds || !g->has_field)
         return fail(MPD_ERR_ARG, "mpd_autotune: set the winds (or Courant numbers) and the field first");
+    if (one_kernel_passes(g)) {
+        // the one-kernel pass has a single tile shape and a z-chunk chosen from the size:
+        // nothing to time (its trial steps would misread as the two-kernel pair)
+        if (tile_x) *tile_x = g->tile_x;
+        if (tile_y) *tile_y = g->tile_y;
+        if (zc) *zc = g->zc;
+        return MPD_OK;
+    }
     const std::string opts = "i" + std::to_string(g->iord) + "n" + std::to_string(g->nonosc) +
                              "a" + std::to_string(g->arith) + "q" + std::to_string(g->has_q) +
                              ":" + std::to_string(g->nx) + "x" + std::to_string(g->ny) + "x" +

@@ -97,12 +97,12 @@ int sentinel_check_launch(mpd_grid_s* g);
 // Records a timing event after a kernel launch when g->trace is set.
 void trace_mark(mpd_grid_s* g);
 // Whether a PERIODIC grid's corrective passes run as one kernel each: path 3, or the
-// automatic path on nonoscillatory grids too large for the Infinity Cache (measured on
-// MI355X: 256^3 0.52 -> 0.45 ms, 512x512x256 iord 3 4.29 -> 3.55 ms per step; at 128^3
-// the two-kernel path is faster, 0.067 against 0.072 ms).
+// automatic path on nonoscillatory grids (measured on MI355X, iord 2, against two kernels
+// per pass: 64^3 0.021 / 0.028 ms, 128^3 0.058 / 0.065, 192^3 0.180 / 0.238, 256^3
+// 0.36 / 0.52 ms per step; 512x512x256 iord 3 3.09 / 4.12 ms).
 inline bool one_kernel_passes(const mpd_grid_s* g)
 {
-    return g->nonosc && (g->path == 3 || (g->path == 0 && g->ncell >= (size_t(1) << 22)));
+    return g->nonosc && (g->path == 3 || g->path == 0);
 }
 // The stream a grid's fused step launches on.
 inline hipStream_t launch_stream(const mpd_grid_s* g)

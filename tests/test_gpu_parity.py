@@ -277,6 +277,7 @@ def test_autotune_keeps_state_and_parity():
     U, V, W, c = per_case(nx, ny, nz, 5)
     want = O.per_run(nx, ny, nz, U, V, W, 2, True, c, 3)
     g = M.Grid(nx, ny, nz, M.PERIODIC, iord=2, nonosc=True)
+    g.set_path(2)  # the two-kernel path is the one with launch shapes to tune
     g.set_arith(M.ARITH_EXACT)
     g.set_courant(U, V, W)
     g.set_field(c)
@@ -306,6 +307,7 @@ def test_zchunk_lengths_match_oracle(zc, iord):
     U, V, W, c = per_case(nx, ny, nz, 8)
     want = O.per_run(nx, ny, nz, U, V, W, iord, True, c, 2)
     g = M.Grid(nx, ny, nz, M.PERIODIC, iord=iord, nonosc=True)
+    g.set_path(2)  # the two-kernel path's chunks (test_gpu_onepass.py covers the other's)
     g.set_arith(M.ARITH_EXACT)
     g.set_zchunk(zc)
     g.set_courant(U, V, W)
@@ -315,12 +317,14 @@ def test_zchunk_lengths_match_oracle(zc, iord):
     g.close()
 
 
+@pytest.mark.parametrize("path", [0, 2])
 @pytest.mark.parametrize("iord,nonosc", [(2, True), (2, False), (3, True)])
-def test_graph_replays_match_eager_steps(iord, nonosc):
+def test_graph_replays_match_eager_steps(iord, nonosc, path):
     """mpd_step replays blocks of 8 steps as captured graphs; the same steps launched one
     by one give identical fields -- over several replays with eager work (field reads)
     between them, odd step counts, a tile change that forces a new capture, and an
-    emission array."""
+    emission array; on the automatic path (one kernel per pass with the limiter) and on
+    two kernels per pass."""
     nx, ny, nz = 33, 17, 9
     U, V, W, c = per_case(nx, ny, nz, 12)
     q = np.random.default_rng(3).uniform(0.0, 1e-3, c.size)
@@ -328,6 +332,7 @@ def test_graph_replays_match_eager_steps(iord, nonosc):
         gs = []
         for on in (True, False):
             g = M.Grid(nx, ny, nz, M.PERIODIC, iord=iord, nonosc=nonosc)
+            g.set_path(path)
             g.set_arith(arith)
             g.set_graph(on)
             g.set_courant(U, V, W)
@@ -390,6 +395,7 @@ def test_autotune_reports_both_kernels_shapes():
     nx, ny, nz = 40, 24, 13
     U, V, W, c = per_case(nx, ny, nz, 31)
     g = M.Grid(nx, ny, nz, M.PERIODIC, iord=2, nonosc=True)
+    g.set_path(2)
     g.set_courant(U, V, W)
     g.set_field(c)
     before = g.field()
@@ -404,12 +410,32 @@ def test_autotune_reports_both_kernels_shapes():
     g.close()
 
 
+def test_autotune_leaves_one_kernel_grids_alone():
+    """A grid whose steps take one kernel per corrective pass (the automatic path of a
+    nonoscillatory grid) has no launch shapes to tune: the tuner returns at once, the
+    grid's two-kernel settings, path and state unchanged."""
+    nx, ny, nz = 40, 24, 13
+    U, V, W, c = per_case(nx, ny, nz, 32)
+    g = M.Grid(nx, ny, nz, M.PERIODIC, iord=2, nonosc=True)
+    g.set_courant(U, V, W)
+    g.set_field(c)
+    assert g.path() == 3
+    before = g.tiles()
+    assert g.autotune(1) == before[0]
+    assert g.tiles() == before and g.path() == 3
+    np.testing.assert_array_equal(g.field(), c)
+    g.step(3)
+    assert relerr(g.field(), O.per_run(nx, ny, nz, U, V, W, 2, True, c, 3)) <= TOL
+    g.close()
+
+
 def test_autotune_with_many_passes():
     """iord 11: twenty kernels per step, more than the autotuner's first guess of a
     per-step kernel count; it times them all and the tuned grid matches the oracle."""
     nx, ny, nz = 20, 12, 8
     U, V, W, c = per_case(nx, ny, nz, 41)
     g = M.Grid(nx, ny, nz, M.PERIODIC, iord=11, nonosc=True)
+    g.set_path(2)
     g.set_courant(U, V, W)
     g.set_field(c)
     g.autotune(1)

@@ -105,6 +105,7 @@ def test_autotune_trial_steps_do_not_count():
     dims = (36, 20, 11)  # a shape no other test tunes
     U, V, W, c = case(*dims, 7)
     g = grid(dims, True, U, V, W, c)
+    g.set_path(2)  # the two-kernel path: the one with launch shapes to tune
     g.set_sentinel(4, nonnegative=True)
     g.step(3)
     before = g.field()
