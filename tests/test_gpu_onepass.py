@@ -77,9 +77,10 @@ def test_one_kernel_per_pass_matches_oracle(dims, arith):
 
 
 def test_automatic_path_selection_and_agreement():
-    """The automatic path takes one kernel per pass for nonoscillatory grids and two
-    kernels for unlimited ones; the two paths agree bit for bit in exact arithmetic, and
-    the fast one-kernel path with the chosen z-chunk stays within 1e-12 of them."""
+    """The automatic path takes one kernel per pass for nonoscillatory grids of at least
+    8 planes and two kernels for thinner or unlimited ones; the two paths agree bit for
+    bit in exact arithmetic, and the fast one-kernel path with the chosen z-chunk stays
+    within 1e-12 of them."""
     nx, ny, nz = 256, 128, 128  # 2^22 cells
     U, V, W, c, q = case(nx, ny, nz, 5)
     out = {}
@@ -101,8 +102,8 @@ def test_automatic_path_selection_and_agreement():
     np.testing.assert_array_equal(out[(0, M.ARITH_EXACT)], out[(2, M.ARITH_EXACT)])
     np.testing.assert_array_equal(out[(3, M.ARITH_EXACT)], out[(2, M.ARITH_EXACT)])
     assert relerr(out[(0, M.ARITH_FAST)], out[(2, M.ARITH_EXACT)]) <= 2 * TOL
-    for dims, nonosc, want in (((128, 128, 128), True, 3), ((7, 6, 4), True, 3),
-                               ((256, 128, 128), False, 2)):
+    for dims, nonosc, want in (((128, 128, 128), True, 3), ((7, 6, 8), True, 3),
+                               ((128, 128, 7), True, 2), ((256, 128, 128), False, 2)):
         g = M.Grid(*dims, M.PERIODIC, iord=2, nonosc=nonosc)
         assert g.path() == want
         g.set_path(3)
