@@ -159,8 +159,10 @@ def per_case(nx, ny, nz, seed, cmax=0.2):
 PER_CASES = [(16, 16, 16), (24, 20, 1), (13, 11, 7), (64, 8, 5), (1, 9, 10), (5, 1, 1)]
 
 
-# (path, arith): staged exact, fused exact, fused fast
-PATHS = [(1, M.ARITH_EXACT), (2, M.ARITH_EXACT), (2, M.ARITH_FAST)]
+# (path, arith): staged exact, fused exact, fused fast, automatic (one kernel per pass
+# for the nonoscillatory grids of at least 8 planes, its z-chunk from the cost model)
+PATHS = [(1, M.ARITH_EXACT), (2, M.ARITH_EXACT), (2, M.ARITH_FAST), (0, M.ARITH_EXACT),
+         (0, M.ARITH_FAST)]
 
 
 @pytest.mark.parametrize("path,arith", PATHS)
@@ -235,7 +237,7 @@ def test_periodic_matches_padded_reference_fixture():
             g.close()
 
 
-@pytest.mark.parametrize("path", [1, 2])
+@pytest.mark.parametrize("path", [1, 2, 0])
 def test_periodic_emission_and_long_run(path):
     nx, ny, nz = 48, 40, 32
     U, V, W, c = per_case(nx, ny, nz, 5, cmax=0.25)
