@@ -685,10 +685,6 @@ extern "C" int mpd_set_pass_zchunk(mpd_grid g, int zc)
     return MPD_OK;
 }
 
-namespace mpd {
-int pass_zchunk(const mpd_grid_s* g);
-}
-
 extern "C" int mpd_get_pass_zchunk(mpd_grid g, int* zc)
 {
     GRID_OR_FAIL(g);

@@ -72,6 +72,8 @@ int fused_batch_launch_f12_exact(const FusedArgs* table, const int* item0, int n
                                  int nitems, bool donor, bool emit, bool nt, hipStream_t s);
 int fused_batch_launch_f12_fast(const FusedArgs* table, const int* item0, int ninst,
                                 int nitems, bool donor, bool emit, bool nt, hipStream_t s);
+// z-chunk length of a grid's one-kernel passes: its own setting, else the cost model's.
+int pass_zchunk(const mpd_grid_s* g);
 // One exact-arithmetic iord-2 step cin -> cout of a PERIODIC grid that also stores the
 // forward's intermediates for the adjoint: the donor field c1, the antidiffusive faces
 // A[3] and, with the limiter, its data (see FusedArgs; null pointers when unlimited).

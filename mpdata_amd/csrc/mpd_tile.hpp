@@ -158,6 +158,11 @@ struct ZCursor {
 // became a 64-bit vector value and every load and store needed a 64-bit vector add.
 __device__ __forceinline__ void keep32(unsigned& off) { asm volatile("" : "+v"(off)); }
 
+// A value formed before a conditional use: without it the compiler sinks the whole
+// computation (its LDS reads included) into the branch, after everything else in the
+// block, where it cannot overlap the other stages.
+__device__ __forceinline__ void keep_v(double& x) { asm volatile("" : "+v"(x)); }
+
 // Element of a plane at byte offset colb of the thread's column (a 32-bit offset: the
 // host keeps every plane below 4 GiB), so the access is a scalar base plus a vector
 // offset.
