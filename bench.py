@@ -192,6 +192,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3,
                     help="oracle steps of the headline workload: the CPU baseline sample "
                          "and the parity reference")
+    ap.add_argument("--repeats", type=int, default=5,
+                    help="further timed runs of K headline steps after the timed one, "
+                         "reported with their median beside the value (0: none)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="N")
     ap.add_argument("--ring3d", type=int, default=128,
@@ -298,17 +301,22 @@ def headline(M, args):
     wall = time.perf_counter() - t0
     kms = g.kernel_times(2)
     mass, cmin, _ = g.stats()
+    # the state after the warm-up, timed and kernel-timing steps (compared below with the
+    # same steps in exact arithmetic)
+    fast = None if args.profile else g.field()
+    # the spread of the figure: R more runs of K steps, each timed the same way
+    reps = [] if args.profile else [g.step_timed(args.steps) / args.steps
+                                    for _ in range(args.repeats)]
     ms_per_step = ev_ms / args.steps
     value = cells * args.steps / (ev_ms * 1e-3)
     kernels = step_kernels(M, 3, kms)
     tfile = os.path.join(ROOT, "profiles", "traffic_deform512_iord3.json")
     roof = roofline(cells, kms, kernels, tfile if args.arith == 1 else None, tuned)
-    # the whole trajectory (warm-up, timed and kernel-timing steps; the autotuner restores
-    # the state it found) again in exact arithmetic, which equals the oracle's bit for bit
+    # those steps (the autotuner restores the state it found) again in exact arithmetic,
+    # which equals the oracle's bit for bit
     taken = args.warmup + args.steps + 2
     err_run = None
     if not args.profile:
-        fast = g.field()
         g.set_arith(M.ARITH_EXACT)
         g.set_field(c0)
         g.step(taken)
@@ -339,6 +347,10 @@ def headline(M, args):
         "wall_s_timed": wall,
         "mass_rel_drift": abs(mass - mass0) / mass0, "min": cmin,
     }
+    if reps:
+        line["repeats"] = {"runs": len(reps), "steps_each": args.steps,
+                           "ms_per_step": reps, "median_ms_per_step": float(np.median(reps)),
+                           "median_cell_updates_per_s": cells / (float(np.median(reps)) * 1e-3)}
     par = {f"steps_{taken}_vs_exact_kernels": {"rel_err": err_run}} if err_run is not None else {}
     if not args.no_cpu and args.cpu_steps > 0:
         ks = args.cpu_steps
