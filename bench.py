@@ -522,9 +522,9 @@ def ensemble_leg(M, n):
     ms_per = b.step_timed(10, M.BATCH_PER_INSTANCE) / 10
     taken = 2 + 100 + 10 + 1 + 10
     mass_ok = all(np.isfinite(g.stats()[0]) for g in b.instances[:4])
-    # against a single grid of each size stepped alone (its own path: two kernels per pass
-    # at these sizes; the batched graph takes one, so the fast arithmetic agrees to
-    # rounding), and the smallest instance against the oracle over the whole run
+    # against a single grid of each size stepped alone (one kernel per pass too, in the
+    # z-chunks of its own size: the same arithmetic per cell), and the smallest instance
+    # against the oracle over the whole run
     worst, err = 0.0, None
     path = b.path()
     for i in firsts:

@@ -265,11 +265,15 @@ int mpd_batch_get_tiles(mpd_batch b, int* v);
 int mpd_batch_set_path(mpd_batch b, int path);
 /* The path the batched launches take (2 or 3). */
 int mpd_batch_get_path(mpd_batch b, int* path);
-/* z-chunk of the one-kernel batched launches (0: min(nz, 64) per instance). */
+/* z-chunk of the one-kernel batched launches (0: 32 planes, at most each instance's
+ * height). */
 int mpd_batch_set_pass_zchunk(mpd_batch b, int zc);
 /* Times every tile shape x z-chunk x grid kind over nsteps steps on the batch itself
  * (fields restored afterwards), applies and returns the fastest (any output may be
- * NULL).  Cached per process; also in the file named by MPD_TUNE_CACHE when set. */
+ * NULL).  A batch on one kernel per pass has one tile shape (32 x 16, persistent 0):
+ * there the batched launches' z-chunk is timed (16 .. 128 planes, up to the tallest
+ * instance) and returned in zc.  Cached per process; also in the file named by
+ * MPD_TUNE_CACHE when set. */
 int mpd_batch_autotune(mpd_batch b, int nsteps, int* tile_x, int* tile_y, int* zc,
                        int* persistent);
 int mpd_batch_step(mpd_batch b, int nsteps, int mode);

@@ -38,7 +38,7 @@ struct mpd_batch_s {
     int tx = 32, ty = 8, zc = 16, persistent = 1;
     int tx2 = 0, ty2 = 0, zc2 = 0;  // the limiter kernel's launch shape (0: F1's)
     int path = 0;  // 0 automatic, 2 two kernels per pass, 3 one kernel per pass
-    int zc3 = 0;   // z-chunk of the one-kernel launches (0: min(nz, 64) per instance)
+    int zc3 = 0;   // z-chunk of the one-kernel launches (0: kPassZchunk)
     int cur = 0;
     long long cells = 0;
     std::vector<mpd_grid_s*> g;
@@ -67,6 +67,12 @@ struct mpd_batch_s {
 };
 
 namespace {
+
+// Default z-chunk of the one-kernel batched launches (at most an instance's height).  On
+// the 192-instance ensemble of 64-128-plane instances: 32 planes 7.82 ms per step, 40
+// 7.83, 64 7.98, 96 8.00, 24 8.37, 128 8.75, 16 8.73 -- chunks that divide every
+// instance evenly give every item the same length.
+constexpr int kPassZchunk = 32;
 
 // Whether the batched launches run one kernel per corrective pass: path 3, or the
 // automatic path when the ensemble's arrays exceed the Infinity Cache (2^22 cells), as
@@ -122,7 +128,7 @@ int build_tables(mpd_batch_s* b)
             geo.nz = g->nz;
             geo.np = (long long)g->nx * g->ny;
             geo.nt = fused_streaming((size_t)b->cells);
-            geo.zc = std::min(b->zc3 ? b->zc3 : 64, g->nz);
+            geo.zc = std::min(b->zc3 ? b->zc3 : kPassZchunk, g->nz);
             fused_geometry(geo, F12_TILE_X, F12_TILE_Y);
             b->hitem0[i] = items;
             items += fused_items(geo);
@@ -545,14 +551,49 @@ extern "C" int mpd_batch_step_timed(mpd_batch b, int nsteps, int mode, double* m
     return rc;
 }
 
+namespace {
+
+// The instances' current fields, copied aside (save holds one buffer per instance) before
+// timed trials and put back after.
+int save_fields(mpd_batch_s* b, std::vector<DevBuf>& save)
+{
+    int rc;
+    hipStream_t s = stream();
+    for (int i = 0; i < b->ninst; i++) {
+        mpd_grid_s* g = b->g[i];
+        if ((rc = save[i].alloc(g->ncell))) return rc;
+        MPD_HIP(hipMemcpyAsync(save[i].p, g->c[g->cur].p, g->ncell * 8, hipMemcpyDeviceToDevice,
+                               s));
+    }
+    return MPD_OK;
+}
+
+int restore_fields(mpd_batch_s* b, const std::vector<DevBuf>& save, int start)
+{
+    hipStream_t s = stream();
+    set_parity(b, start);
+    for (int i = 0; i < b->ninst; i++) {
+        mpd_grid_s* g = b->g[i];
+        MPD_HIP(hipMemcpyAsync(g->c[g->cur].p, save[i].p, g->ncell * 8, hipMemcpyDeviceToDevice,
+                               s));
+    }
+    MPD_HIP(hipStreamSynchronize(s));
+    return MPD_OK;
+}
+
+}  // namespace
+
 // Autotuner: times every compiled tile shape x z-chunk x grid kind on the batch itself
 // (MPD_BATCH_FUSED, nsteps steps each after one warm-up step) and keeps the fastest; then,
 // with that kept for the first kernel, every shape x z-chunk for the limiter kernel
 // alone, and with the best of those kept, every shape x z-chunk for the first kernel
-// again.  The fields are saved before and restored after, so tuning leaves the ensemble
-// unchanged.  Results are cached per process by the ensemble's shape signature; with
-// MPD_TUNE_CACHE set they are also appended to / read from that file ("batch2" entries
-// hold both kernels' shapes; a "batch" entry serves both kernels).
+// again.  A batch on one kernel per pass has one tile shape: there the z-chunk of the
+// batched launches is timed (16 .. 128 planes, up to the tallest instance).  The fields
+// are saved before and restored after, so tuning leaves the ensemble unchanged.  Results
+// are cached per process by the ensemble's shape signature; with MPD_TUNE_CACHE set they
+// are also appended to / read from that file ("batch2" entries hold both kernels'
+// shapes, a "batch" entry serves both kernels, "batch3" entries hold the one-kernel
+// z-chunk).
 extern "C" int mpd_batch_autotune(mpd_batch b, int nsteps, int* tx, int* ty, int* zc,
                                   int* persistent)
 {
@@ -566,12 +607,37 @@ extern "C" int mpd_batch_autotune(mpd_batch b, int nsteps, int* tx, int* ty, int
     const std::string key = "batch2:" + opts;
     int rc;
     int c[7];
+    // one timed trial of the configuration set on the batch
+    auto trial = [&](double& ms) {
+        int r = mpd_batch_step(b, 1, MPD_BATCH_FUSED);
+        return r ? r : mpd_batch_step_timed(b, nsteps, MPD_BATCH_FUSED, &ms);
+    };
     if (one_kernel(b)) {
-        // the one-kernel launches have a single tile shape; nothing to tune
-        if (tx) *tx = b->tx;
-        if (ty) *ty = b->ty;
-        if (zc) *zc = b->zc;
-        if (persistent) *persistent = b->persistent;
+        int z3 = 0;
+        if (!tune_lookup("batch3:" + opts, 1, &z3)) {
+            int tallest = 1;
+            for (auto* g : b->g) tallest = std::max(tallest, g->nz);
+            std::vector<DevBuf> save(b->ninst);
+            const int start = b->cur;
+            if ((rc = save_fields(b, save))) return rc;
+            double best = 1e300, ms = 0;
+            for (int z : {16, 24, 32, 48, 64, 96, 128}) {
+                if (z > tallest) break;
+                b->zc3 = z;
+                if ((rc = trial(ms))) return rc;
+                if (ms < best) {
+                    best = ms;
+                    z3 = z;
+                }
+            }
+            if ((rc = restore_fields(b, save, start))) return rc;
+            tune_store("batch3:" + opts, 1, &z3);
+        }
+        b->zc3 = z3;
+        if (tx) *tx = F12_TILE_X;
+        if (ty) *ty = F12_TILE_Y;
+        if (zc) *zc = z3 ? z3 : kPassZchunk;
+        if (persistent) *persistent = 0;
         b->tuned = true;
         return MPD_OK;
     }
@@ -580,22 +646,12 @@ extern "C" int mpd_batch_autotune(mpd_batch b, int nsteps, int* tx, int* ty, int
             c[4] = c[5] = c[6] = 0;
         } else {
             std::vector<DevBuf> save(b->ninst);
-            hipStream_t s = stream();
-            for (int i = 0; i < b->ninst; i++) {
-                mpd_grid_s* g = b->g[i];
-                if ((rc = save[i].alloc(g->ncell))) return rc;
-                MPD_HIP(hipMemcpyAsync(save[i].p, g->c[g->cur].p, g->ncell * 8, hipMemcpyDeviceToDevice, s));
-            }
             const int start = b->cur;
+            if ((rc = save_fields(b, save))) return rc;
             const int shapes[][2] = {
 #define MPD_TILE(X, Y) {X, Y},
                 MPD_FUSED_TILES(MPD_TILE)
 #undef MPD_TILE
-            };
-            // one timed trial of the configuration set on the batch
-            auto trial = [&](double& ms) {
-                int r = mpd_batch_step(b, 1, MPD_BATCH_FUSED);
-                return r ? r : mpd_batch_step_timed(b, nsteps, MPD_BATCH_FUSED, &ms);
             };
             double best = 1e300, ms = 0;
             int f1[4] = {b->tx, b->ty, b->zc, b->persistent}, f2[3] = {0, 0, 0};
@@ -631,12 +687,7 @@ extern "C" int mpd_batch_autotune(mpd_batch b, int nsteps, int* tx, int* ty, int
                             f1[0] = sh[0], f1[1] = sh[1], f1[2] = z;
                         }
                     }
-            set_parity(b, start);
-            for (int i = 0; i < b->ninst; i++) {
-                mpd_grid_s* g = b->g[i];
-                MPD_HIP(hipMemcpyAsync(g->c[g->cur].p, save[i].p, g->ncell * 8, hipMemcpyDeviceToDevice, s));
-            }
-            MPD_HIP(hipStreamSynchronize(s));
+            if ((rc = restore_fields(b, save, start))) return rc;
             for (int i = 0; i < 4; i++) c[i] = f1[i];
             for (int i = 0; i < 3; i++) c[4 + i] = f2[i];
         }

@@ -197,6 +197,30 @@ def test_autotune_restores_fields():
     b.close()
 
 
+def test_one_kernel_autotune_times_the_zchunk():
+    """On one kernel per pass the autotuner times the batched launches' z-chunk: it
+    reports the kernel's tile, a candidate chunk no taller than the tallest instance (or
+    the default when every instance is shorter than the smallest candidate), leaves the
+    fields as they were, and the tuned launches equal per-instance launches bit for bit."""
+    b, _ = make_batch(2, M.ARITH_EXACT)
+    b.set_path(3)
+    before = [g.field() for g in b.instances]
+    tx, ty, zc, pers = b.autotune(1)
+    tallest = max(nz for _, _, nz in DIMS)
+    assert (tx, ty, pers) == (32, 16, 0)
+    assert zc in [z for z in (16, 24, 32, 48, 64, 96, 128) if z <= tallest] or (
+        zc == 32 and tallest < 16)
+    for g, f in zip(b.instances, before):
+        np.testing.assert_array_equal(g.field(), f)
+    ref, _ = make_batch(2, M.ARITH_EXACT)
+    b.step(3, M.BATCH_GRAPH)
+    ref.step(3, M.BATCH_PER_INSTANCE)
+    for a, c in zip(b.instances, ref.instances):
+        np.testing.assert_array_equal(a.field(), c.field())
+    b.close()
+    ref.close()
+
+
 def test_small_ensemble_config_runs():
     b = K.ensemble(6, seed=4)
     m0 = [g.stats()[0] for g in b.instances]
