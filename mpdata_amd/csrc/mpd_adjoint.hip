@@ -422,27 +422,57 @@ __device__ __forceinline__ int ld_code(const int* plane, unsigned colb)
     return *(const int*)((const char*)plane + colb / 2);
 }
 
+// The LDS planes of k_adj_lim_t.
+template <int R>
+struct LimLds {
+    double lo[R], bu[R], bd[R], c[3][R], a0[3][R], a1[3][R];
+    double iu[2][R], id[2][R], li[2][R], lx[2][R];
+    int cd[2][R];
+};
+
+// Ring-2 cell j of a TX x TY tile's region without its four corners, which no stage
+// reads (stage 1 on ring 1 reads the four in-plane neighbours only): the rows below and
+// above ring 1, then the two side columns.
 template <int TX, int TY>
-__global__ void __launch_bounds__((tile::GeoB<TX, TY>::NTH)) __attribute__((amdgpu_waves_per_eu(2)))
-k_adj_lim_t(LimArgs a)
+__device__ __forceinline__ void ring2_nc(int j, int& rx, int& ry)
+{
+    if (j < 2 * (TX + 2)) {
+        ry = j < TX + 2 ? 0 : TY + 3;
+        rx = 1 + j % (TX + 2);
+    } else {
+        const int r = j - 2 * (TX + 2);
+        ry = 1 + r / 2;
+        rx = (r & 1) ? TX + 3 : 0;
+    }
+}
+
+// One role's share of a work item.  TILE: lane = tile cell (stages 1 and 2, the stores).
+// !TILE: lane of the ring wave, with its ring-1 cell (stage 1; the last lanes have none
+// and load a duplicate column) and its ring-2 cell (loads and publishes only).  The two
+// roles are separate code paths, so no thread holds the rolling registers of two
+// columns and the tile path's stage 2 does not share its registers with ring-2 loads.
+template <int TX, int TY, bool TILE>
+__device__ __forceinline__ void lim_part(const LimArgs& a, int item, int lane,
+                                         LimLds<tile::GeoB<TX, TY>::R>& sm)
 {
     using G = tile::GeoB<TX, TY>;
     using tile::ld;
     using tile::st_nt;  // streaming: the adjoint's arrays far exceed the caches at its sizes
-    constexpr int RX = G::RX, R = G::R;
-    __shared__ double s_lo[R], s_bu[R], s_bd[R], s_c[3][R], s_a0[3][R], s_a1[3][R];
-    __shared__ double s_iu[2][R], s_id[2][R], s_li[2][R], s_lx[2][R];
-    __shared__ int s_cd[2][R];
-
-    const int item = tile::xcd_item(blockIdx.x, gridDim.x);
-    const int t = G::role(item);
+    constexpr int RX = G::RX;
     const int bx = item % a.tiles_x, by = (item / a.tiles_x) % a.tiles_y;
     const int bz = item / (a.tiles_x * a.tiles_y);
     // own cell (tile: stages 1 and 2; ring 1: stage 1) and ring-2 cell (loads)
-    int ox, oy, sx, sy;
-    const int role = G::own(t, ox, oy);
-    const bool comp = role >= 0, tcell = role == 0;
-    const bool sec = G::second(t, sx, sy);
+    int ox, oy, sx = 0, sy = 0;
+    bool comp = true;
+    if constexpr (TILE) {
+        ox = 2 + lane % TX;
+        oy = 2 + lane / TX;
+    } else {
+        comp = lane < G::H1;
+        G::G::halo(comp ? lane : 0, ox, oy);
+        ring2_nc<TX, TY>(lane, sx, sy);
+    }
+    constexpr bool tcell = TILE, sec = !TILE;
     const int L = oy * RX + ox, S = sy * RX + sx;
     unsigned colb = tile::column<TX, TY>(a.nx, a.ny, bx, by, ox, oy);
     unsigned scolb = tile::column<TX, TY>(a.nx, a.ny, bx, by, sx, sy);
@@ -513,12 +543,12 @@ k_adj_lim_t(LimArgs a)
         q.cd = q.ncd;
     };
     LimCol P, H;
-    if (comp) init(P, colb, true);
+    init(P, colb, true);
     if (sec) init(H, scolb, false);
     // c1 of plane z0 - 2: stage 1 of the first iteration reads it from LDS (the loop
     // publishes planes from z0 - 1 on)
-    if (comp) s_c[(z0 + 1) % 3][L] = P.c1[0];
-    if (sec) s_c[(z0 + 1) % 3][S] = H.c1[0];
+    if (comp) sm.c[(z0 + 1) % 3][L] = P.c1[0];
+    if (sec) sm.c[(z0 + 1) % 3][S] = H.c1[0];
 
     // tile cell: stage-1 results of planes p-2, p-1 (index 0, 1; 2 = plane p), the codes,
     // the carried donor sum and lam_Al * m of plane p-1, c1 and A2 of plane p-2
@@ -535,26 +565,26 @@ k_adj_lim_t(LimArgs a)
         const int s3 = (p + 3) % 3, k3 = (p + 2) % 3;  // slots of planes p, p-1 (c1, A0, A1)
         const int s2 = (p + 2) % 2, k2 = (p + 1) % 2;  // slots of planes p, p-1 (stage 1)
         if (comp) {
-            s_lo[L] = P.lo[1];
-            s_bu[L] = P.bu[1];
-            s_bd[L] = P.bd[1];
-            s_c[s3][L] = P.c1[1];
-            s_a0[s3][L] = P.a0;
-            s_a1[s3][L] = P.a1;
+            sm.lo[L] = P.lo[1];
+            sm.bu[L] = P.bu[1];
+            sm.bd[L] = P.bd[1];
+            sm.c[s3][L] = P.c1[1];
+            sm.a0[s3][L] = P.a0;
+            sm.a1[s3][L] = P.a1;
         }
         if (sec) {
-            s_lo[S] = H.lo[1];
-            s_bu[S] = H.bu[1];
-            s_bd[S] = H.bd[1];
-            s_c[s3][S] = H.c1[1];
-            s_a0[s3][S] = H.a0;
-            s_a1[s3][S] = H.a1;
+            sm.lo[S] = H.lo[1];
+            sm.bu[S] = H.bu[1];
+            sm.bd[S] = H.bd[1];
+            sm.c[s3][S] = H.c1[1];
+            sm.a0[s3][S] = H.a0;
+            sm.a1[s3][S] = H.a1;
         }
         tile::lds_barrier();
         // stage 1 on cell M of plane p (the merged k_adj_donor_limit, same operation order)
         auto stage1 = [&](int M, const LimCol& q, double& dacc, double lam[3], double& iu,
                           double& id, double& li, double& lx) {
-            const double* C = s_c[s3];
+            const double* C = sm.c[s3];
             const double lt = q.lo[1], ct = q.c1[1], bu_t = q.bu[1], bd_t = q.bd[1];
             double acc = lt, su = 0.0, sd = 0.0;
             double fl[3], fu[3];  // the unlimited donor fluxes of the lower / upper faces
@@ -564,9 +594,9 @@ k_adj_lim_t(LimArgs a)
                 {  // upper face (M, r)
                     double a_, lo_r, c_r, bu_r, bd_r;
                     if (d == 0) {
-                        a_ = q.a0; lo_r = s_lo[M + 1]; c_r = C[M + 1]; bu_r = s_bu[M + 1]; bd_r = s_bd[M + 1];
+                        a_ = q.a0; lo_r = sm.lo[M + 1]; c_r = C[M + 1]; bu_r = sm.bu[M + 1]; bd_r = sm.bd[M + 1];
                     } else if (d == 1) {
-                        a_ = q.a1; lo_r = s_lo[M + RX]; c_r = C[M + RX]; bu_r = s_bu[M + RX]; bd_r = s_bd[M + RX];
+                        a_ = q.a1; lo_r = sm.lo[M + RX]; c_r = C[M + RX]; bu_r = sm.bu[M + RX]; bd_r = sm.bd[M + RX];
                     } else {
                         a_ = q.a2c; lo_r = q.lo[2]; c_r = q.c1[2]; bu_r = q.bu[2]; bd_r = q.bd[2];
                     }
@@ -590,11 +620,11 @@ k_adj_lim_t(LimArgs a)
                 {  // lower face (l, M)
                     double a_, lo_l, c_l, bu_l, bd_l;
                     if (d == 0) {
-                        a_ = s_a0[s3][M - 1]; lo_l = s_lo[M - 1]; c_l = C[M - 1]; bu_l = s_bu[M - 1]; bd_l = s_bd[M - 1];
+                        a_ = sm.a0[s3][M - 1]; lo_l = sm.lo[M - 1]; c_l = C[M - 1]; bu_l = sm.bu[M - 1]; bd_l = sm.bd[M - 1];
                     } else if (d == 1) {
-                        a_ = s_a1[s3][M - RX]; lo_l = s_lo[M - RX]; c_l = C[M - RX]; bu_l = s_bu[M - RX]; bd_l = s_bd[M - RX];
+                        a_ = sm.a1[s3][M - RX]; lo_l = sm.lo[M - RX]; c_l = C[M - RX]; bu_l = sm.bu[M - RX]; bd_l = sm.bd[M - RX];
                     } else {
-                        a_ = q.a2m; lo_l = q.lo[0]; c_l = s_c[k3][M]; bu_l = q.bu[0]; bd_l = q.bd[0];
+                        a_ = q.a2m; lo_l = q.lo[0]; c_l = sm.c[k3][M]; bu_l = q.bu[0]; bd_l = q.bd[0];
                     }
                     fl[d] = 0.5 * F2(c_l, ct, a_);
                     const bool pf = a_ > 0, nf = a_ < 0;
@@ -632,25 +662,25 @@ k_adj_lim_t(LimArgs a)
             // the codes of plane p go to LDS here, after the first barrier, like the
             // stage-1 results: their slot held plane p - 2, which stage 2 of the previous
             // iteration may still be reading until every wave has reached that barrier
-            s_cd[s2][L] = P.cd;
-            s_iu[s2][L] = iu;
-            s_id[s2][L] = id;
-            s_li[s2][L] = li;
-            s_lx[s2][L] = lx;
+            sm.cd[s2][L] = P.cd;
+            sm.iu[s2][L] = iu;
+            sm.id[s2][L] = id;
+            sm.li[s2][L] = li;
+            sm.lx[s2][L] = lx;
         }
         if (store && p >= z0 && p < z1 && a.lq) st_nt(a.lq + zp0.off, colb, lq_p + P.lo[1]);
         tile::lds_barrier();
         // stage 2 on the tile cell of plane k = p - 1 (k_adj_limit3, same operation order)
         const int k = p - 1;
         if (tcell && k >= z0 && k < z1) {
-            const double* C = s_c[k3];
-            const double *IU = s_iu[k2], *ID = s_id[k2], *LI = s_li[k2], *LX = s_lx[k2];
-            const int* CD = s_cd[k2];
+            const double* C = sm.c[k3];
+            const double *IU = sm.iu[k2], *ID = sm.id[k2], *LI = sm.li[k2], *LX = sm.lx[k2];
+            const int* CD = sm.cd[k2];
             const double ct = C[L];  // c1 of plane k
             // the cell's own column: stage-1 results and codes of planes k (LDS slot k2),
             // k + 1 (slot s2) and k - 1 (registers)
-            const double iu_k = IU[L], id_k = ID[L], iu_n = s_iu[s2][L], id_n = s_id[s2][L];
-            const int cd_k = CD[L], cd_n = s_cd[s2][L];
+            const double iu_k = IU[L], id_k = ID[L], iu_n = sm.iu[s2][L], id_n = sm.id[s2][L];
+            const int cd_k = CD[L], cd_n = sm.cd[s2][L];
             // neighbour y, the code q under which y sees t, whether y is t itself (extents
             // of 1 wrap a neighbour onto the cell)
             double s0 = 0.0, s1 = 0.0;
@@ -691,11 +721,11 @@ k_adj_lim_t(LimArgs a)
                 {  // upper face (t, r)
                     double a_, c_r, li_r, lx_r;
                     if (d == 0) {
-                        a_ = s_a0[k3][L]; c_r = C[L + 1]; li_r = LI[L + 1]; lx_r = LX[L + 1];
+                        a_ = sm.a0[k3][L]; c_r = C[L + 1]; li_r = LI[L + 1]; lx_r = LX[L + 1];
                     } else if (d == 1) {
-                        a_ = s_a1[k3][L]; c_r = C[L + RX]; li_r = LI[L + RX]; lx_r = LX[L + RX];
+                        a_ = sm.a1[k3][L]; c_r = C[L + RX]; li_r = LI[L + RX]; lx_r = LX[L + RX];
                     } else {
-                        a_ = P.a2m; c_r = s_c[s3][L]; li_r = s_li[s2][L]; lx_r = s_lx[s2][L];
+                        a_ = P.a2m; c_r = sm.c[s3][L]; li_r = sm.li[s2][L]; lx_r = sm.lx[s2][L];
                     }
                     // (lx_t [f>0] - li_t [f<0]) + (li_r [f>0] - lx_r [f<0]), the zero
                     // terms dropped (the same value up to the sign of a zero)
@@ -708,11 +738,11 @@ k_adj_lim_t(LimArgs a)
                 {  // lower face (l, t)
                     double a_, c_l, li_l, lx_l;
                     if (d == 0) {
-                        a_ = s_a0[k3][L - 1]; c_l = C[L - 1]; li_l = LI[L - 1]; lx_l = LX[L - 1];
+                        a_ = sm.a0[k3][L - 1]; c_l = C[L - 1]; li_l = LI[L - 1]; lx_l = LX[L - 1];
                     } else if (d == 1) {
-                        a_ = s_a1[k3][L - RX]; c_l = C[L - RX]; li_l = LI[L - RX]; lx_l = LX[L - RX];
+                        a_ = sm.a1[k3][L - RX]; c_l = C[L - RX]; li_l = LI[L - RX]; lx_l = LX[L - RX];
                     } else {
-                        a_ = a2_m2; c_l = s_c[(p + 1) % 3][L]; li_l = sli0; lx_l = slx0;
+                        a_ = a2_m2; c_l = sm.c[(p + 1) % 3][L]; li_l = sli0; lx_l = slx0;
                     }
                     const double f = 0.5 * F2(c_l, ct, a_);
                     const double lf = f > 0 ? lx_l + li_t : (f < 0 ? -(li_l + lx_t) : 0.0);
@@ -730,17 +760,19 @@ k_adj_lim_t(LimArgs a)
         }
         // shift the tile cell's history by one plane: plane p - 1 becomes p - 2 (its LDS
         // slot is overwritten by the next iteration's stage 1)
-        siu0 = s_iu[k2][L];
-        sid0 = s_id[k2][L];
-        sli0 = s_li[k2][L];
-        slx0 = s_lx[k2][L];
-        scd0 = s_cd[k2][L];
-        dacc_m = dacc;
-        lam_m[0] = lam[0];
-        lam_m[1] = lam[1];
-        lam_m[2] = lam[2];
-        a2_m2 = P.a2m;
-        if (comp) rotate(P);
+        if constexpr (TILE) {
+            siu0 = sm.iu[k2][L];
+            sid0 = sm.id[k2][L];
+            sli0 = sm.li[k2][L];
+            slx0 = sm.lx[k2][L];
+            scd0 = sm.cd[k2][L];
+            dacc_m = dacc;
+            lam_m[0] = lam[0];
+            lam_m[1] = lam[1];
+            lam_m[2] = lam[2];
+            a2_m2 = P.a2m;
+        }
+        rotate(P);
         if (sec) rotate(H);
         zp2.next();
         zp1.next();
@@ -748,11 +780,31 @@ k_adj_lim_t(LimArgs a)
         zm1.next();
         if (p + 2 <= z1) {  // for the rotation after the next iteration: fields at p + 3,
                             // faces at p + 2 (the cursors now stand one plane higher)
-            if (comp) load_next(P, colb, true, zp2, zp1);
+            load_next(P, colb, true, zp2, zp1);
             if (sec) load_next(H, scolb, false, zp2, zp1);
         }
         if (store && p + 1 >= z0 && p + 1 < z1 && a.lq) lq_p = ld(a.lq + zp0.off, colb);
     }
+}
+
+// One workgroup per (tile, z-chunk): NT/64 tile waves and one ring wave.  The role
+// numbering rotates by whole waves from item to item, so the ring wave (lighter: stage 1
+// only) falls on a different SIMD in the workgroups that share a CU.  The LDS (53,760 B
+// at 16x12) and the tile path's registers allow three workgroups per CU.
+template <int TX, int TY>
+__global__ void __launch_bounds__((tile::GeoB<TX, TY>::NTH)) __attribute__((amdgpu_waves_per_eu(3)))
+k_adj_lim_t(LimArgs a)
+{
+    using G = tile::GeoB<TX, TY>;
+    static_assert(G::NTH == G::NT + 64 && G::H1 <= 64, "one ring wave holds ring 1");
+    static_assert(2 * (TX + 2) + 2 * (TY + 2) == 64, "one ring-2 cell per ring lane");
+    __shared__ LimLds<G::R> sm;
+    const int item = tile::xcd_item(blockIdx.x, gridDim.x);
+    const int t = __builtin_amdgcn_readfirstlane(G::role(item)) + (int)(threadIdx.x & 63);
+    if (t < G::NT)
+        lim_part<TX, TY, true>(a, item, t, sm);
+    else
+        lim_part<TX, TY, false>(a, item, t - G::NT, sm);
 }
 
 int lim_tiled(const mpd_grid_s* g, const double* const in[6], const int* code,
