@@ -57,6 +57,62 @@ __device__ __forceinline__ int pick(double a, double b)
     return 1.0 < m ? 0 : w;
 }
 
+// Both sides of the stage-1 adjoint of a z face between a cell t and the cell r above it
+// in the same column (the donor gather and the limiter terms of k_adj_lim_t): the terms
+// t adds for its upper face and those r adds for its lower face.  They are the same
+// face, so the same flux, selection and products; the tiled kernel forms them once per
+// face and carries r's share to the next plane of the column.
+struct ZFace {
+    double f;              // the unlimited donor flux 0.5 F2
+    double glf, sdu, suu;  // t: donor-gather term, sd and su terms (selection w = 1)
+    double lam;            // lambda of the limited face
+    double grlf, sul, sdl; // r: donor-gather term, su and sd terms (selection w = 2)
+};
+
+__device__ __forceinline__ ZFace zface1(double ct, double c_r, double a_, double lt,
+                                        double lo_r, double bu_t, double bd_t, double bu_r,
+                                        double bd_r)
+{
+    ZFace z;
+    z.f = 0.5 * F2(ct, c_r, a_);
+    const bool pf = a_ > 0, nf = a_ < 0;
+    const double x = pf ? bd_t : bu_t, y = pf ? bu_r : bd_r;
+    const int w = pick(x, y);
+    const double m = (pf || nf) ? (w == 0 ? 1.0 : (w == 1 ? x : y)) : 0.0;
+    const double au = (pf || nf) ? m * a_ : 0.0;
+    double gl, gr, gv;
+    fpart(ct, c_r, au, gl, gr, gv);
+    const double lf = lo_r - lt;
+    z.glf = gl * lf;
+    z.grlf = gr * lf;
+    const double Lf = gv * lf;
+    const double sel1 = (w == 1 && (pf || nf)) ? a_ * Lf : 0.0;
+    const double sel2 = (w == 2 && (pf || nf)) ? a_ * Lf : 0.0;
+    z.sdu = pf ? sel1 : 0.0;
+    z.suu = nf ? sel1 : 0.0;
+    z.sul = pf ? sel2 : 0.0;
+    z.sdl = nf ? sel2 : 0.0;
+    z.lam = m * Lf;
+    return z;
+}
+
+// Stage 2 of k_adj_lim_t on the z face between a cell t and the cell r above it (the
+// flux-sum adjoint): t's donor-gather term gl*lf and its face term gv*lf, and the term
+// gr*lf that r adds for its lower face -- formed once per face, r's share carried.
+__device__ __forceinline__ void zface2(double ct, double c_r, double a_, double li_t,
+                                       double lx_t, double li_r, double lx_r, double& glf,
+                                       double& gvlf, double& grlf)
+{
+    // (lx_t [f>0] - li_t [f<0]) + (li_r [f>0] - lx_r [f<0]), the zero terms dropped
+    const double f = 0.5 * F2(ct, c_r, a_);
+    const double lf = f > 0 ? lx_t + li_r : (f < 0 ? -(li_t + lx_r) : 0.0);
+    double gl, gr, gv;
+    fpart(ct, c_r, a_, gl, gr, gv);
+    glf = gl * lf;
+    gvlf = gv * lf;
+    grlf = gr * lf;
+}
+
 struct Geo {
     int nx, ny, nz;
     size_t np, n;
@@ -398,7 +454,9 @@ __global__ void k_adj_anti_atomic(Geo g, AdjBufs b, const double* __restrict__ l
 // 7-point neighbourhood and the flux-sum adjoints of the six faces).  LDS slots: lout,
 // bup, bdn one plane (read in stage 1 only), c1, A0, A1 three (stage 1 of plane p, stage
 // 2 of plane p - 1), the stage-1 results and the codes two (both written after the
-// first barrier: stage 2 of the previous iteration reads the other slot).
+// first barrier: stage 2 of the previous iteration reads the other slot).  A z face lies
+// between two planes of one column, so both stages form it once: as the upper face of a
+// plane, carrying the lower face's terms of the next plane in registers (zface1, zface2).
 struct LimArgs {
     int nx, ny, nz;
     long long np;
@@ -545,18 +603,20 @@ __device__ __forceinline__ void lim_part(const LimArgs& a, int item, int lane,
     LimCol P, H;
     init(P, colb, true);
     if (sec) init(H, scolb, false);
-    // c1 of plane z0 - 2: stage 1 of the first iteration reads it from LDS (the loop
-    // publishes planes from z0 - 1 on)
-    if (comp) sm.c[(z0 + 1) % 3][L] = P.c1[0];
-    if (sec) sm.c[(z0 + 1) % 3][S] = H.c1[0];
 
     // tile cell: stage-1 results of planes p-2, p-1 (index 0, 1; 2 = plane p), the codes,
-    // the carried donor sum and lam_Al * m of plane p-1, c1 and A2 of plane p-2
-    double siu0 = 0, sid0 = 0, sli0 = 0, slx0 = 0, dacc_m = 0.0, lam_m[3] = {0, 0, 0};
-    double a2_m2 = 0.0;
+    // the carried donor sum and lam_Al * m of plane p-1
+    double siu0 = 0, sid0 = 0, dacc_m = 0.0, lam_m[3] = {0, 0, 0};
+    // stage 2: the term gr*lf of the z face below the cell of plane k (its upper z face
+    // at plane k - 1)
+    double z2 = 0.0;
     int scd0 = 0;
     // the emission adjoint of plane p, loaded one iteration ahead
     double lq_p = 0.0;
+    // stage 1: the z face below the own cell of plane p, formed as the upper z face of
+    // plane p - 1 (first: the face between planes z0 - 2 and z0 - 1, from the loads)
+    ZFace zc = zface1(P.c1[0], P.c1[1], P.a2m, P.lo[0], P.lo[1], P.bu[0], P.bd[0], P.bu[1],
+                      P.bd[1]);
 
     for (int pl = z0 - 1; pl <= z1; pl++) {
         const int p = __builtin_amdgcn_readfirstlane(pl);
@@ -583,22 +643,20 @@ __device__ __forceinline__ void lim_part(const LimArgs& a, int item, int lane,
         tile::lds_barrier();
         // stage 1 on cell M of plane p (the merged k_adj_donor_limit, same operation order)
         auto stage1 = [&](int M, const LimCol& q, double& dacc, double lam[3], double& iu,
-                          double& id, double& li, double& lx) {
+                          double& id, double& li, double& lx, const ZFace& zc, ZFace& zn) {
             const double* C = sm.c[s3];
             const double lt = q.lo[1], ct = q.c1[1], bu_t = q.bu[1], bd_t = q.bd[1];
             double acc = lt, su = 0.0, sd = 0.0;
             double fl[3], fu[3];  // the unlimited donor fluxes of the lower / upper faces
 #pragma unroll
-            for (int d = 0; d < 3; d++) {
+            for (int d = 0; d < 2; d++) {
                 double gl, gr, gv;
                 {  // upper face (M, r)
                     double a_, lo_r, c_r, bu_r, bd_r;
                     if (d == 0) {
                         a_ = q.a0; lo_r = sm.lo[M + 1]; c_r = C[M + 1]; bu_r = sm.bu[M + 1]; bd_r = sm.bd[M + 1];
-                    } else if (d == 1) {
-                        a_ = q.a1; lo_r = sm.lo[M + RX]; c_r = C[M + RX]; bu_r = sm.bu[M + RX]; bd_r = sm.bd[M + RX];
                     } else {
-                        a_ = q.a2c; lo_r = q.lo[2]; c_r = q.c1[2]; bu_r = q.bu[2]; bd_r = q.bd[2];
+                        a_ = q.a1; lo_r = sm.lo[M + RX]; c_r = C[M + RX]; bu_r = sm.bu[M + RX]; bd_r = sm.bd[M + RX];
                     }
                     // limiter multiplier without branches: the ratio pair of the
                     // flow direction, the selection of mn2(1, mn2(x, y))
@@ -621,10 +679,8 @@ __device__ __forceinline__ void lim_part(const LimArgs& a, int item, int lane,
                     double a_, lo_l, c_l, bu_l, bd_l;
                     if (d == 0) {
                         a_ = sm.a0[s3][M - 1]; lo_l = sm.lo[M - 1]; c_l = C[M - 1]; bu_l = sm.bu[M - 1]; bd_l = sm.bd[M - 1];
-                    } else if (d == 1) {
-                        a_ = sm.a1[s3][M - RX]; lo_l = sm.lo[M - RX]; c_l = C[M - RX]; bu_l = sm.bu[M - RX]; bd_l = sm.bd[M - RX];
                     } else {
-                        a_ = q.a2m; lo_l = q.lo[0]; c_l = sm.c[k3][M]; bu_l = q.bu[0]; bd_l = q.bd[0];
+                        a_ = sm.a1[s3][M - RX]; lo_l = sm.lo[M - RX]; c_l = C[M - RX]; bu_l = sm.bu[M - RX]; bd_l = sm.bd[M - RX];
                     }
                     fl[d] = 0.5 * F2(c_l, ct, a_);
                     const bool pf = a_ > 0, nf = a_ < 0;
@@ -640,6 +696,20 @@ __device__ __forceinline__ void lim_part(const LimArgs& a, int item, int lane,
                     su += pf ? sel : 0.0;
                     sd += nf ? sel : 0.0;
                 }
+            }
+            {  // the upper z face, both sides at once (the upper cell's share is carried
+               // to the next plane), then the lower z face: the upper z face of the
+               // plane below
+                zn = zface1(ct, q.c1[2], q.a2c, lt, q.lo[2], bu_t, bd_t, q.bu[2], q.bd[2]);
+                fu[2] = zn.f;
+                acc += zn.glf;
+                sd += zn.sdu;
+                su += zn.suu;
+                lam[2] = zn.lam;
+                fl[2] = zc.f;
+                acc += zc.grlf;
+                su += zc.sul;
+                sd += zc.sdl;
             }
             dacc = acc;
             // the flux sums of the forward, formed again from the same six fluxes
@@ -658,7 +728,9 @@ __device__ __forceinline__ void lim_part(const LimArgs& a, int item, int lane,
         double dacc = 0.0, lam[3] = {0.0, 0.0, 0.0};
         if (comp) {
             double iu, id, li, lx;
-            stage1(L, P, dacc, lam, iu, id, li, lx);
+            ZFace zn;
+            stage1(L, P, dacc, lam, iu, id, li, lx, zc, zn);
+            zc = zn;
             // the codes of plane p go to LDS here, after the first barrier, like the
             // stage-1 results: their slot held plane p - 2, which stage 2 of the previous
             // iteration may still be reading until every wave has reached that barrier
@@ -714,18 +786,16 @@ __device__ __forceinline__ void lim_part(const LimArgs& a, int item, int lane,
             }
             // flux sums of the six faces
             const double li_t = LI[L], lx_t = LX[L];
-            double acc = 0.0, lAf[3];
+            double acc = 0.0, lAf[3], z2n;
 #pragma unroll
-            for (int d = 0; d < 3; d++) {
+            for (int d = 0; d < 2; d++) {
                 double gl, gr, gv;
                 {  // upper face (t, r)
                     double a_, c_r, li_r, lx_r;
                     if (d == 0) {
                         a_ = sm.a0[k3][L]; c_r = C[L + 1]; li_r = LI[L + 1]; lx_r = LX[L + 1];
-                    } else if (d == 1) {
-                        a_ = sm.a1[k3][L]; c_r = C[L + RX]; li_r = LI[L + RX]; lx_r = LX[L + RX];
                     } else {
-                        a_ = P.a2m; c_r = sm.c[s3][L]; li_r = sm.li[s2][L]; lx_r = sm.lx[s2][L];
+                        a_ = sm.a1[k3][L]; c_r = C[L + RX]; li_r = LI[L + RX]; lx_r = LX[L + RX];
                     }
                     // (lx_t [f>0] - li_t [f<0]) + (li_r [f>0] - lx_r [f<0]), the zero
                     // terms dropped (the same value up to the sign of a zero)
@@ -739,16 +809,23 @@ __device__ __forceinline__ void lim_part(const LimArgs& a, int item, int lane,
                     double a_, c_l, li_l, lx_l;
                     if (d == 0) {
                         a_ = sm.a0[k3][L - 1]; c_l = C[L - 1]; li_l = LI[L - 1]; lx_l = LX[L - 1];
-                    } else if (d == 1) {
-                        a_ = sm.a1[k3][L - RX]; c_l = C[L - RX]; li_l = LI[L - RX]; lx_l = LX[L - RX];
                     } else {
-                        a_ = a2_m2; c_l = sm.c[(p + 1) % 3][L]; li_l = sli0; lx_l = slx0;
+                        a_ = sm.a1[k3][L - RX]; c_l = C[L - RX]; li_l = LI[L - RX]; lx_l = LX[L - RX];
                     }
                     const double f = 0.5 * F2(c_l, ct, a_);
                     const double lf = f > 0 ? lx_l + li_t : (f < 0 ? -(li_l + lx_t) : 0.0);
                     fpart(c_l, ct, a_, gl, gr, gv);
                     acc += gr * lf;
                 }
+            }
+            {  // the upper z face (its gr*lf carried to the next plane), then the lower
+               // z face: the upper z face of plane k - 1
+                double glf, gvlf;
+                zface2(ct, sm.c[s3][L], P.a2m, li_t, lx_t, sm.li[s2][L], sm.lx[s2][L], glf,
+                       gvlf, z2n);
+                acc += glf;
+                lAf[2] = lam_m[2] + gvlf;
+                acc += z2;
             }
             if (store) {
                 st_nt(a.lin + zm1.off, colb, s0);
@@ -757,20 +834,23 @@ __device__ __forceinline__ void lim_part(const LimArgs& a, int item, int lane,
                 st_nt(a.lA1 + zm1.off, colb, lAf[1]);
                 st_nt(a.lA2 + zm1.off, colb, lAf[2]);
             }
+            z2 = z2n;
+        } else if (tcell && k == z0 - 1) {
+            // the z face between planes z0 - 1 and z0: the lower face of the first plane
+            double glf, gvlf;
+            zface2(sm.c[k3][L], sm.c[s3][L], P.a2m, sm.li[k2][L], sm.lx[k2][L], sm.li[s2][L],
+                   sm.lx[s2][L], glf, gvlf, z2);
         }
         // shift the tile cell's history by one plane: plane p - 1 becomes p - 2 (its LDS
         // slot is overwritten by the next iteration's stage 1)
         if constexpr (TILE) {
             siu0 = sm.iu[k2][L];
             sid0 = sm.id[k2][L];
-            sli0 = sm.li[k2][L];
-            slx0 = sm.lx[k2][L];
             scd0 = sm.cd[k2][L];
             dacc_m = dacc;
             lam_m[0] = lam[0];
             lam_m[1] = lam[1];
             lam_m[2] = lam[2];
-            a2_m2 = P.a2m;
         }
         rotate(P);
         if (sec) rotate(H);
