@@ -66,6 +66,21 @@ __device__ __forceinline__ double donor(double c, double cxl, double cxr, double
     return c - 0.5 * ((fx + fy) + fz);
 }
 
+// donor() with the lower z face's doubled flux F2(czl, c, wzl) given: a streaming kernel
+// carries it from the plane below, where it was that plane's upper z face, F2(c, czr,
+// wzr), which this returns in fzu2 (the same operations, so the same values).
+__device__ __forceinline__ double donor_zc(double c, double cxl, double cxr, double cyl,
+                                           double cyr, double czr, double uxl, double uxr,
+                                           double vyl, double vyr, double wzr, double fzl2,
+                                           double& fzu2)
+{
+    double fx = F2(c, cxr, uxr) - F2(cxl, c, uxl);
+    double fy = F2(c, cyr, vyr) - F2(cyl, c, vyl);
+    fzu2 = F2(c, czr, wzr);
+    double fz = fzu2 - fzl2;
+    return c - 0.5 * ((fx + fy) + fz);
+}
+
 // uu.c:18-32 (vv.c:21-35, ww.c:180-194) -- antidiffusive pseudo-velocity of one face:
 // a = previous-pass Courant number of the face, p/m = the cell values above/below the
 // face, s1/s2 = averaged transverse velocities, n/d = transverse gradient ratios.

@@ -156,11 +156,15 @@ struct Col3 {
     double u_m1 = 0, u_0 = 0, nu = 0, v_m1 = 0, v_0 = 0, nv = 0;          // U, V(p-1 .. p+1)
     double fld = 0, nf = 0;                                               // pass input (p, p+1)
     double wr_m = 0, wu_m = 0;        // W of the x / y neighbour at plane p-2
-    double c1_m4 = 0, c1_m3 = 0, c1_m2 = 0, c1_m1 = 0, c1_0 = 0;         // c1(p-4 .. p)
+    double c1_m3 = 0, c1_m2 = 0, c1_m1 = 0, c1_0 = 0;                    // c1(p-3 .. p)
     double ax_m3 = 0, ay_m3 = 0, ax_m2 = 0, ay_m2 = 0;                   // own A_x, A_y
-    double az_m4 = 0, az_m3 = 0, az_m2 = 0;                              // own A_z
+    double az_m3 = 0, az_m2 = 0;                                         // own A_z
     double axl_m3 = 0, ayd_m3 = 0;    // A_x of the x-, A_y of the y- neighbour, plane p-3
-    double bu_m4 = 0, bd_m4 = 0, bu_m3 = 0, bd_m3 = 0;                   // own ratios
+    double bu_m3 = 0, bd_m3 = 0;                                         // own ratios
+    // z faces formed once per face, as the upper face of a plane, and carried to the
+    // plane above as its lower face: the donor field's doubled flux (F2), the ratios'
+    // unlimited flux (0.5 F2) and the limited donor pass's doubled flux
+    double fc_z = 0, fr_z = 0, fl_z = 0;
 };
 
 // One role's share of a work item.  TILE: lane = tile cell (all stages, the stores).
@@ -230,6 +234,8 @@ __device__ __forceinline__ void f12_part(const FusedArgs& a, int item, int lane,
             S.nv = ld(a.V + zc.off, S.colb);
         }
     }
+    // the donor field's lower z face of the first plane (pstart), from the loads
+    if constexpr (DONOR) T.fc_z = F2(T.cn_m1, T.cn_0, T.w_m1);
     double nq = 0;  // emission of plane p-2, used for the output plane of the next iteration
     // (an instance of a batch without emission reads a plane of its field instead: an
     // unconditional load, whatever the item)
@@ -298,10 +304,15 @@ __device__ __forceinline__ void f12_part(const FusedArgs& a, int item, int lane,
                 const double* cn = sm.cn[s4];
                 const double* up = sm.u[s3];
                 const double* vp = sm.v[s3];
-                T.c1_0 = DONOR ? donor(T.cn_0, cn[L - 1], cn[L + 1], cn[L - RX], cn[L + RX],
-                                       T.cn_m1, T.cn_p1, up[L - 1], T.u_0, vp[L - RX], T.v_0,
-                                       T.w_m1, T.w_0)
-                               : T.fld;
+                if constexpr (DONOR) {
+                    double fcu;
+                    T.c1_0 = donor_zc(T.cn_0, cn[L - 1], cn[L + 1], cn[L - RX], cn[L + RX],
+                                      T.cn_p1, up[L - 1], T.u_0, vp[L - RX], T.v_0, T.w_0,
+                                      T.fc_z, fcu);
+                    T.fc_z = fcu;
+                } else {
+                    T.c1_0 = T.fld;
+                }
                 sm.c1[s4][L] = T.c1_0;
                 if constexpr (SAVE && TILE)
                     if (store && p >= z0 && p < z1) put<NT>(a.c1out + zp0.off, T.colb, T.c1_0);
@@ -412,7 +423,7 @@ __device__ __forceinline__ void f12_part(const FusedArgs& a, int item, int lane,
         const double* axm = sm.ax[ma];
         const double* aym = sm.ay[ma];
         auto ratios = [&](int M, const Col3& C, double& bu, double& bd, double& axl,
-                          double& ayd) __attribute__((always_inline)) {
+                          double& ayd, double& fzr_) __attribute__((always_inline)) {
             const double c = C.c1_m2;
             const double xl = c1m[M - 1], xr = c1m[M + 1], yd = c1m[M - RX], yu = c1m[M + RX];
             const double hi =
@@ -429,8 +440,9 @@ __device__ __forceinline__ void f12_part(const FusedArgs& a, int item, int lane,
             ayd = aym[M - RX];
             const double fxl = 0.5 * F2(xl, c, axl), fxr = 0.5 * F2(c, xr, C.ax_m2);
             const double fyl = 0.5 * F2(yd, c, ayd), fyr = 0.5 * F2(c, yu, C.ay_m2);
-            const double fzl = 0.5 * F2(C.c1_m3, c, C.az_m3), fzr = 0.5 * F2(c, C.c1_m1, C.az_m2);
+            const double fzl = C.fr_z, fzr = 0.5 * F2(c, C.c1_m1, C.az_m2);
             beta(hi, lo, c, fxl, fxr, fyl, fyr, fzl, fzr, bu, bd);
+            fzr_ = fzr;
             if constexpr (SAVE && TILE) {
                 if (store && m >= z0 && m < z1) {
                     // which element holds each bound: element 2q + s is the c^n (s = 0) or
@@ -473,9 +485,14 @@ __device__ __forceinline__ void f12_part(const FusedArgs& a, int item, int lane,
                 }
         }
         if (doB && r1) {
-            ratios(L, T, bu_m, bd_m, axl_m, ayd_m);
+            double fzr;
+            ratios(L, T, bu_m, bd_m, axl_m, ayd_m, fzr);
+            T.fr_z = fzr;
             sm.bu[ma][L] = bu_m;
             sm.bd[ma][L] = bd_m;
+        } else if (!ST && r1 && m == z0 - 2) {
+            // the upper z face of plane z0 - 2: the lower face of the first ratio plane
+            T.fr_z = 0.5 * F2(T.c1_m2, T.c1_m1, T.az_m2);
         }
         // limited donor pass of plane o on the tile
         if constexpr (TILE) {
@@ -490,10 +507,11 @@ __device__ __forceinline__ void f12_part(const FusedArgs& a, int item, int lane,
                 const double vyr = limit_face_clamped(T.ay_m3, b0u, b0d, bu[L + RX], bd[L + RX]);
                 const double vyl = limit_face_clamped(T.ayd_m3, bu[L - RX], bd[L - RX], b0u, b0d);
                 const double wzr = limit_face_clamped(T.az_m3, b0u, b0d, bu_m, bd_m);
-                const double wzl = limit_face_clamped(T.az_m4, T.bu_m4, T.bd_m4, b0u, b0d);
-                double r = donor(T.c1_m3, co[L - 1], co[L + 1], co[L - RX], co[L + RX], T.c1_m4,
-                                 T.c1_m2, uxl, uxr, vyl, vyr, wzl, wzr);
+                double fzu;
+                double r = donor_zc(T.c1_m3, co[L - 1], co[L + 1], co[L - RX], co[L + RX],
+                                    T.c1_m2, uxl, uxr, vyl, vyr, wzr, T.fl_z, fzu);
                 keep_v(r);
+                T.fl_z = fzu;
                 if (store) {
                     if (EMIT && a.q) r = r + qo;
                     put<NT>(a.cout + zm3.off, T.colb, r);
@@ -503,6 +521,11 @@ __device__ __forceinline__ void f12_part(const FusedArgs& a, int item, int lane,
                         put<NT>(a.Lz + zm3.off, T.colb, wzr);
                     }
                 }
+            } else if (!ST && o == z0 - 1) {
+                // the upper z face of plane z0 - 1: the lower face of the first output
+                // plane (its limited velocity and doubled flux)
+                const double wz = limit_face_clamped(T.az_m3, T.bu_m3, T.bd_m3, bu_m, bd_m);
+                T.fl_z = F2(T.c1_m3, T.c1_m2, wz);
             }
         }
         // rotate the per-column registers
@@ -516,7 +539,6 @@ __device__ __forceinline__ void f12_part(const FusedArgs& a, int item, int lane,
         T.wu_m = wku;
         T.u_m1 = T.u_0;
         T.v_m1 = T.v_0;
-        T.c1_m4 = T.c1_m3;
         T.c1_m3 = T.c1_m2;
         T.c1_m2 = T.c1_m1;
         T.c1_m1 = T.c1_0;
@@ -524,14 +546,11 @@ __device__ __forceinline__ void f12_part(const FusedArgs& a, int item, int lane,
         T.ay_m3 = T.ay_m2;
         T.ax_m2 = ax_k;
         T.ay_m2 = ay_k;
-        T.az_m4 = T.az_m3;
         T.az_m3 = T.az_m2;
         T.az_m2 = az_k;
         if constexpr (TILE) {
             T.axl_m3 = axl_m;
             T.ayd_m3 = ayd_m;
-            T.bu_m4 = T.bu_m3;
-            T.bd_m4 = T.bd_m3;
             T.bu_m3 = bu_m;
             T.bd_m3 = bd_m;
         }
