@@ -61,6 +61,8 @@ def tuned_config(g):
     if g.path() == 3:
         return {"kernel": "one per corrective pass", "tile": [32, 16], "zc": g.pass_zchunk(),
                 "waves": "8 tile + 4 ring"}
+    if g.path() == 4:
+        return {"kernel": "one per corrective pass, single-plane form", "tile": [16, 16]}
     (tx, ty, zc), (lx, ly, lz) = g.tiles()
     out = {"tile": [tx, ty], "zc": zc}
     if lx:

@@ -100,9 +100,10 @@ int mpd_sentinel_check(mpd_grid g, long long* first_bad_step);
  * kernel per stage, intermediates in HBM), 2 = fused (two kernels per corrective pass:
  * antidiffusive velocities, then limiter and limited donor pass), 3 = one kernel per
  * corrective pass (nonoscillatory grids; the donor field and the antidiffusive
- * velocities stay on chip); 0 picks 3 for nonoscillatory grids of at least 8 z-planes
- * and 2 otherwise; grids with more than 2^29 cells per z-plane always take the staged
- * path.  FRAME grids: 1 = the per-face
+ * velocities stay on chip), 4 = its single-plane form (nonoscillatory grids of one
+ * z-plane; other grids take 2); 0 picks 4 for nonoscillatory grids of one z-plane, 3 for
+ * those of at least 8 z-planes and 2 otherwise; grids with more than 2^29 cells per
+ * z-plane always take the staged path.  FRAME grids: 1 = the per-face
  * antidiffusive kernel and a separate donor pass, otherwise the tiled kernel that forms
  * the donor field and the velocities together (results are identical). */
 int mpd_set_path(mpd_grid g, int path);
@@ -134,7 +135,8 @@ int mpd_set_pass_zchunk(mpd_grid g, int zc);
 int mpd_get_pass_zchunk(mpd_grid g, int* zc);
 /* The kernel path the grid's steps take: 1 staged (or donor only), 2 two kernels per
  * corrective pass, 3 one kernel per corrective pass (automatic for nonoscillatory
- * PERIODIC grids of at least 8 z-planes).  FRAME grids: 1 per-face kernels, 2 tiled. */
+ * PERIODIC grids of at least 8 z-planes), 4 its single-plane form (automatic for those
+ * of one z-plane).  FRAME grids: 1 per-face kernels, 2 tiled. */
 int mpd_get_path(mpd_grid g, int* path);
 /* Times every tile shape x z-chunk length on the grid itself (one warm-up step and nsteps
  * timed steps each, each kernel between events of its own), then times whole steps of the

@@ -320,7 +320,8 @@ class Grid:
         return v.value
 
     def path(self):
-        """The kernel path the steps take (1 staged, 2 two kernels per pass, 3 one)."""
+        """The kernel path the steps take (1 staged, 2 two kernels per pass, 3 one, 4 one
+        in its single-plane form)."""
         v = C.c_int(0)
         _check(lib().mpd_get_path(self.h, C.byref(v)))
         return v.value

@@ -625,7 +625,7 @@ extern "C" int mpd_field_stats(mpd_grid g, double* mass, double* cmin, double* c
 extern "C" int mpd_set_path(mpd_grid g, int path)
 {
     GRID_OR_FAIL(g);
-    if (path < 0 || path > 3) return fail(MPD_ERR_ARG, "mpd_set_path: %d not in 0..3", path);
+    if (path < 0 || path > 4) return fail(MPD_ERR_ARG, "mpd_set_path: %d not in 0..4", path);
     g->path = path;
     return MPD_OK;
 }
@@ -702,7 +702,7 @@ extern "C" int mpd_get_path(mpd_grid g, int* path)
     else if (g->path == 1 || g->iord < 2 || !fused_plane_fits(g->nx, g->ny))
         *path = 1;
     else
-        *path = one_kernel_passes(g) ? 3 : 2;
+        *path = plane_kernel_passes(g) ? 4 : one_kernel_passes(g) ? 3 : 2;
     return MPD_OK;
 }
 
@@ -723,7 +723,7 @@ extern "C" int mpd_autotune(mpd_grid g, int nsteps, int* tile_x, int* tile_y, in
         return fail(MPD_ERR_UNSUPPORTED, "mpd_autotune: tunes the fused PERIODIC path only");
     if (!g->has_winds || !g->has_field)
         return fail(MPD_ERR_ARG, "mpd_autotune: set the winds (or Courant numbers) and the field first");
-    if (one_kernel_passes(g)) {
+    if (one_kernel_passes(g) || plane_kernel_passes(g)) {
         // the one-kernel pass has a single tile shape and a z-chunk chosen from the size:
         // nothing to time (its trial steps would misread as the two-kernel pair)
         if (tile_x) *tile_x = g->tile_x;

@@ -837,11 +837,12 @@ int MPD_CAT(periodic_fused_step, MPD_ARITH)(mpd_grid_s* g)
     double* out = g->c[1 - g->cur].p;
     const double* prev[3] = {g->face[0].p, g->face[1].p, g->face[2].p};
     const double* field = cin;
-    if (one_kernel_passes(g)) {
-        // one kernel per corrective pass; the limited faces of pass p (read by pass p+1)
-        // alternate between the two face sets
-        const int zc = pass_zchunk(g);
-        a.zc = zc;
+    const bool plane = plane_kernel_passes(g);
+    if (plane || one_kernel_passes(g)) {
+        // one kernel per corrective pass (on a grid of one plane its single-plane form);
+        // the limited faces of pass p (read by pass p+1) alternate between the two face
+        // sets
+        a.zc = plane ? 1 : pass_zchunk(g);
         fused_geometry(a, F12_TILE_X, F12_TILE_Y);
         const int nb = fused_items(a);
         const Work w{nullptr, nullptr, 0, nb, nb % 8 == 0 ? 1 : 0};
@@ -861,9 +862,10 @@ int MPD_CAT(periodic_fused_step, MPD_ARITH)(mpd_grid_s* g)
             hipEvent_t ev[2];
             const hipEvent_t* use;
             if ((rc = span(g, ev, &use))) return rc;
-            if ((rc = MPD_CAT(onepass_launch, MPD_ARITH)(a, w, nb, p == 1, a.q != nullptr, s,
-                                                          use)))
-                return rc;
+            rc = plane ? MPD_CAT(plane_launch, MPD_ARITH)(a, p == 1, a.q != nullptr, s, use)
+                       : MPD_CAT(onepass_launch, MPD_ARITH)(a, w, nb, p == 1, a.q != nullptr,
+                                                            s, use);
+            if (rc) return rc;
             for (int d = 0; d < 3; d++) prev[d] = lf[d].p;
             field = a.cout;
         }

@@ -106,6 +106,12 @@ inline bool one_kernel_passes(const mpd_grid_s* g)
 {
     return g->nonosc && (g->path == 3 || (g->path == 0 && g->nz >= 8));
 }
+// Whether a grid's corrective passes take the single-plane form of that kernel (path 4,
+// automatic for nonoscillatory grids of one z-plane).
+inline bool plane_kernel_passes(const mpd_grid_s* g)
+{
+    return g->nonosc && g->nz == 1 && (g->path == 4 || g->path == 0);
+}
 // The stream a grid's fused step launches on.
 inline hipStream_t launch_stream(const mpd_grid_s* g)
 {

@@ -674,6 +674,203 @@ int choose_zc_f12(const mpd_grid_s* g)
     return best;
 }
 
+// ---------------------------------------------------------------------------------------
+// Grids of one z-plane (nz = 1): every z-neighbour of a cell is the cell itself (the
+// periodic wrap of one plane), and every "plane below / above" of the one-kernel pass is
+// the same plane.  F12's plane pipeline, which streams five planes beyond a chunk to
+// keep c1, A and the ratios of three planes in flight, collapses into four
+// barrier-separated stages over the one plane: load c^n, U, V, W (and the pass input) of
+// the tile and its three-cell ring; c1 on rings 0-2; the faces; the ratios on rings 0-1;
+// the limited donor pass on the tile.  The formulas are F12's, with the z-neighbours of a
+// cell read at the cell (so the exact build is the oracle's arithmetic bit for bit).
+// One workgroup per tile; the threads loop over the cells of each stage.
+template <int TX, int TY>
+struct Plane2 {
+    using G = Geo3<TX, TY>;
+    static constexpr int RX = G::RX, RY = G::RY, R = G::R;
+    double cn[R], u[R], v[R], w[R], c1[R], ax[R], ay[R], az[R], bu[R], bd[R];
+};
+
+constexpr int P2_TX = 16, P2_TY = 16, P2_NTH = 256;
+
+template <int TX, int TY, bool DONOR, bool EMIT>
+__global__ void __launch_bounds__(P2_NTH) k_f12_plane(FusedArgs a)
+{
+    using P = Plane2<TX, TY>;
+    constexpr int RX = P::RX, RY = P::RY, R = P::R;
+    __shared__ P sm;
+    const int bx = blockIdx.x % a.tiles_x, by = blockIdx.x / a.tiles_x;
+    auto col = [&](int rx, int ry) {  // element index of region cell (rx, ry)
+        const int gx = wrapi(bx * TX + rx - 3, a.nx), gy = wrapi(by * TY + ry - 3, a.ny);
+        return (size_t)gy * a.nx + gx;
+    };
+    // stage 0: the region's inputs
+    for (int L = threadIdx.x; L < R; L += P2_NTH) {
+        const size_t o = col(L % RX, L / RX);
+        sm.cn[L] = a.cn[o];
+        sm.u[L] = a.U[o];
+        sm.v[L] = a.V[o];
+        sm.w[L] = a.W[o];
+        if constexpr (!DONOR) sm.c1[L] = a.cin[o];
+    }
+    __syncthreads();
+    // stage 1: the donor field on rings 0-2 (first pass; later passes read their input)
+    if constexpr (DONOR) {
+        constexpr int W1 = RX - 2, N1 = W1 * (RY - 2);
+        for (int i = threadIdx.x; i < N1; i += P2_NTH) {
+            const int L = (1 + i / W1) * RX + 1 + i % W1;
+            const double c = sm.cn[L], w = sm.w[L];
+            sm.c1[L] = donor(c, sm.cn[L - 1], sm.cn[L + 1], sm.cn[L - RX], sm.cn[L + RX], c, c,
+                             sm.u[L - 1], sm.u[L], sm.v[L - RX], sm.v[L], w, w);
+        }
+        __syncthreads();
+    }
+    // stage 2: antidiffusive velocities (the operand order of F12's faces): x-faces on
+    // rows 2..TY+3 from column 1 (the ring-2 cells left of ring 1) to TX+3, y-faces on
+    // columns 2..TX+3 from row 1, z-faces on rings 0-1
+    {
+        constexpr int W2 = TX + 3, N2 = W2 * (TY + 3);  // columns 1..TX+3, rows 1..TY+3
+        const double* cc = sm.c1;
+        const double* uk = sm.u;
+        const double* vk = sm.v;
+        for (int i = threadIdx.x; i < N2; i += P2_NTH) {
+            const int rx = 1 + i % W2, ry = 1 + i / W2, M = ry * RX + rx;
+            const double c = cc[M];
+            if (ry >= 2) {  // x-face (uu.c:81-87)
+                const double av = uk[M];
+                const double vr = vk[M + 1 - RX], vru = vk[M + 1], vu = vk[M], vcc = vk[M - RX];
+                const double wr_ = sm.w[M + 1], wrf = wr_, wf = sm.w[M], wc = wf;
+                const double cr = cc[M + 1], cru = cc[M + 1 + RX], cu = cc[M + RX];
+                const double cd = cc[M - RX], crd = cc[M + 1 - RX];
+                const double crf = cr, cf = c, cb = c, crb = cr;
+#ifdef MPD_FAST_DIV
+                const double v1 = vr + vcc, v2 = vru + vu, w1 = wr_ + wc, w2 = wrf + wf;
+                const double b1 = cru + cu, b2 = cd + crd, e1 = crf + cf, e2 = cb + crb;
+                const double sv = 0.25 * (v1 + v2);
+                const double sw = 0.25 * (w1 + w2);
+                const double div = 0.25 * av * ((uk[M + 1] - uk[M - 1]) + (v1 - v2) + (w1 - w2));
+                sm.ax[M] = antidiff(div, av, cr, c, sv, b1 - b2, b1 + b2, sw, e1 - e2, e1 + e2);
+#else
+                const double sv = 0.25 * (vr + vru + vu + vcc);
+                const double sw = 0.25 * (wr_ + wrf + wf + wc);
+                const double div = 0.25 * av * (uk[M + 1] - uk[M - 1] + vr - vru - vu + vcc +
+                                                wr_ - wrf - wf + wc);
+                sm.ax[M] = antidiff(div, av, cr, c, sv, cru + cu - cd - crd, cru + cu + cd + crd,
+                                    sw, crf + cf - cb - crb, crf + cf + cb + crb);
+#endif
+            }
+            if (rx >= 2) {  // y-face (vv.c:85-92)
+                const double av = vk[M];
+                const double uu = uk[M - 1 + RX], uru = uk[M + RX], ur = uk[M], ucc = uk[M - 1];
+                const double ulu = uk[M - 2 + RX], ul = uk[M - 2];
+                const double wu_ = sm.w[M + RX], wuf = wu_, wf = sm.w[M], wc = wf;
+                const double cu = cc[M + RX], cru = cc[M + 1 + RX], cr = cc[M + 1];
+                const double cl = cc[M - 1], clu = cc[M - 1 + RX];
+                const double cuf = cu, cf = c, cb = c, cub = cu;
+#ifdef MPD_FAST_DIV
+                const double u1 = uu + ucc, w1 = wu_ + wc, w2 = wuf + wf;
+                const double b1 = cru + cr, b2 = cl + clu, e1 = cuf + cf, e2 = cb + cub;
+                const double su = 0.25 * (u1 + (uru + ur));
+                const double sw = 0.25 * (w1 + w2);
+                const double div =
+                    0.25 * av * ((vk[M + RX] - vk[M - RX]) + (w1 - w2) + (u1 - (ulu + ul)));
+                sm.ay[M] = antidiff(div, av, cu, c, su, b1 - b2, b1 + b2, sw, e1 - e2, e1 + e2);
+#else
+                const double su = 0.25 * (uu + uru + ur + ucc);
+                const double sw = 0.25 * (wu_ + wuf + wf + wc);
+                const double div = 0.25 * av * (vk[M + RX] - vk[M - RX] + wu_ - wuf - wf + wc +
+                                                uu - ulu - ul + ucc);
+                sm.ay[M] = antidiff(div, av, cu, c, su, cru + cr - cl - clu, cru + cr + cl + clu,
+                                    sw, cuf + cf - cb - cub, cuf + cf + cb + cub);
+#endif
+            }
+            if (rx >= 2 && ry >= 2) {  // z-face (ww.c:248-255)
+                const double av = sm.w[M];
+                const double ur = uk[M], urf = ur, uf = uk[M - 1], ucc = uf;
+                const double vf = vk[M - RX], vuf = vk[M], vu = vuf, vcc = vf;
+                const double cf = c, crf = cc[M + 1], cr = crf, cl = cc[M - 1], clf = cl;
+                const double cuf = cc[M + RX], cu = cuf, cd = cc[M - RX], cdf = cd;
+#ifdef MPD_FAST_DIV
+                const double u1 = uf + ucc, u2 = urf + ur, v1 = vf + vcc, v2 = vuf + vu;
+                const double b1 = crf + cr, b2 = cl + clf, e1 = cuf + cu, e2 = cd + cdf;
+                const double su = 0.25 * (u1 + u2);
+                const double sv = 0.25 * (v1 + v2);
+                const double div = 0.25 * av * ((av - av) + (u1 - u2) + (v1 - v2));
+                sm.az[M] = antidiff(div, av, cf, c, su, b1 - b2, b1 + b2, sv, e1 - e2, e1 + e2);
+#else
+                const double su = 0.25 * (ur + urf + uf + ucc);
+                const double sv = 0.25 * (vf + vuf + vu + vcc);
+                const double div = 0.25 * av * (av - av + uf - urf - ur + ucc + vf - vuf - vu +
+                                                vcc);
+                sm.az[M] = antidiff(div, av, cf, c, su, crf + cr - cl - clf, crf + cr + cl + clf,
+                                    sv, cuf + cu - cd - cdf, cuf + cu + cd + cdf);
+#endif
+            }
+        }
+    }
+    __syncthreads();
+    // stage 3: limiter ratios on rings 0-1, clamped to at most 1 (F12's ratios)
+    {
+        constexpr int W3 = TX + 2, N3 = W3 * (TY + 2);
+        for (int i = threadIdx.x; i < N3; i += P2_NTH) {
+            const int M = (2 + i / W3) * RX + 2 + i % W3;
+            const double c = sm.c1[M], cn = sm.cn[M];
+            const double xl = sm.c1[M - 1], xr = sm.c1[M + 1], yd = sm.c1[M - RX];
+            const double yu = sm.c1[M + RX];
+            const double hi =
+                mx2(mx2(mx2(mx2(cn, c), mx2(sm.cn[M - 1], xl)),
+                        mx2(mx2(sm.cn[M + 1], xr), mx2(sm.cn[M - RX], yd))),
+                    mx2(mx2(mx2(sm.cn[M + RX], yu), mx2(cn, c)), mx2(cn, c)));
+            const double lo =
+                mn2(mn2(mn2(mn2(cn, c), mn2(sm.cn[M - 1], xl)),
+                        mn2(mn2(sm.cn[M + 1], xr), mn2(sm.cn[M - RX], yd))),
+                    mn2(mn2(mn2(sm.cn[M + RX], yu), mn2(cn, c)), mn2(cn, c)));
+            const double az = sm.az[M];
+            const double fxl = 0.5 * F2(xl, c, sm.ax[M - 1]), fxr = 0.5 * F2(c, xr, sm.ax[M]);
+            const double fyl = 0.5 * F2(yd, c, sm.ay[M - RX]), fyr = 0.5 * F2(c, yu, sm.ay[M]);
+            const double fzl = 0.5 * F2(c, c, az), fzr = fzl;
+            double bu, bd;
+            beta(hi, lo, c, fxl, fxr, fyl, fyr, fzl, fzr, bu, bd);
+            sm.bu[M] = mn2(bu, 1.0);
+            sm.bd[M] = mn2(bd, 1.0);
+        }
+    }
+    __syncthreads();
+    // stage 4: the limited donor pass on the tile (+ emission, the limited faces)
+    for (int i = threadIdx.x; i < TX * TY; i += P2_NTH) {
+        const int rx = 3 + i % TX, ry = 3 + i / TX, M = ry * RX + rx;
+        if (bx * TX + rx - 3 >= a.nx || by * TY + ry - 3 >= a.ny) continue;
+        const double b0u = sm.bu[M], b0d = sm.bd[M];
+        const double uxr = limit_face_clamped(sm.ax[M], b0u, b0d, sm.bu[M + 1], sm.bd[M + 1]);
+        const double uxl = limit_face_clamped(sm.ax[M - 1], sm.bu[M - 1], sm.bd[M - 1], b0u, b0d);
+        const double vyr = limit_face_clamped(sm.ay[M], b0u, b0d, sm.bu[M + RX], sm.bd[M + RX]);
+        const double vyl =
+            limit_face_clamped(sm.ay[M - RX], sm.bu[M - RX], sm.bd[M - RX], b0u, b0d);
+        const double wz = limit_face_clamped(sm.az[M], b0u, b0d, b0u, b0d);
+        const double c = sm.c1[M];
+        double r = donor(c, sm.c1[M - 1], sm.c1[M + 1], sm.c1[M - RX], sm.c1[M + RX], c, c, uxl,
+                         uxr, vyl, vyr, wz, wz);
+        const size_t o = col(rx, ry);
+        if (EMIT && a.q) r = r + a.q[o];
+        a.cout[o] = r;
+        if (a.Lx) {
+            a.Lx[o] = uxr;
+            a.Ly[o] = vyr;
+            a.Lz[o] = wz;
+        }
+    }
+}
+
+template <bool DONOR>
+void launch_plane_as(const FusedArgs& a, bool emit, hipStream_t s, const hipEvent_t* ev)
+{
+    const dim3 grid((unsigned)(a.tiles_x * a.tiles_y)), blk(P2_NTH);
+    if (emit)
+        launch_k(k_f12_plane<P2_TX, P2_TY, DONOR, true>, grid, blk, s, ev, a);
+    else
+        launch_k(k_f12_plane<P2_TX, P2_TY, DONOR, false>, grid, blk, s, ev, a);
+}
+
 }  // namespace
 
 #ifdef MPD_FAST_DIV
@@ -685,6 +882,19 @@ int MPD_CAT(onepass_launch, MPD_ARITH)(const FusedArgs& a, const Work& w, int gr
                                        bool emit, hipStream_t s, const hipEvent_t* ev)
 {
     return launch_f12(a, w, grid, donor, emit, s, ev);
+}
+
+int MPD_CAT(plane_launch, MPD_ARITH)(FusedArgs a, bool donor, bool emit, hipStream_t s,
+                                     const hipEvent_t* ev)
+{
+    a.tiles_x = (a.nx + P2_TX - 1) / P2_TX;
+    a.tiles_y = (a.ny + P2_TY - 1) / P2_TY;
+    if (donor)
+        launch_plane_as<true>(a, emit, s, ev);
+    else
+        launch_plane_as<false>(a, emit, s, ev);
+    MPD_HIP(hipGetLastError());
+    return MPD_OK;
 }
 
 int MPD_CAT(fused_batch_launch_f12, MPD_ARITH)(const FusedArgs* table, const int* item0,

@@ -7,7 +7,8 @@ three-cell ring.  Cases cover grids narrower than the ring (the region wraps ont
 extents that are not multiples of the tile, z-chunks of one plane up to the whole
 column, iord 2..4 (passes after the first read the previous pass's field and limited
 faces) and the emission increment.  Exact arithmetic: bit for bit equal to the oracle;
-fast arithmetic: <= 1e-12 normwise per step.
+fast arithmetic: <= 1e-12 normwise per step.  Grids of one z-plane take the kernel's
+single-plane form (path 4), tested the same way.
 """
 import numpy as np
 import pytest
@@ -103,12 +104,60 @@ def test_automatic_path_selection_and_agreement():
     np.testing.assert_array_equal(out[(3, M.ARITH_EXACT)], out[(2, M.ARITH_EXACT)])
     assert relerr(out[(0, M.ARITH_FAST)], out[(2, M.ARITH_EXACT)]) <= 2 * TOL
     for dims, nonosc, want in (((128, 128, 128), True, 3), ((7, 6, 8), True, 3),
-                               ((128, 128, 7), True, 2), ((256, 128, 128), False, 2)):
+                               ((128, 128, 7), True, 2), ((256, 128, 128), False, 2),
+                               ((128, 128, 1), True, 4), ((128, 128, 1), False, 2)):
         g = M.Grid(*dims, M.PERIODIC, iord=2, nonosc=nonosc)
         assert g.path() == want
         g.set_path(3)
         assert g.path() == (3 if nonosc else 2)
+        g.set_path(4)
+        assert g.path() == (4 if nonosc and dims[2] == 1 else 2)
         g.close()
+
+
+PLANES = [(128, 128), (33, 17), (7, 5), (1, 9), (16, 16), (40, 3), (17, 1)]
+
+
+@pytest.mark.parametrize("dims", PLANES)
+@pytest.mark.parametrize("arith", [M.ARITH_EXACT, M.ARITH_FAST])
+def test_single_plane_pass_matches_oracle(dims, arith):
+    """Grids of one z-plane take the single-plane form of the one-kernel pass (path 4,
+    automatic there): four stages over the plane, every z-neighbour the cell itself.
+    Tiles of 16 x 16 with a three-cell ring, so the cases cover planes narrower than the
+    ring in either axis and extents that are not multiples of the tile.  Exact arithmetic
+    bit for bit equal to the oracle over three steps, fast within 1e-12 per step; the
+    explicit and the automatic path agree bit for bit."""
+    nx, ny = dims
+    U, V, W, c, q = case(nx, ny, 1, 7 * nx + ny)
+    for iord in (2, 3, 4):
+        for with_q in (False, True):
+            fields = []
+            for path in (4, 0):
+                print(f"case plane dims={dims} arith={arith} iord={iord} q={with_q} "
+                      f"path={path}", flush=True)
+                g = M.Grid(nx, ny, 1, M.PERIODIC, iord=iord, nonosc=True)
+                g.set_path(path)
+                assert g.path() == 4
+                g.set_arith(arith)
+                g.set_courant(U, V, W)
+                g.set_field(c)
+                if with_q:
+                    g.set_emission(q)
+                want = c
+                for s in range(3):
+                    g.step(1)
+                    want = O.per_step(nx, ny, 1, U, V, W, iord, True, want,
+                                      q if with_q else None)
+                    got = g.field()
+                    if arith == M.ARITH_EXACT:
+                        if not np.array_equal(got, want):
+                            raise AssertionError(f"step {s}: differs, rel {relerr(got, want):.3e}")
+                    else:
+                        e = relerr(got, want)
+                        assert e <= TOL, f"step {s}: rel err {e:.3e}"
+                fields.append(got)
+                g.close()
+            np.testing.assert_array_equal(fields[0], fields[1])
 
 
 @pytest.mark.parametrize("dims", [(40, 20, 10), (33, 17, 5), (7, 6, 4)])
