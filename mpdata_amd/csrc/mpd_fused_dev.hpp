@@ -29,7 +29,8 @@ int onepass_launch_exact(const FusedArgs& a, const Work& w, int grid, bool donor
 int onepass_launch_fast(const FusedArgs& a, const Work& w, int grid, bool donor, bool emit,
                         hipStream_t s, const hipEvent_t* ev);
 // One launch of the single-plane form of the one-kernel pass over descriptor a of a grid
-// of one z-plane (16 x 16 tiles, one workgroup each); donor, emit and ev as above.
+// of one z-plane (16 x 16 or, on small planes, 8 x 8 tiles, one workgroup each); donor,
+// emit and ev as above.
 int plane_launch_exact(FusedArgs a, bool donor, bool emit, hipStream_t s,
                        const hipEvent_t* ev);
 int plane_launch_fast(FusedArgs a, bool donor, bool emit, hipStream_t s, const hipEvent_t* ev);

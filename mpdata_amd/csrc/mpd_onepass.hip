@@ -691,7 +691,7 @@ struct Plane2 {
     double cn[R], u[R], v[R], w[R], c1[R], ax[R], ay[R], az[R], bu[R], bd[R];
 };
 
-constexpr int P2_TX = 16, P2_TY = 16, P2_NTH = 256;
+constexpr int P2_NTH = 256;
 
 template <int TX, int TY, bool DONOR, bool EMIT>
 __global__ void __launch_bounds__(P2_NTH) k_f12_plane(FusedArgs a)
@@ -861,14 +861,30 @@ __global__ void __launch_bounds__(P2_NTH) k_f12_plane(FusedArgs a)
     }
 }
 
-template <bool DONOR>
-void launch_plane_as(const FusedArgs& a, bool emit, hipStream_t s, const hipEvent_t* ev)
+template <int TX, int TY, bool DONOR>
+void launch_plane_as(FusedArgs a, bool emit, hipStream_t s, const hipEvent_t* ev)
 {
+    a.tiles_x = (a.nx + TX - 1) / TX;
+    a.tiles_y = (a.ny + TY - 1) / TY;
     const dim3 grid((unsigned)(a.tiles_x * a.tiles_y)), blk(P2_NTH);
     if (emit)
-        launch_k(k_f12_plane<P2_TX, P2_TY, DONOR, true>, grid, blk, s, ev, a);
+        launch_k(k_f12_plane<TX, TY, DONOR, true>, grid, blk, s, ev, a);
     else
-        launch_k(k_f12_plane<P2_TX, P2_TY, DONOR, false>, grid, blk, s, ev, a);
+        launch_k(k_f12_plane<TX, TY, DONOR, false>, grid, blk, s, ev, a);
+}
+
+// Tile of the single-plane pass: 16 x 16, or 8 x 8 when the plane has fewer 16 x 16
+// tiles than the chip has CUs (its stages are then latency, not throughput: 128 x 128,
+// iord 2, measured 3.9 us per step in 8 x 8 tiles against 5.7 in 16 x 16 and 7.8 in
+// 32 x 16; 512 x 512: 11.4 in 16 x 16, 13.5 in 8 x 8; 2048 x 2048: 107, 143).
+template <bool DONOR>
+void launch_plane(const FusedArgs& a, bool emit, hipStream_t s, const hipEvent_t* ev)
+{
+    const long long tiles16 = (long long)((a.nx + 15) / 16) * ((a.ny + 15) / 16);
+    if (tiles16 < 256)
+        launch_plane_as<8, 8, DONOR>(a, emit, s, ev);
+    else
+        launch_plane_as<16, 16, DONOR>(a, emit, s, ev);
 }
 
 }  // namespace
@@ -887,12 +903,10 @@ int MPD_CAT(onepass_launch, MPD_ARITH)(const FusedArgs& a, const Work& w, int gr
 int MPD_CAT(plane_launch, MPD_ARITH)(FusedArgs a, bool donor, bool emit, hipStream_t s,
                                      const hipEvent_t* ev)
 {
-    a.tiles_x = (a.nx + P2_TX - 1) / P2_TX;
-    a.tiles_y = (a.ny + P2_TY - 1) / P2_TY;
     if (donor)
-        launch_plane_as<true>(a, emit, s, ev);
+        launch_plane<true>(a, emit, s, ev);
     else
-        launch_plane_as<false>(a, emit, s, ev);
+        launch_plane<false>(a, emit, s, ev);
     MPD_HIP(hipGetLastError());
     return MPD_OK;
 }

@@ -115,7 +115,9 @@ def test_automatic_path_selection_and_agreement():
         g.close()
 
 
-PLANES = [(128, 128), (33, 17), (7, 5), (1, 9), (16, 16), (40, 3), (17, 1)]
+# (planes with fewer 16 x 16 tiles than 256 take 8 x 8 tiles, the last two 16 x 16)
+PLANES = [(128, 128), (33, 17), (7, 5), (1, 9), (16, 16), (40, 3), (17, 1), (256, 256),
+          (260, 250)]
 
 
 @pytest.mark.parametrize("dims", PLANES)
@@ -123,8 +125,8 @@ PLANES = [(128, 128), (33, 17), (7, 5), (1, 9), (16, 16), (40, 3), (17, 1)]
 def test_single_plane_pass_matches_oracle(dims, arith):
     """Grids of one z-plane take the single-plane form of the one-kernel pass (path 4,
     automatic there): four stages over the plane, every z-neighbour the cell itself.
-    Tiles of 16 x 16 with a three-cell ring, so the cases cover planes narrower than the
-    ring in either axis and extents that are not multiples of the tile.  Exact arithmetic
+    Tiles of 8 x 8 or 16 x 16 with a three-cell ring, so the cases cover planes narrower
+    than the ring in either axis and extents that are not multiples of the tile.  Exact arithmetic
     bit for bit equal to the oracle over three steps, fast within 1e-12 per step; the
     explicit and the automatic path agree bit for bit."""
     nx, ny = dims
