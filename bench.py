@@ -15,7 +15,10 @@ Prints ONE JSON line:
                the step's kernels (each launched between HIP events of its own), against
                MI355X's 8 TB/s HBM peak; traffic = rocprof HBM bytes of a step (FETCH_SIZE
                + WRITE_SIZE, calibrated against a plain copy) when measured for the same
-               launch shapes; per-kernel figures in roofline.kernels
+               launch shapes; per-kernel figures in roofline.kernels; roofline.valu, the
+               vector-ALU side from the rocprof instruction counts of the same launch
+               shapes: VALU issue rate, FP64 TFLOP/s and the HBM fraction the step would
+               reach at full issue (its instruction-throughput ceiling)
   parity       max|gpu - oracle| / max|oracle| after the CPU-baseline steps, and the whole
                timed trajectory against the exact-arithmetic kernels (which equal the
                oracle bit for bit)
@@ -42,6 +45,26 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published peak
 BYTES_PER_CELL = 40    # c^n + U + V + W read, c^{n+1} written (FP64)
+# the vector ALU side of MI355X: 256 CUs, each issuing at most one wave64 VALU instruction
+# per cycle (FP64 included), at the 2.4 GHz peak engine clock; FP64 vector peak 78.6 TFLOP/s
+CUS, CLOCK_GHZ, FP64_PEAK_TFLOPS = 256, 2.4, 78.6
+
+
+def valu_roofline(counts, kern_ms, hbm_frac):
+    """The step's vector-ALU side from its rocprof instruction counts (wave64 VALU
+    instructions and FP64 FLOP per step, measured with the same launch shapes) over the
+    live kernel time: the VALU issue rate as a fraction of one instruction per CU and
+    cycle, the FP64 rate against its peak, and the HBM fraction the same step would reach
+    at full issue (the instruction-throughput ceiling in the HBM roofline's terms)."""
+    if not counts:
+        return None
+    issue = counts["valu_wave_instr_per_step"] / (kern_ms * 1e-3 * CUS * CLOCK_GHZ * 1e9)
+    tflops = counts["fp64_flop_per_step"] / (kern_ms * 1e-3) / 1e12
+    return {"valu_wave_instr_per_step": counts["valu_wave_instr_per_step"],
+            "issue_frac": issue, "fp64_TFLOP/s": tflops,
+            "fp64_frac": tflops / FP64_PEAK_TFLOPS,
+            "hbm_frac_at_full_issue": hbm_frac / issue,
+            "source": counts.get("valu_source")}
 DEFORM_SHAPE = (512, 512, 256)
 
 
@@ -147,16 +170,20 @@ def roofline(cells, kms, kernels, tfile, tuned):
         b = k["io_bytes_per_cell"]
         k["io_GB/s"] = (cells * b / (k["avg_ms"] * 1e-3) / 1e9) if b else None
     dominant = max(kernels, key=lambda k: k["avg_ms"])
-    traffic = None
+    traffic, counts = None, None
     if tfile and os.path.exists(tfile):
         t = json.load(open(tfile))
         # counters measured with the same launch shapes only
         if t.get("config") == tuned:
             traffic = t.get("hbm_bytes_per_step")
+            if "valu_wave_instr_per_step" in t:
+                counts = t
+    frac = achieved / HBM_PEAK_GBS
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "frac": frac, "traffic": traffic,
             "algorithmic_bytes_per_step": cells * BYTES_PER_CELL, "step_kernel_ms": kern_ms,
-            "dominant_kernel": dominant["name"], "kernels": kernels}
+            "dominant_kernel": dominant["name"], "kernels": kernels,
+            "valu": valu_roofline(counts, kern_ms, frac)}
 
 
 def cpu_calibration(O, n=128, order=2, reps=2):
@@ -593,9 +620,16 @@ def large_leg(M, n, iord, arith, check=True, path=0):
     cells = n ** 3
     kern = sum(kms)
     ach = cells * BYTES_PER_CELL / (kern * 1e-3) / 1e9
+    tfile = os.path.join(ROOT, "profiles", f"traffic_rotation{n}_iord{iord}.json")
+    counts = None
+    if arith == 1 and os.path.exists(tfile):
+        t = json.load(open(tfile))
+        if t.get("config") == tuned and "valu_wave_instr_per_step" in t:
+            counts = t
     return {"workload": f"periodic {n}^3 rotation, iord {iord} nonoscillatory",
             "autotuned": tuned, "ms_per_step": ms, "cell_updates_per_s": cells / (ms * 1e-3),
             "kernel_ms": kms, "achieved_GB/s": ach, "frac_of_hbm_peak": ach / HBM_PEAK_GBS,
+            "valu": valu_roofline(counts, kern, ach / HBM_PEAK_GBS),
             "rel_err_1_step_vs_oracle": err1, f"rel_err_{taken}_steps_vs_exact": err_run}
 
 
