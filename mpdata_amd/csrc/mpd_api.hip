@@ -6,6 +6,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/smolar.h"
@@ -85,15 +86,119 @@ int periodic_step(mpd_grid_s* g)
     return periodic_staged_step(g);
 }
 
+// Copies between the caller's (pageable) host arrays and the device.  Copies of 8 MiB
+// and more go through two pinned staging buffers of 8 MiB: chunks alternate between
+// them, the host copying one (on four threads) while the DMA engine moves the other.
+// Measured on the MI355X host link with 64 MiB: pageable 28-30 GB/s to the device and
+// 17 GB/s back; staged 44 GB/s either way (one host thread: 27-29; pinned DMA alone 56).
+// The buffers are per thread (the drop-in smolar() is reentrant per thread), allocated
+// on the first large copy and kept until the process ends.  h2d returns once the host
+// array has been read (its DMA may still be in flight); d2h returns with the host array
+// written.
+namespace {
+// (below 8 MiB the direct pageable copy was as fast: 64^3 smolar() 0.63 ms per call direct,
+// 0.78 staged)
+constexpr size_t kStageChunk = (size_t)8 << 20, kStageMin = (size_t)8 << 20;
+constexpr int kCopyThreads = 4;
+
+// memcpy split over the calling thread and kCopyThreads - 1 helpers
+void host_copy(char* dst, const char* src, size_t len)
+{
+    if (len < ((size_t)2 << 20)) {
+        std::memcpy(dst, src, len);
+        return;
+    }
+    std::thread helper[kCopyThreads - 1];
+    for (int k = 1; k < kCopyThreads; k++) {
+        const size_t a = len * k / kCopyThreads, b = len * (k + 1) / kCopyThreads;
+        helper[k - 1] = std::thread([=] { std::memcpy(dst + a, src + a, b - a); });
+    }
+    std::memcpy(dst, src, len / kCopyThreads);
+    for (auto& t : helper) t.join();
+}
+
+struct Staging {
+    char* buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool busy[2] = {false, false};  // an enqueued DMA may still use the buffer
+    int init()
+    {
+        if (buf[1]) return MPD_OK;
+        for (int i = 0; i < 2; i++) {
+            if (!buf[i]) MPD_HIP(hipHostMalloc((void**)&buf[i], kStageChunk, hipHostMallocDefault));
+            if (!ev[i]) MPD_HIP(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+        }
+        return MPD_OK;
+    }
+    // waits for the last DMA of buffer i
+    int settle(int i)
+    {
+        if (busy[i]) MPD_HIP(hipEventSynchronize(ev[i]));
+        busy[i] = false;
+        return MPD_OK;
+    }
+    int dma(int i, void* dst, const void* src, size_t len, hipMemcpyKind kind, hipStream_t s)
+    {
+        MPD_HIP(hipMemcpyAsync(dst, src, len, kind, s));
+        MPD_HIP(hipEventRecord(ev[i], s));
+        busy[i] = true;
+        return MPD_OK;
+    }
+};
+thread_local Staging t_stage;
+}  // namespace
+
 static int h2d(double* d, const double* h, size_t n)
 {
-    MPD_HIP(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, stream()));
+    const size_t bytes = n * sizeof(double);
+    hipStream_t s = stream();
+    if (bytes < kStageMin) {
+        MPD_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
+        return MPD_OK;
+    }
+    int rc;
+    if ((rc = t_stage.init())) return rc;
+    const char* src = (const char*)h;
+    char* dst = (char*)d;
+    for (size_t off = 0, k = 0; off < bytes; off += kStageChunk, k++) {
+        const int i = (int)(k & 1);
+        const size_t len = std::min(kStageChunk, bytes - off);
+        if ((rc = t_stage.settle(i))) return rc;
+        host_copy(t_stage.buf[i], src + off, len);
+        if ((rc = t_stage.dma(i, dst + off, t_stage.buf[i], len, hipMemcpyHostToDevice, s)))
+            return rc;
+    }
     return MPD_OK;
 }
 
 static int d2h(double* h, const double* d, size_t n)
 {
-    MPD_HIP(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    const size_t bytes = n * sizeof(double);
+    hipStream_t s = stream();
+    if (bytes < kStageMin) {
+        MPD_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, s));
+        return MPD_OK;
+    }
+    int rc;
+    if ((rc = t_stage.init())) return rc;
+    const char* src = (const char*)d;
+    char* dst = (char*)h;
+    const size_t nch = (bytes + kStageChunk - 1) / kStageChunk;
+    auto issue = [&](size_t k) {
+        const int i = (int)(k & 1);
+        const size_t off = k * kStageChunk, len = std::min(kStageChunk, bytes - off);
+        int r = t_stage.settle(i);
+        return r ? r : t_stage.dma(i, t_stage.buf[i], src + off, len, hipMemcpyDeviceToHost, s);
+    };
+    if ((rc = issue(0))) return rc;
+    for (size_t k = 0; k < nch; k++) {
+        // the next chunk's DMA runs while this chunk is copied out on the host
+        if (k + 1 < nch && (rc = issue(k + 1))) return rc;
+        const int i = (int)(k & 1);
+        if ((rc = t_stage.settle(i))) return rc;
+        const size_t off = k * kStageChunk;
+        host_copy(dst + off, t_stage.buf[i], std::min(kStageChunk, bytes - off));
+    }
     return MPD_OK;
 }
 
