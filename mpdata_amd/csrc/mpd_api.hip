@@ -101,7 +101,8 @@ namespace {
 constexpr size_t kStageChunk = (size_t)8 << 20, kStageMin = (size_t)8 << 20;
 constexpr int kCopyThreads = 4;
 
-// memcpy split over the calling thread and kCopyThreads - 1 helpers
+// memcpy split over the calling thread and up to kCopyThreads - 1 helpers (a helper that
+// cannot be started leaves its part to the calling thread; nothing throws past the ABI)
 void host_copy(char* dst, const char* src, size_t len)
 {
     if (len < ((size_t)2 << 20)) {
@@ -109,12 +110,19 @@ void host_copy(char* dst, const char* src, size_t len)
         return;
     }
     std::thread helper[kCopyThreads - 1];
+    bool started[kCopyThreads - 1] = {};
     for (int k = 1; k < kCopyThreads; k++) {
         const size_t a = len * k / kCopyThreads, b = len * (k + 1) / kCopyThreads;
-        helper[k - 1] = std::thread([=] { std::memcpy(dst + a, src + a, b - a); });
+        try {
+            helper[k - 1] = std::thread([=] { std::memcpy(dst + a, src + a, b - a); });
+            started[k - 1] = true;
+        } catch (...) {
+            std::memcpy(dst + a, src + a, b - a);
+        }
     }
     std::memcpy(dst, src, len / kCopyThreads);
-    for (auto& t : helper) t.join();
+    for (int k = 0; k < kCopyThreads - 1; k++)
+        if (started[k]) helper[k].join();
 }
 
 struct Staging {
