@@ -1379,40 +1379,49 @@ int adjoint_supported(const mpd_grid_s* g)
 // One adjoint step on host arrays (for tests): lin = A'(c)^T lout, lq += lout.
 extern "C" int mpd_set_adjoint_scatter(mpd_grid g, int atomic)
 {
+    MPD_API_BEGIN
     clear_error();
     if (!g) return fail(MPD_ERR_ARG, "mpd_set_adjoint_scatter: NULL grid");
     g->adj_scatter = atomic ? 1 : 0;
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_set_adjoint_tiled(mpd_grid g, int tiled)
 {
+    MPD_API_BEGIN
     clear_error();
     if (!g) return fail(MPD_ERR_ARG, "mpd_set_adjoint_tiled: NULL grid");
     g->adj_tiled = tiled ? 1 : 0;
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_set_adjoint_tape(mpd_grid g, long long max_bytes)
 {
+    MPD_API_BEGIN
     clear_error();
     if (!g) return fail(MPD_ERR_ARG, "mpd_set_adjoint_tape: NULL grid");
     g->adj_tape = max_bytes;
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_get_adjoint_tape(mpd_grid g, long long* max_bytes, int* kept_segments)
 {
+    MPD_API_BEGIN
     clear_error();
     if (!g) return fail(MPD_ERR_ARG, "mpd_get_adjoint_tape: NULL grid");
     if (max_bytes) *max_bytes = g->adj_tape;
     if (kept_segments) *kept_segments = g->adj_kept;
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_adjoint_step(mpd_grid g, const double* c, const double* lout, double* lin,
                                 double* lq)
 {
+    MPD_API_BEGIN
     clear_error();
     if (!g || !c || !lout || !lin) return fail(MPD_ERR_ARG, "mpd_adjoint_step: NULL argument");
     int rc;
@@ -1436,6 +1445,7 @@ extern "C" int mpd_adjoint_step(mpd_grid g, const double* c, const double* lout,
     if (lq) MPD_HIP(hipMemcpyAsync(lq, dq.p, g->ncell * 8, hipMemcpyDeviceToHost, s));
     MPD_HIP(hipStreamSynchronize(s));
     return MPD_OK;
+    MPD_API_END
 }
 
 // Forward run from the grid's field recording the receptor values every sample_every
@@ -1443,6 +1453,7 @@ extern "C" int mpd_adjoint_step(mpd_grid g, const double* c, const double* lout,
 extern "C" int mpd_sample_receptors(mpd_grid g, int nsteps, int nrec, const int* receptors,
                                     int sample_every, double* out)
 {
+    MPD_API_BEGIN
     clear_error();
     if (!g || nsteps < 1 || nrec < 1 || !receptors || sample_every < 1 || !out)
         return fail(MPD_ERR_ARG, "mpd_sample_receptors: bad arguments");
@@ -1481,6 +1492,7 @@ extern "C" int mpd_sample_receptors(mpd_grid g, int nsteps, int nrec, const int*
         MPD_HIP(hipMemcpyAsync(out, vals.p, (size_t)nsamp * nrec * 8, hipMemcpyDeviceToHost, s));
     MPD_HIP(hipStreamSynchronize(s));
     return MPD_OK;
+    MPD_API_END
 }
 
 // Receptor-misfit loss and its gradient.  From the grid's current field c^0 and
@@ -1497,6 +1509,7 @@ extern "C" int mpd_adjoint_run(mpd_grid g, int nsteps, int checkpoint_every, int
                                const int* receptors, int sample_every, const double* obs,
                                double* J, double* grad_c0, double* grad_q, double* times)
 {
+    MPD_API_BEGIN
     clear_error();
     if (!g || nsteps < 1 || checkpoint_every < 1 || nrec < 1 || !receptors ||
         sample_every < 1 || !obs || !J)
@@ -1710,4 +1723,5 @@ extern "C" int mpd_adjoint_run(mpd_grid g, int nsteps, int checkpoint_every, int
         times[2] = fr;
     }
     return MPD_OK;
+    MPD_API_END
 }

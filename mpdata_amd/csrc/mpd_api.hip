@@ -266,6 +266,7 @@ extern "C" int smolar(int nx, int ny, int ns, double* dx, double* dy, double* ds
                       double* w0, double* w1, double** c0, double** c1, int flow_type,
                       int order)
 {
+    MPD_API_BEGIN
     (void)flow_type;  // never read by the reference (uu.c:111)
     clear_error();
     if (!dx || !dy || !ds || !u0 || !u1 || !v0 || !v1 || !w0 || !w1 || !c0 || !c1 || !*c0 ||
@@ -296,6 +297,7 @@ extern "C" int smolar(int nx, int ny, int ns, double* dx, double* dy, double* ds
         *c0 = h1;
     }
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int msmolar(void** desc, int nx, int ny, int ns, double* dx, double* dy,
@@ -303,6 +305,7 @@ extern "C" int msmolar(void** desc, int nx, int ny, int ns, double* dx, double* 
                        double* v1, double* w0, double* w1, double** c0, double** c1,
                        int flow_type, int order, int sx, int sy, int ss, int mode)
 {
+    MPD_API_BEGIN
     (void)flow_type;
     (void)sx;
     (void)sy;
@@ -339,6 +342,7 @@ extern "C" int msmolar(void** desc, int nx, int ny, int ns, double* dx, double* 
         *c1 = t;
     }
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int msmolar_gather(void** desc, int nx, int ny, int ns, double* dx, double* dy,
@@ -347,6 +351,7 @@ extern "C" int msmolar_gather(void** desc, int nx, int ny, int ns, double* dx, d
                               double** c1, int flow_type, int order, int sx, int sy, int ss,
                               int mode)
 {
+    MPD_API_BEGIN
     (void)dx; (void)dy; (void)ds; (void)dt; (void)u0; (void)u1; (void)v0; (void)v1;
     (void)w0; (void)w1; (void)flow_type; (void)order; (void)sx; (void)sy; (void)ss;
     (void)mode; (void)nx; (void)ny; (void)ns;
@@ -359,15 +364,18 @@ extern "C" int msmolar_gather(void** desc, int nx, int ny, int ns, double* dx, d
         (rc = d2h(*c0, g->c[1 - g->cur].p, g->ncell)) || (rc = sync()))
         return rc;
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int msmolar_free(void** desc)
 {
+    MPD_API_BEGIN
     if (desc && *desc) {
         delete (mpd_grid_s*)*desc;
         *desc = nullptr;
     }
     return MPD_OK;
+    MPD_API_END
 }
 
 // -------------------------------------------------------------- mpd_* interface ---
@@ -381,6 +389,7 @@ extern "C" int msmolar_free(void** desc)
 extern "C" int mpd_create(mpd_grid* out, int nx, int ny, int nz, int boundary, int iord,
                           int nonosc)
 {
+    MPD_API_BEGIN
     clear_error();
     if (!out) return fail(MPD_ERR_ARG, "mpd_create: NULL handle pointer");
     *out = nullptr;
@@ -392,21 +401,25 @@ extern "C" int mpd_create(mpd_grid* out, int nx, int ny, int nz, int boundary, i
     }
     *out = g;
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_destroy(mpd_grid g)
 {
+    MPD_API_BEGIN
     if (g) {
         (void)hipStreamSynchronize(stream());
         delete g;
     }
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_set_winds(mpd_grid g, const double* dx, const double* dy, const double* dz,
                              double dt, const double* u0, const double* u1, const double* v0,
                              const double* v1, const double* w0, const double* w1)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     if (!dx || !dy || !dz || !u0 || !u1 || !v0 || !v1 || !w0 || !w1)
         return fail(MPD_ERR_ARG, "mpd_set_winds: NULL array");
@@ -415,10 +428,12 @@ extern "C" int mpd_set_winds(mpd_grid g, const double* dx, const double* dy, con
     int rc = winds_to_courant(g, dx, dy, dz, dt, w, sp, wd);
     if (rc) return rc;
     return sync();  // the temporaries are freed on return
+    MPD_API_END
 }
 
 extern "C" int mpd_set_courant(mpd_grid g, const double* U, const double* V, const double* W)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     if (!U || !V || !W) return fail(MPD_ERR_ARG, "mpd_set_courant: NULL array");
     const double* h[3] = {U, V, W};
@@ -427,10 +442,12 @@ extern "C" int mpd_set_courant(mpd_grid g, const double* U, const double* V, con
         if ((rc = h2d(g->face[d].p, h[d], g->nface[d]))) return rc;
     g->has_winds = true;
     return sync();
+    MPD_API_END
 }
 
 extern "C" int mpd_get_courant(mpd_grid g, double* U, double* V, double* W)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     if (!U || !V || !W) return fail(MPD_ERR_ARG, "mpd_get_courant: NULL array");
     double* h[3] = {U, V, W};
@@ -438,38 +455,46 @@ extern "C" int mpd_get_courant(mpd_grid g, double* U, double* V, double* W)
     for (int d = 0; d < 3; d++)
         if ((rc = d2h(h[d], g->face[d].p, g->nface[d]))) return rc;
     return sync();
+    MPD_API_END
 }
 
 extern "C" int mpd_set_field(mpd_grid g, const double* c)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     if (!c) return fail(MPD_ERR_ARG, "mpd_set_field: NULL array");
     int rc = h2d(g->c[g->cur].p, c, g->ncell);
     if (rc) return rc;
     g->has_field = true;
     return sync();
+    MPD_API_END
 }
 
 extern "C" int mpd_get_field(mpd_grid g, double* c)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     if (!c) return fail(MPD_ERR_ARG, "mpd_get_field: NULL array");
     int rc = d2h(c, g->c[g->cur].p, g->ncell);
     if (rc) return rc;
     return sync();
+    MPD_API_END
 }
 
 extern "C" int mpd_get_previous_field(mpd_grid g, double* c)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     if (!c) return fail(MPD_ERR_ARG, "mpd_get_previous_field: NULL array");
     int rc = d2h(c, g->c[1 - g->cur].p, g->ncell);
     if (rc) return rc;
     return sync();
+    MPD_API_END
 }
 
 extern "C" int mpd_set_emission(mpd_grid g, const double* q)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     if (!q) {
         g->has_q = false;
@@ -482,16 +507,19 @@ extern "C" int mpd_set_emission(mpd_grid g, const double* q)
     if ((rc = h2d(g->q.p, q, g->ncell))) return rc;
     g->has_q = true;
     return sync();
+    MPD_API_END
 }
 
 extern "C" int mpd_get_emission(mpd_grid g, double* q)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     if (!q) return fail(MPD_ERR_ARG, "mpd_get_emission: NULL array");
     if (!g->has_q) return fail(MPD_ERR_ARG, "mpd_get_emission: no emission is set");
     int rc = d2h(q, g->q.p, g->ncell);
     if (rc) return rc;
     return sync();
+    MPD_API_END
 }
 
 // Steps per captured graph: even, so a replay leaves the field parity unchanged.
@@ -546,6 +574,7 @@ static int graph_steps(mpd_grid_s* g)
 
 extern "C" int mpd_step(mpd_grid g, int nsteps)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     if (nsteps < 0) return fail(MPD_ERR_ARG, "mpd_step: nsteps %d < 0", nsteps);
     if (!g->has_winds || !g->has_field)
@@ -561,10 +590,12 @@ extern "C" int mpd_step(mpd_grid g, int nsteps)
             return rc;
     }
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_set_sentinel(mpd_grid g, int every, int nonnegative)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     if (every < 0) return fail(MPD_ERR_ARG, "mpd_set_sentinel: every %d < 0", every);
     if (every && !g->sentinel_first)
@@ -575,10 +606,12 @@ extern "C" int mpd_set_sentinel(mpd_grid g, int every, int nonnegative)
     g->sentinel_nonneg = nonnegative ? 1 : 0;
     g->sentinel_steps = 0;
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_sentinel_check(mpd_grid g, long long* first_bad_step)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     if (first_bad_step) *first_bad_step = -1;
     if (!g->sentinel_first) return MPD_OK;
@@ -589,31 +622,39 @@ extern "C" int mpd_sentinel_check(mpd_grid g, long long* first_bad_step)
     if (first_bad_step) *first_bad_step = (long long)f;
     return fail(MPD_ERR_NUMERIC, "step sentinel: %s value in the field after step %llu",
                 g->sentinel_nonneg ? "a non-finite or negative" : "a non-finite", f);
+    MPD_API_END
 }
 
 extern "C" int mpd_set_graph(mpd_grid g, int on)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     g->use_graph = on != 0;
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_synchronize(mpd_grid g)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     return sync();
+    MPD_API_END
 }
 
 extern "C" int mpd_field_device_ptr(mpd_grid g, double** dptr)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     if (!dptr) return fail(MPD_ERR_ARG, "mpd_field_device_ptr: NULL");
     *dptr = g->c[g->cur].p;
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_step_timed(mpd_grid g, int nsteps, double* ms)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     if (!ms) return fail(MPD_ERR_ARG, "mpd_step_timed: NULL");
     hipEvent_t a, b;
@@ -632,6 +673,7 @@ extern "C" int mpd_step_timed(mpd_grid g, int nsteps, double* ms)
     if (e != hipSuccess) return hip_fail(e, "mpd_step_timed event");
     *ms = f;
     return MPD_OK;
+    MPD_API_END
 }
 
 namespace mpd {
@@ -647,6 +689,7 @@ void trace_mark(mpd_grid_s* g)
 
 extern "C" int mpd_step_kernel_times(mpd_grid g, int nsteps, double* ms, int max, int* n)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     if (!ms || !n || nsteps < 1 || max < 1)
         return fail(MPD_ERR_ARG, "mpd_step_kernel_times: bad arguments");
@@ -691,6 +734,7 @@ extern "C" int mpd_step_kernel_times(mpd_grid g, int nsteps, double* ms, int max
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     (void)hipEventDestroy(start);
     return rc;
+    MPD_API_END
 }
 
 namespace {
@@ -704,6 +748,7 @@ __global__ void k_copy(const double* __restrict__ src, double* __restrict__ dst,
 
 extern "C" int mpd_calibration_copy(size_t n, int reps, double* ms)
 {
+    MPD_API_BEGIN
     clear_error();
     if (n == 0 || reps < 1 || !ms) return fail(MPD_ERR_ARG, "mpd_calibration_copy: bad arguments");
     int rc = ensure_device();
@@ -727,43 +772,53 @@ extern "C" int mpd_calibration_copy(size_t n, int reps, double* ms)
     (void)hipEventDestroy(e1);
     *ms = f / reps;
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_field_stats(mpd_grid g, double* mass, double* cmin, double* cmax)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     return field_stats(g, mass, cmin, cmax);
+    MPD_API_END
 }
 
 extern "C" int mpd_set_path(mpd_grid g, int path)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     if (path < 0 || path > 4) return fail(MPD_ERR_ARG, "mpd_set_path: %d not in 0..4", path);
     g->path = path;
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_set_arith(mpd_grid g, int arith)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     if (arith != MPD_ARITH_EXACT && arith != MPD_ARITH_FAST)
         return fail(MPD_ERR_ARG, "mpd_set_arith: %d is neither exact nor fast", arith);
     g->arith = arith;
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_set_tile(mpd_grid g, int tile_x, int tile_y)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     if (tile_x < 0 || tile_y < 0) return fail(MPD_ERR_ARG, "mpd_set_tile: negative tile");
     g->tile_x = tile_x;
     g->tile_y = tile_y;
     g->tile2_x = g->tile2_y = g->zc2 = 0;  // the limiter kernel follows
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_set_limiter_tile(mpd_grid g, int tile_x, int tile_y, int zc)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     if (tile_x < 0 || tile_y < 0 || zc < 0 || (tile_x == 0) != (tile_y == 0))
         return fail(MPD_ERR_ARG, "mpd_set_limiter_tile: bad tile or z-chunk");
@@ -771,43 +826,53 @@ extern "C" int mpd_set_limiter_tile(mpd_grid g, int tile_x, int tile_y, int zc)
     g->tile2_y = tile_y;
     g->zc2 = tile_x ? zc : 0;
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_get_tiles(mpd_grid g, int* v)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     if (!v) return fail(MPD_ERR_ARG, "mpd_get_tiles: NULL array");
     const int t[6] = {g->tile_x, g->tile_y, g->zc, g->tile2_x, g->tile2_y, g->zc2};
     for (int i = 0; i < 6; i++) v[i] = t[i];
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_set_zchunk(mpd_grid g, int zc)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     if (zc < 0) return fail(MPD_ERR_ARG, "mpd_set_zchunk: negative z-chunk length");
     g->zc = zc;
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_set_pass_zchunk(mpd_grid g, int zc)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     if (zc < 0) return fail(MPD_ERR_ARG, "mpd_set_pass_zchunk: negative z-chunk length");
     g->zc3 = zc;
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_get_pass_zchunk(mpd_grid g, int* zc)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     if (!zc) return fail(MPD_ERR_ARG, "mpd_get_pass_zchunk: NULL");
     *zc = pass_zchunk(g);
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_get_path(mpd_grid g, int* path)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     if (!path) return fail(MPD_ERR_ARG, "mpd_get_path: NULL");
     if (g->boundary == MPD_FRAME)
@@ -817,6 +882,7 @@ extern "C" int mpd_get_path(mpd_grid g, int* path)
     else
         *path = plane_kernel_passes(g) ? 4 : one_kernel_passes(g) ? 3 : 2;
     return MPD_OK;
+    MPD_API_END
 }
 
 // Autotuner of one grid: times every compiled tile shape x z-chunk length on the grid
@@ -829,6 +895,7 @@ extern "C" int mpd_get_path(mpd_grid g, int* path)
 // "grid" entry (one shape) serves both kernels.
 extern "C" int mpd_autotune(mpd_grid g, int nsteps, int* tile_x, int* tile_y, int* zc)
 {
+    MPD_API_BEGIN
     GRID_OR_FAIL(g);
     if (nsteps < 1) return fail(MPD_ERR_ARG, "mpd_autotune: nsteps < 1");
     if (g->boundary != MPD_PERIODIC || g->iord < 2 || g->path == 1 ||
@@ -977,4 +1044,5 @@ extern "C" int mpd_autotune(mpd_grid g, int nsteps, int* tile_x, int* tile_y, in
     if (tile_y) *tile_y = c[1];
     if (zc) *zc = c[2];
     return MPD_OK;
+    MPD_API_END
 }

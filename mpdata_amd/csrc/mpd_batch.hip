@@ -364,6 +364,7 @@ int run(mpd_batch_s* b, int nsteps, int mode)
 
 extern "C" int mpd_batch_create(mpd_batch* out, int ninst, const int* dims, int iord, int nonosc)
 {
+    MPD_API_BEGIN
     clear_error();
     if (!out || !dims || ninst < 1) return fail(MPD_ERR_ARG, "mpd_batch_create: bad arguments");
     *out = nullptr;
@@ -389,45 +390,55 @@ extern "C" int mpd_batch_create(mpd_batch* out, int ninst, const int* dims, int 
     }
     *out = b;
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_batch_destroy(mpd_batch b)
 {
+    MPD_API_BEGIN
     if (b) {
         (void)hipStreamSynchronize(stream());
         delete b;
     }
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_batch_instance(mpd_batch b, int i, mpd_grid* g)
 {
+    MPD_API_BEGIN
     BATCH_OR_FAIL(b);
     if (!g || i < 0 || i >= b->ninst) return fail(MPD_ERR_ARG, "mpd_batch_instance: bad index");
     *g = b->g[i];
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_batch_cells(mpd_batch b, long long* cells)
 {
+    MPD_API_BEGIN
     BATCH_OR_FAIL(b);
     if (!cells) return fail(MPD_ERR_ARG, "mpd_batch_cells: NULL");
     *cells = b->cells;
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_batch_set_arith(mpd_batch b, int arith)
 {
+    MPD_API_BEGIN
     BATCH_OR_FAIL(b);
     if (arith != MPD_ARITH_EXACT && arith != MPD_ARITH_FAST)
         return fail(MPD_ERR_ARG, "mpd_batch_set_arith: bad value");
     b->arith = arith;
     for (auto* g : b->g) g->arith = arith;
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_batch_set_tile(mpd_batch b, int tx, int ty, int zc, int persistent)
 {
+    MPD_API_BEGIN
     BATCH_OR_FAIL(b);
     bool ok = false;
 #define MPD_TILE(X, Y) ok = ok || (tx == X && ty == Y);
@@ -445,10 +456,12 @@ extern "C" int mpd_batch_set_tile(mpd_batch b, int tx, int ty, int zc, int persi
         g->tile2_x = g->tile2_y = g->zc2 = 0;
     }
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_batch_set_limiter_tile(mpd_batch b, int tx, int ty, int zc)
 {
+    MPD_API_BEGIN
     BATCH_OR_FAIL(b);
     bool ok = tx == 0 && ty == 0;
 #define MPD_TILE(X, Y) ok = ok || (tx == X && ty == Y && zc >= 1);
@@ -464,53 +477,65 @@ extern "C" int mpd_batch_set_limiter_tile(mpd_batch b, int tx, int ty, int zc)
         g->zc2 = 0;
     }
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_batch_set_path(mpd_batch b, int path)
 {
+    MPD_API_BEGIN
     BATCH_OR_FAIL(b);
     if (path != 0 && path != 2 && path != 3)
         return fail(MPD_ERR_ARG, "mpd_batch_set_path: %d is not 0, 2 or 3", path);
     b->path = path;
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_batch_get_path(mpd_batch b, int* path)
 {
+    MPD_API_BEGIN
     BATCH_OR_FAIL(b);
     if (!path) return fail(MPD_ERR_ARG, "mpd_batch_get_path: NULL");
     *path = one_kernel(b) ? 3 : 2;
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_batch_set_pass_zchunk(mpd_batch b, int zc)
 {
+    MPD_API_BEGIN
     BATCH_OR_FAIL(b);
     if (zc < 0) return fail(MPD_ERR_ARG, "mpd_batch_set_pass_zchunk: negative");
     b->zc3 = zc;
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_batch_get_tiles(mpd_batch b, int* v)
 {
+    MPD_API_BEGIN
     BATCH_OR_FAIL(b);
     if (!v) return fail(MPD_ERR_ARG, "mpd_batch_get_tiles: NULL array");
     const int t[7] = {b->tx, b->ty, b->zc, b->persistent, b->tx2, b->ty2, b->zc2};
     for (int i = 0; i < 7; i++) v[i] = t[i];
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_batch_step(mpd_batch b, int nsteps, int mode)
 {
+    MPD_API_BEGIN
     BATCH_OR_FAIL(b);
     if (nsteps < 0 || mode < MPD_BATCH_PER_INSTANCE || mode > MPD_BATCH_GRAPH)
         return fail(MPD_ERR_ARG, "mpd_batch_step: bad nsteps or mode");
     if (nsteps == 0) return MPD_OK;
     return run(b, nsteps, mode);
+    MPD_API_END
 }
 
 extern "C" int mpd_batch_step_timed(mpd_batch b, int nsteps, int mode, double* ms)
 {
+    MPD_API_BEGIN
     BATCH_OR_FAIL(b);
     if (!ms || nsteps < 1) return fail(MPD_ERR_ARG, "mpd_batch_step_timed: bad arguments");
     int rc;
@@ -549,6 +574,7 @@ extern "C" int mpd_batch_step_timed(mpd_batch b, int nsteps, int mode, double* m
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return rc;
+    MPD_API_END
 }
 
 namespace {
@@ -597,6 +623,7 @@ int restore_fields(mpd_batch_s* b, const std::vector<DevBuf>& save, int start)
 extern "C" int mpd_batch_autotune(mpd_batch b, int nsteps, int* tx, int* ty, int* zc,
                                   int* persistent)
 {
+    MPD_API_BEGIN
     BATCH_OR_FAIL(b);
     if (nsteps < 1) return fail(MPD_ERR_ARG, "mpd_batch_autotune: nsteps < 1");
     std::string opts = "i" + std::to_string(b->iord) + "n" + std::to_string(b->nonosc) + "a" +
@@ -701,4 +728,5 @@ extern "C" int mpd_batch_autotune(mpd_batch b, int nsteps, int* tx, int* ty, int
     if (persistent) *persistent = c[3];
     b->tuned = true;
     return MPD_OK;
+    MPD_API_END
 }

@@ -144,9 +144,11 @@ extern "C" const char* mpd_error_string(void) { return mpd::g_err.c_str(); }
 
 extern "C" int mpd_device_count(int* n)
 {
+    MPD_API_BEGIN
     if (!n) return mpd::fail(MPD_ERR_ARG, "mpd_device_count: NULL");
     int c = 0;
     if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
     *n = c;
     return MPD_OK;
+    MPD_API_END
 }

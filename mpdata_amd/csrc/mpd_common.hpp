@@ -1,6 +1,7 @@
 // Host-side helpers shared by the library's translation units: error reporting,
 // the per-device launch stream and a checked-allocation helper.
 #pragma once
+#include <new>
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
@@ -17,6 +18,20 @@ void clear_error();
 
 // HIP status -> MPD code with a message naming the call site.
 int hip_fail(hipError_t e, const char* what);
+
+// Bodies of the C ABI's entry points: a C++ exception (a failed host allocation) never
+// crosses into a C caller, it becomes an error code and message.
+#define MPD_API_BEGIN try {
+#define MPD_API_END                                                                        \
+    }                                                                                      \
+    catch (const std::bad_alloc&)                                                          \
+    {                                                                                      \
+        return ::mpd::fail(MPD_ERR_NOMEM, "host memory allocation failed");                \
+    }                                                                                      \
+    catch (...)                                                                            \
+    {                                                                                      \
+        return ::mpd::fail(MPD_ERR_DEVICE, "unexpected C++ exception in the library");    \
+    }
 
 #define MPD_HIP(call)                                                   \
     do {                                                                \

@@ -113,6 +113,7 @@ unsigned nblk(size_t n) { return (unsigned)((n + TPB - 1) / TPB); }
 
 extern "C" int mpd_generate_wind(mpd_grid g, const mpd_wind_spec* w)
 {
+    MPD_API_BEGIN
     clear_error();
     if (!g || !w) return fail(MPD_ERR_ARG, "mpd_generate_wind: NULL argument");
     if (g->boundary != MPD_PERIODIC)
@@ -167,11 +168,13 @@ extern "C" int mpd_generate_wind(mpd_grid g, const mpd_wind_spec* w)
     MPD_HIP(hipStreamSynchronize(st));
     g->has_winds = true;
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int mpd_generate_blobs(mpd_grid g, int target, int nblobs, const double* centres,
                                   const double* sigma, const double* amp, int periodic)
 {
+    MPD_API_BEGIN
     clear_error();
     if (!g || !centres || !sigma || !amp || nblobs < 1 || nblobs > MPD_MAX_BLOBS)
         return fail(MPD_ERR_ARG, "mpd_generate_blobs: bad arguments (1..%d blobs)", MPD_MAX_BLOBS);
@@ -205,4 +208,5 @@ extern "C" int mpd_generate_blobs(mpd_grid g, int target, int nblobs, const doub
     else
         g->has_field = true;
     return MPD_OK;
+    MPD_API_END
 }

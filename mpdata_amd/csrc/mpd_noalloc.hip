@@ -84,6 +84,7 @@ extern "C" int mpd_step_periodic_noalloc(int nx, int ny, int nz, int iord, int n
                                          const double* q, const double* cin, double* cout,
                                          void* stream)
 {
+    MPD_API_BEGIN
     clear_error();
     if (nx < 1 || ny < 1 || nz < 1)
         return fail(MPD_ERR_ARG, "mpd_step_periodic_noalloc: grid %dx%dx%d", nx, ny, nz);
@@ -140,4 +141,5 @@ extern "C" int mpd_step_periodic_noalloc(int nx, int ny, int nz, int iord, int n
     g.caller_stream_set = true;
     g.caller_stream = static_cast<hipStream_t>(stream);
     return periodic_fused_step_fast(&g);
+    MPD_API_END
 }

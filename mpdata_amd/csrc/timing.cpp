@@ -57,6 +57,7 @@ struct smolar_device_timer_s {
 
 extern "C" int smolar_device_timer_create(smolar_device_timer* t)
 {
+    MPD_API_BEGIN
     if (!t) return mpd::fail(MPD_ERR_ARG, "smolar_device_timer_create: NULL");
     int rc = mpd::ensure_device();
     if (rc) return rc;
@@ -67,38 +68,47 @@ extern "C" int smolar_device_timer_create(smolar_device_timer* t)
     }
     *t = x;
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int smolar_device_timer_start(smolar_device_timer t)
 {
+    MPD_API_BEGIN
     if (!t) return mpd::fail(MPD_ERR_ARG, "NULL timer");
     MPD_HIP(hipEventRecord(t->a, mpd::stream()));
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int smolar_device_timer_stop(smolar_device_timer t)
 {
+    MPD_API_BEGIN
     if (!t) return mpd::fail(MPD_ERR_ARG, "NULL timer");
     MPD_HIP(hipEventRecord(t->b, mpd::stream()));
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int smolar_device_timer_elapsed_ms(smolar_device_timer t, double* ms)
 {
+    MPD_API_BEGIN
     if (!t || !ms) return mpd::fail(MPD_ERR_ARG, "NULL timer or output");
     MPD_HIP(hipEventSynchronize(t->b));
     float f = 0.f;
     MPD_HIP(hipEventElapsedTime(&f, t->a, t->b));
     *ms = f;
     return MPD_OK;
+    MPD_API_END
 }
 
 extern "C" int smolar_device_timer_destroy(smolar_device_timer t)
 {
+    MPD_API_BEGIN
     if (t) {
         (void)hipEventDestroy(t->a);
         (void)hipEventDestroy(t->b);
         delete t;
     }
     return MPD_OK;
+    MPD_API_END
 }
