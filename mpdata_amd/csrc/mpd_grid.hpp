@@ -101,7 +101,8 @@ void trace_mark(mpd_grid_s* g);
 // 2, against two kernels per pass: 64^3 0.021 / 0.028 ms, 128^3 0.058 / 0.065, 192^3
 // 0.180 / 0.238, 256^3 0.36 / 0.52 ms per step; 512x512x256 iord 3 3.09 / 4.12 ms; but
 // a chunk streams five planes beyond its own, and on thin grids that loses: 128x128x1
-// 0.017 / 0.010 ms, 512x512x4 0.053 / 0.045, 128x128x8 0.020 / 0.027).
+// 0.017 / 0.010 ms, 512x512x4 0.053 / 0.045, 128x128x8 0.020 / 0.027).  Grids of one
+// plane take the kernel's single-plane form instead (plane_kernel_passes).
 inline bool one_kernel_passes(const mpd_grid_s* g)
 {
     return g->nonosc && (g->path == 3 || (g->path == 0 && g->nz >= 8));

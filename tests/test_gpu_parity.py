@@ -185,7 +185,8 @@ PER_CASES = [(16, 16, 16), (24, 20, 1), (13, 11, 7), (64, 8, 5), (1, 9, 10), (5,
 
 
 # (path, arith): staged exact, fused exact, fused fast, automatic (one kernel per pass
-# for the nonoscillatory grids of at least 8 planes, its z-chunk from the cost model)
+# for the nonoscillatory grids of at least 8 planes, its z-chunk from the cost model, and
+# its single-plane form for those of one plane)
 PATHS = [(1, M.ARITH_EXACT), (2, M.ARITH_EXACT), (2, M.ARITH_FAST), (0, M.ARITH_EXACT),
          (0, M.ARITH_FAST)]
 
