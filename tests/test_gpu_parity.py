@@ -345,15 +345,16 @@ def test_zchunk_lengths_match_oracle(zc, iord):
     g.close()
 
 
+@pytest.mark.parametrize("dims", [(33, 17, 9), (33, 17, 1)])
 @pytest.mark.parametrize("path", [0, 2])
 @pytest.mark.parametrize("iord,nonosc", [(2, True), (2, False), (3, True)])
-def test_graph_replays_match_eager_steps(iord, nonosc, path):
+def test_graph_replays_match_eager_steps(iord, nonosc, path, dims):
     """mpd_step replays blocks of 8 steps as captured graphs; the same steps launched one
     by one give identical fields -- over several replays with eager work (field reads)
     between them, odd step counts, a tile change that forces a new capture, and an
-    emission array; on the automatic path (one kernel per pass with the limiter) and on
-    two kernels per pass."""
-    nx, ny, nz = 33, 17, 9
+    emission array; on the automatic path (one kernel per pass with the limiter, in its
+    single-plane form on the plane) and on two kernels per pass."""
+    nx, ny, nz = dims
     U, V, W, c = per_case(nx, ny, nz, 12)
     q = np.random.default_rng(3).uniform(0.0, 1e-3, c.size)
     for arith in (M.ARITH_EXACT, M.ARITH_FAST):
