@@ -678,7 +678,7 @@ int choose_zc_f12(const mpd_grid_s* g)
 // Grids of one z-plane (nz = 1): every z-neighbour of a cell is the cell itself (the
 // periodic wrap of one plane), and every "plane below / above" of the one-kernel pass is
 // the same plane.  F12's plane pipeline, which streams five planes beyond a chunk to
-// keep c1, A and the ratios of three planes in flight, collapses into four
+// keep c1, A and the ratios of three planes in flight, collapses into five
 // barrier-separated stages over the one plane: load c^n, U, V, W (and the pass input) of
 // the tile and its three-cell ring; c1 on rings 0-2; the faces; the ratios on rings 0-1;
 // the limited donor pass on the tile.  The formulas are F12's, with the z-neighbours of a

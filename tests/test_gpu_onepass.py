@@ -124,7 +124,7 @@ PLANES = [(128, 128), (33, 17), (7, 5), (1, 9), (16, 16), (40, 3), (17, 1), (256
 @pytest.mark.parametrize("arith", [M.ARITH_EXACT, M.ARITH_FAST])
 def test_single_plane_pass_matches_oracle(dims, arith):
     """Grids of one z-plane take the single-plane form of the one-kernel pass (path 4,
-    automatic there): four stages over the plane, every z-neighbour the cell itself.
+    automatic there): five stages over the plane, every z-neighbour the cell itself.
     Tiles of 8 x 8 or 16 x 16 with a three-cell ring, so the cases cover planes narrower
     than the ring in either axis and extents that are not multiples of the tile.  Exact arithmetic
     bit for bit equal to the oracle over three steps, fast within 1e-12 per step; the
