@@ -200,11 +200,10 @@ __device__ __forceinline__ void f1_tile(const FusedArgs& a, int item)
                 // the divergence term and the ratio terms (the same sums, reassociated)
                 const double v1 = vr + vcc, v2 = vru + vu, w1 = wr + wc, w2 = wrf + wf;
                 const double b1 = cru + cu, b2 = cd + crd, e1 = crf + cf, e2 = cb + crb;
-                double sv = 0.25 * (v1 + v2);
-                double sw = 0.25 * (w1 + w2);
-                double div = 0.25 * av * ((uk[L + 1] - uk[L - 1]) + (v1 - v2) + (w1 - w2));
+                const double D = (uk[L + 1] - uk[L - 1]) + (v1 - v2) + (w1 - w2);
                 put<NT>(a.Ax + zm1.off, colb,
-                   antidiff(div, av, cr, c, sv, b1 - b2, b1 + b2, sw, e1 - e2, e1 + e2));
+                        antidiff_s(D, av, cr, c, v1 + v2, b1 - b2, b1 + b2, w1 + w2, e1 - e2,
+                                   e1 + e2));
 #else
                 double sv = 0.25 * (vr + vru + vu + vcc);
                 double sw = 0.25 * (wr + wrf + wf + wc);
@@ -227,11 +226,10 @@ __device__ __forceinline__ void f1_tile(const FusedArgs& a, int item)
                 // (the divergence term's x strokes are shifted one cell left: uu, ucc shared)
                 const double u1 = uu + ucc, w1 = wu + wc, w2 = wuf + wf;
                 const double b1 = cru + cr, b2 = cl + clu, e1 = cuf + cf, e2 = cb + cub;
-                double su = 0.25 * (u1 + (uru + ur));
-                double sw = 0.25 * (w1 + w2);
-                double div = 0.25 * av * ((vk[L + RX] - vk[L - RX]) + (w1 - w2) + (u1 - (ulu + ul)));
+                const double D = (vk[L + RX] - vk[L - RX]) + (w1 - w2) + (u1 - (ulu + ul));
                 put<NT>(a.Ay + zm1.off, colb,
-                   antidiff(div, av, cu, c, su, b1 - b2, b1 + b2, sw, e1 - e2, e1 + e2));
+                        antidiff_s(D, av, cu, c, u1 + (uru + ur), b1 - b2, b1 + b2, w1 + w2,
+                                   e1 - e2, e1 + e2));
 #else
                 double su = 0.25 * (uu + uru + ur + ucc);
                 double sw = 0.25 * (wu + wuf + wf + wc);
@@ -252,11 +250,10 @@ __device__ __forceinline__ void f1_tile(const FusedArgs& a, int item)
 #ifdef MPD_FAST_DIV
                 const double u1 = uf + ucc, u2 = urf + ur, v1 = vf + vcc, v2 = vuf + vu;
                 const double b1 = crf + cr, b2 = cl + clf, e1 = cuf + cu, e2 = cd + cdf;
-                double su = 0.25 * (u1 + u2);
-                double sv = 0.25 * (v1 + v2);
-                double div = 0.25 * av * ((wn - wmm) + (u1 - u2) + (v1 - v2));
+                const double D = (wn - wmm) + (u1 - u2) + (v1 - v2);
                 put<NT>(a.Az + zm1.off, colb,
-                   antidiff(div, av, cf, c, su, b1 - b2, b1 + b2, sv, e1 - e2, e1 + e2));
+                        antidiff_s(D, av, cf, c, u1 + u2, b1 - b2, b1 + b2, v1 + v2, e1 - e2,
+                                   e1 + e2));
 #else
                 double su = 0.25 * (ur + urf + uf + ucc);
                 double sv = 0.25 * (vf + vuf + vu + vcc);
@@ -472,10 +469,10 @@ __device__ __forceinline__ void f2_tile(const FusedArgs& a, int item)
                     *code = hc | (lc << 8);
                 }
             }
-            const double fxl = 0.5 * F2(nbp[1], c, axl), fxr = 0.5 * F2(c, nbp[2], axr);
-            const double fyl = 0.5 * F2(nbp[3], c, ayl), fyr = 0.5 * F2(c, nbp[4], ayr);
-            const double fzl = 0.5 * F2(cplo, c, azlo), fzr = 0.5 * F2(c, cphi, azhi);
-            beta(hi, lo, c, fxl, fxr, fyl, fyr, fzl, fzr, bu, bd);
+            const double fxl = F2(nbp[1], c, axl), fxr = F2(c, nbp[2], axr);
+            const double fyl = F2(nbp[3], c, ayl), fyr = F2(c, nbp[4], ayr);
+            const double fzl = F2(cplo, c, azlo), fzr = F2(c, cphi, azhi);
+            beta2(hi, lo, c, fxl, fxr, fyl, fyr, fzl, fzr, bu, bd);
             if constexpr (!SAVE) {  // clamped once here (SAVE records the ratios themselves)
                 bu = mn2(bu, 1.0);
                 bd = mn2(bd, 1.0);

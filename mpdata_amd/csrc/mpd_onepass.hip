@@ -162,8 +162,8 @@ struct Col3 {
     double axl_m3 = 0, ayd_m3 = 0;    // A_x of the x-, A_y of the y- neighbour, plane p-3
     double bu_m3 = 0, bd_m3 = 0;                                         // own ratios
     // z faces formed once per face, as the upper face of a plane, and carried to the
-    // plane above as its lower face: the donor field's doubled flux (F2), the ratios'
-    // unlimited flux (0.5 F2) and the limited donor pass's doubled flux
+    // plane above as its lower face: the doubled donor fluxes (F2) of the donor field, of
+    // the ratios (unlimited faces) and of the limited donor pass
     double fc_z = 0, fr_z = 0, fl_z = 0;
 };
 
@@ -354,10 +354,9 @@ __device__ __forceinline__ void f12_part(const FusedArgs& a, int item, int lane,
 #ifdef MPD_FAST_DIV
                 const double v1 = vr + vcc, v2 = vru + vu, w1 = wr_ + wc, w2 = wrf + wf;
                 const double b1 = cru + cu, b2 = cd + crd, e1 = crf + cf, e2 = cb + crb;
-                double sv = 0.25 * (v1 + v2);
-                double sw = 0.25 * (w1 + w2);
-                double div = 0.25 * av * ((uk[M + 1] - uk[M - 1]) + (v1 - v2) + (w1 - w2));
-                ox = antidiff(div, av, cr, c, sv, b1 - b2, b1 + b2, sw, e1 - e2, e1 + e2);
+                const double D = (uk[M + 1] - uk[M - 1]) + (v1 - v2) + (w1 - w2);
+                ox = antidiff_s(D, av, cr, c, v1 + v2, b1 - b2, b1 + b2, w1 + w2, e1 - e2,
+                                e1 + e2);
 #else
                 double sv = 0.25 * (vr + vru + vu + vcc);
                 double sw = 0.25 * (wr_ + wrf + wf + wc);
@@ -378,11 +377,9 @@ __device__ __forceinline__ void f12_part(const FusedArgs& a, int item, int lane,
 #ifdef MPD_FAST_DIV
                 const double u1 = uu + ucc, w1 = wu_ + wc, w2 = wuf + wf;
                 const double b1 = cru + cr, b2 = cl + clu, e1 = cuf + cf, e2 = cb + cub;
-                double su = 0.25 * (u1 + (uru + ur));
-                double sw = 0.25 * (w1 + w2);
-                double div =
-                    0.25 * av * ((vk[M + RX] - vk[M - RX]) + (w1 - w2) + (u1 - (ulu + ul)));
-                oy = antidiff(div, av, cu, c, su, b1 - b2, b1 + b2, sw, e1 - e2, e1 + e2);
+                const double D = (vk[M + RX] - vk[M - RX]) + (w1 - w2) + (u1 - (ulu + ul));
+                oy = antidiff_s(D, av, cu, c, u1 + (uru + ur), b1 - b2, b1 + b2, w1 + w2,
+                                e1 - e2, e1 + e2);
 #else
                 double su = 0.25 * (uu + uru + ur + ucc);
                 double sw = 0.25 * (wu_ + wuf + wf + wc);
@@ -402,10 +399,9 @@ __device__ __forceinline__ void f12_part(const FusedArgs& a, int item, int lane,
 #ifdef MPD_FAST_DIV
                 const double u1 = uf + ucc, u2 = urf + ur, v1 = vf + vcc, v2 = vuf + vu;
                 const double b1 = crf + cr, b2 = cl + clf, e1 = cuf + cu, e2 = cd + cdf;
-                double su = 0.25 * (u1 + u2);
-                double sv = 0.25 * (v1 + v2);
-                double div = 0.25 * av * ((C.w_0 - C.w_m2) + (u1 - u2) + (v1 - v2));
-                oz = antidiff(div, av, cf, c, su, b1 - b2, b1 + b2, sv, e1 - e2, e1 + e2);
+                const double D = (C.w_0 - C.w_m2) + (u1 - u2) + (v1 - v2);
+                oz = antidiff_s(D, av, cf, c, u1 + u2, b1 - b2, b1 + b2, v1 + v2, e1 - e2,
+                                e1 + e2);
 #else
                 double su = 0.25 * (ur + urf + uf + ucc);
                 double sv = 0.25 * (vf + vuf + vu + vcc);
@@ -438,10 +434,10 @@ __device__ __forceinline__ void f12_part(const FusedArgs& a, int item, int lane,
                         mn2(C.cn_m1, C.c1_m1)));
             axl = axm[M - 1];
             ayd = aym[M - RX];
-            const double fxl = 0.5 * F2(xl, c, axl), fxr = 0.5 * F2(c, xr, C.ax_m2);
-            const double fyl = 0.5 * F2(yd, c, ayd), fyr = 0.5 * F2(c, yu, C.ay_m2);
-            const double fzl = C.fr_z, fzr = 0.5 * F2(c, C.c1_m1, C.az_m2);
-            beta(hi, lo, c, fxl, fxr, fyl, fyr, fzl, fzr, bu, bd);
+            const double fxl = F2(xl, c, axl), fxr = F2(c, xr, C.ax_m2);
+            const double fyl = F2(yd, c, ayd), fyr = F2(c, yu, C.ay_m2);
+            const double fzl = C.fr_z, fzr = F2(c, C.c1_m1, C.az_m2);
+            beta2(hi, lo, c, fxl, fxr, fyl, fyr, fzl, fzr, bu, bd);
             fzr_ = fzr;
             if constexpr (SAVE && TILE) {
                 if (store && m >= z0 && m < z1) {
@@ -492,7 +488,7 @@ __device__ __forceinline__ void f12_part(const FusedArgs& a, int item, int lane,
             sm.bd[ma][L] = bd_m;
         } else if (!ST && r1 && m == z0 - 2) {
             // the upper z face of plane z0 - 2: the lower face of the first ratio plane
-            T.fr_z = 0.5 * F2(T.c1_m2, T.c1_m1, T.az_m2);
+            T.fr_z = F2(T.c1_m2, T.c1_m1, T.az_m2);
         }
         // limited donor pass of plane o on the tile
         if constexpr (TILE) {
@@ -746,10 +742,9 @@ __global__ void __launch_bounds__(P2_NTH) k_f12_plane(FusedArgs a)
 #ifdef MPD_FAST_DIV
                 const double v1 = vr + vcc, v2 = vru + vu, w1 = wr_ + wc, w2 = wrf + wf;
                 const double b1 = cru + cu, b2 = cd + crd, e1 = crf + cf, e2 = cb + crb;
-                const double sv = 0.25 * (v1 + v2);
-                const double sw = 0.25 * (w1 + w2);
-                const double div = 0.25 * av * ((uk[M + 1] - uk[M - 1]) + (v1 - v2) + (w1 - w2));
-                sm.ax[M] = antidiff(div, av, cr, c, sv, b1 - b2, b1 + b2, sw, e1 - e2, e1 + e2);
+                const double D = (uk[M + 1] - uk[M - 1]) + (v1 - v2) + (w1 - w2);
+                sm.ax[M] = antidiff_s(D, av, cr, c, v1 + v2, b1 - b2, b1 + b2, w1 + w2, e1 - e2,
+                                      e1 + e2);
 #else
                 const double sv = 0.25 * (vr + vru + vu + vcc);
                 const double sw = 0.25 * (wr_ + wrf + wf + wc);
@@ -770,11 +765,9 @@ __global__ void __launch_bounds__(P2_NTH) k_f12_plane(FusedArgs a)
 #ifdef MPD_FAST_DIV
                 const double u1 = uu + ucc, w1 = wu_ + wc, w2 = wuf + wf;
                 const double b1 = cru + cr, b2 = cl + clu, e1 = cuf + cf, e2 = cb + cub;
-                const double su = 0.25 * (u1 + (uru + ur));
-                const double sw = 0.25 * (w1 + w2);
-                const double div =
-                    0.25 * av * ((vk[M + RX] - vk[M - RX]) + (w1 - w2) + (u1 - (ulu + ul)));
-                sm.ay[M] = antidiff(div, av, cu, c, su, b1 - b2, b1 + b2, sw, e1 - e2, e1 + e2);
+                const double D = (vk[M + RX] - vk[M - RX]) + (w1 - w2) + (u1 - (ulu + ul));
+                sm.ay[M] = antidiff_s(D, av, cu, c, u1 + (uru + ur), b1 - b2, b1 + b2, w1 + w2,
+                                      e1 - e2, e1 + e2);
 #else
                 const double su = 0.25 * (uu + uru + ur + ucc);
                 const double sw = 0.25 * (wu_ + wuf + wf + wc);
@@ -793,10 +786,9 @@ __global__ void __launch_bounds__(P2_NTH) k_f12_plane(FusedArgs a)
 #ifdef MPD_FAST_DIV
                 const double u1 = uf + ucc, u2 = urf + ur, v1 = vf + vcc, v2 = vuf + vu;
                 const double b1 = crf + cr, b2 = cl + clf, e1 = cuf + cu, e2 = cd + cdf;
-                const double su = 0.25 * (u1 + u2);
-                const double sv = 0.25 * (v1 + v2);
-                const double div = 0.25 * av * ((av - av) + (u1 - u2) + (v1 - v2));
-                sm.az[M] = antidiff(div, av, cf, c, su, b1 - b2, b1 + b2, sv, e1 - e2, e1 + e2);
+                const double D = (av - av) + (u1 - u2) + (v1 - v2);
+                sm.az[M] = antidiff_s(D, av, cf, c, u1 + u2, b1 - b2, b1 + b2, v1 + v2, e1 - e2,
+                                      e1 + e2);
 #else
                 const double su = 0.25 * (ur + urf + uf + ucc);
                 const double sv = 0.25 * (vf + vuf + vu + vcc);
@@ -826,11 +818,11 @@ __global__ void __launch_bounds__(P2_NTH) k_f12_plane(FusedArgs a)
                         mn2(mn2(sm.cn[M + 1], xr), mn2(sm.cn[M - RX], yd))),
                     mn2(mn2(mn2(sm.cn[M + RX], yu), mn2(cn, c)), mn2(cn, c)));
             const double az = sm.az[M];
-            const double fxl = 0.5 * F2(xl, c, sm.ax[M - 1]), fxr = 0.5 * F2(c, xr, sm.ax[M]);
-            const double fyl = 0.5 * F2(yd, c, sm.ay[M - RX]), fyr = 0.5 * F2(c, yu, sm.ay[M]);
-            const double fzl = 0.5 * F2(c, c, az), fzr = fzl;
+            const double fxl = F2(xl, c, sm.ax[M - 1]), fxr = F2(c, xr, sm.ax[M]);
+            const double fyl = F2(yd, c, sm.ay[M - RX]), fyr = F2(c, yu, sm.ay[M]);
+            const double fzl = F2(c, c, az), fzr = fzl;
             double bu, bd;
-            beta(hi, lo, c, fxl, fxr, fyl, fyr, fzl, fzr, bu, bd);
+            beta2(hi, lo, c, fxl, fxr, fyl, fyr, fzl, fzr, bu, bd);
             sm.bu[M] = mn2(bu, 1.0);
             sm.bd[M] = mn2(bd, 1.0);
         }
