@@ -97,12 +97,13 @@ __device__ __forceinline__ double antidiff(double div, double a, double p, doubl
 // divergence sum D (div = a D / 4) and the unscaled transverse sums X, Y (s1 = X / 4,
 // s2 = Y / 4) it forms r - a (2 D + X n1 / d1 + Y n2 / d2) / 8, the same expression: the
 // factors are powers of two (exact scalings), so only the roundings of the fused steps
-// differ, and four operations of each face go.
+// differ, and five operations of each face go (|a| (1 - |a|) is one fused step,
+// |a| - |a| |a|).
 __device__ __forceinline__ double antidiff_s(double D, double a, double p, double m, double X,
                                              double n1, double d1, double Y, double n2,
                                              double d2)
 {
-    double r = dv((ab(a) * (1.0 - ab(a))) * (p - m), p + m + MINV);
+    double r = dv(__builtin_fma(-ab(a), ab(a), ab(a)) * (p - m), p + m + MINV);
     double s = dv(X * n1, d1 + MINV) + dv(Y * n2, d2 + MINV);
     return __builtin_fma(-0.125 * a, __builtin_fma(2.0, D, s), r);
 }
@@ -166,7 +167,7 @@ __device__ __forceinline__ void beta(double hi, double lo, double c, double fxl,
 // beta() from the doubled donor fluxes F2 of the six faces.  The exact build halves each
 // flux first, as beta()'s callers do (the same operations, so the same bits); the fast
 // build folds the six halvings into the two sums (a power-of-two scaling, exact: the
-// same values as halving first, five multiplications fewer).
+// same values as halving first, seven operations fewer).
 __device__ __forceinline__ void beta2(double hi, double lo, double c, double Fxl, double Fxr,
                                       double Fyl, double Fyr, double Fzl, double Fzr,
                                       double& bup, double& bdn)
@@ -174,9 +175,12 @@ __device__ __forceinline__ void beta2(double hi, double lo, double c, double Fxl
 #ifdef MPD_FAST_DIV
     const double sa = ((ab(Fxl) + ab(Fxr)) + (ab(Fyl) + ab(Fyr))) + (ab(Fzl) + ab(Fzr));
     const double sd = ((Fxl - Fxr) + (Fyl - Fyr)) + (Fzl - Fzr);
-    const double ain = pos(0.25 * (sa + sd)), aout = pos(0.25 * (sa - sd));
-    bup = dv(hi - c, ain + MINV);
-    bdn = dv(c - lo, aout + MINV);
+    // pos(x / 4) + MINV as max(x / 4 + MINV, MINV), the scaling and the guard in one
+    // fused step (x / 4 is exact, so the sums are those of the unfused form)
+    const double dn_in = mx2(__builtin_fma(0.25, sa + sd, MINV), MINV);
+    const double dn_out = mx2(__builtin_fma(0.25, sa - sd, MINV), MINV);
+    bup = dv(hi - c, dn_in);
+    bdn = dv(c - lo, dn_out);
 #else
     beta(hi, lo, c, 0.5 * Fxl, 0.5 * Fxr, 0.5 * Fyl, 0.5 * Fyr, 0.5 * Fzl, 0.5 * Fzr, bup, bdn);
 #endif
