@@ -66,12 +66,12 @@ def test_curl_generator_is_divergence_free_and_scaled():
 DIMS = [(24, 20, 16), (16, 16, 16), (33, 17, 9), (40, 8, 12), (8, 8, 8)]
 
 
-def make_batch(iord, arith, seed=7):
-    b = M.Batch(DIMS, iord=iord, nonosc=True)
+def make_batch(iord, arith, seed=7, dims=DIMS):
+    b = M.Batch(dims, iord=iord, nonosc=True)
     b.set_arith(arith)
     rng = np.random.default_rng(seed)
     init = []
-    for g, (nx, ny, nz) in zip(b.instances, DIMS):
+    for g, (nx, ny, nz) in zip(b.instances, dims):
         n = nx * ny * nz
         U, V, W = (rng.uniform(-0.2, 0.2, n) for _ in range(3))
         c = rng.uniform(0, 1, n) ** 3
@@ -118,22 +118,28 @@ def test_batch_modes_agree_and_match_oracle(iord, arith):
             assert relerr(got, want) <= 1e-12
 
 
+# thin instances: one z-plane (stepped alone on the single-plane form, batched on the
+# streaming kernel over a column of one plane) and two planes
+THIN = [(24, 20, 16), (20, 12, 1), (9, 7, 2), (8, 8, 8)]
+
+
+@pytest.mark.parametrize("dims", [DIMS, THIN], ids=["dims", "thin"])
 @pytest.mark.parametrize("iord", [2, 3])
 @pytest.mark.parametrize("arith", [M.ARITH_EXACT, M.ARITH_FAST])
-def test_one_kernel_batched_launches(iord, arith):
+def test_one_kernel_batched_launches(iord, arith, dims):
     """Batched launches with one kernel per corrective pass (forced on a small batch):
     fused launches and graph replays, several z-chunks and emission on some instances,
     against per-instance launches and the oracle -- bit for bit in exact arithmetic,
     1e-12 in fast arithmetic (the two paths contract differently)."""
     rng = np.random.default_rng(iord)
     qs = [rng.uniform(0, 1e-3, nx * ny * nz) if i % 2 else None
-          for i, (nx, ny, nz) in enumerate(DIMS)]
+          for i, (nx, ny, nz) in enumerate(dims)]
     results = []
     for mode, zc in ((M.BATCH_PER_INSTANCE, 0), (M.BATCH_FUSED, 0), (M.BATCH_FUSED, 3),
                      (M.BATCH_GRAPH, 5), (M.BATCH_GRAPH, 0)):
         print(f"case one-kernel batch iord={iord} arith={arith} mode={mode} zc={zc}",
               flush=True)
-        b, init = make_batch(iord, arith)
+        b, init = make_batch(iord, arith, dims=dims)
         for g, q in zip(b.instances, qs):
             if q is not None:
                 g.set_emission(q)
@@ -154,8 +160,8 @@ def test_one_kernel_batched_launches(iord, arith):
                 np.testing.assert_array_equal(a, c)
             else:
                 assert relerr(a, c) <= 1e-12
-    _, init = make_batch(iord, arith)
-    for (nx, ny, nz), (U, V, W, c), q, got in zip(DIMS, init, qs, results[1]):
+    _, init = make_batch(iord, arith, dims=dims)
+    for (nx, ny, nz), (U, V, W, c), q, got in zip(dims, init, qs, results[1]):
         want = O.per_run(nx, ny, nz, U, V, W, iord, True, c, 7, q)
         if arith == M.ARITH_EXACT:
             np.testing.assert_array_equal(got, want)
