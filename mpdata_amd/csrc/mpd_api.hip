@@ -4,6 +4,7 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <thread>
@@ -91,10 +92,12 @@ int periodic_step(mpd_grid_s* g)
 // them, the host copying one (on four threads) while the DMA engine moves the other.
 // Measured on the MI355X host link with 64 MiB: pageable 28-30 GB/s to the device and
 // 17 GB/s back; staged 44 GB/s either way (one host thread: 27-29; pinned DMA alone 56).
-// The buffers are per thread (the drop-in smolar() is reentrant per thread), allocated
-// on the first large copy and kept until the process ends.  h2d returns once the host
-// array has been read (its DMA may still be in flight); d2h returns with the host array
-// written.
+// A copy leases a buffer pair from a pool per device (the drop-in smolar() is reentrant
+// per thread: concurrent copies each hold a pair of their own); a pair is allocated when
+// no free one is left and kept until the process ends, so the pinned memory is bounded
+// by the number of copies that ran at the same time, not by the threads that ever
+// copied.  h2d returns once the host array has been read (its DMA may still be in
+// flight: the pair's next user waits for it); d2h returns with the host array written.
 namespace {
 // (below 8 MiB the direct pageable copy was as fast: 64^3 smolar() 0.63 ms per call direct,
 // 0.78 staged)
@@ -126,6 +129,7 @@ void host_copy(char* dst, const char* src, size_t len)
 }
 
 struct Staging {
+    Staging* next = nullptr;  // free list of the pool
     char* buf[2] = {nullptr, nullptr};
     hipEvent_t ev[2] = {nullptr, nullptr};
     bool busy[2] = {false, false};  // an enqueued DMA may still use the buffer
@@ -153,7 +157,41 @@ struct Staging {
         return MPD_OK;
     }
 };
-thread_local Staging t_stage;
+// Free pairs per device (events are recorded on the device's stream, so a pair serves
+// one device); released without allocating.
+constexpr int kStageDevices = 64;
+std::mutex g_stage_mu;
+Staging* g_stage_free[kStageDevices] = {};
+
+struct StageLease {
+    Staging* s = nullptr;
+    int dev = 0;
+    // a pair of the current device, buffers allocated; nullptr-safe on failure
+    int acquire()
+    {
+        MPD_HIP(hipGetDevice(&dev));
+        if (dev < 0 || dev >= kStageDevices)
+            return fail(MPD_ERR_UNSUPPORTED, "staged copy: device %d beyond %d", dev,
+                        kStageDevices);
+        {
+            std::lock_guard<std::mutex> lk(g_stage_mu);
+            s = g_stage_free[dev];
+            if (s) g_stage_free[dev] = s->next;
+        }
+        if (!s) s = new Staging();
+        return s->init();
+    }
+    ~StageLease()
+    {
+        if (!s) return;
+        std::lock_guard<std::mutex> lk(g_stage_mu);
+        s->next = g_stage_free[dev];
+        g_stage_free[dev] = s;
+    }
+    StageLease() = default;
+    StageLease(const StageLease&) = delete;
+    StageLease& operator=(const StageLease&) = delete;
+};
 }  // namespace
 
 static int h2d(double* d, const double* h, size_t n)
@@ -165,16 +203,17 @@ static int h2d(double* d, const double* h, size_t n)
         return MPD_OK;
     }
     int rc;
-    if ((rc = t_stage.init())) return rc;
+    StageLease lease;
+    if ((rc = lease.acquire())) return rc;
+    Staging& st = *lease.s;
     const char* src = (const char*)h;
     char* dst = (char*)d;
     for (size_t off = 0, k = 0; off < bytes; off += kStageChunk, k++) {
         const int i = (int)(k & 1);
         const size_t len = std::min(kStageChunk, bytes - off);
-        if ((rc = t_stage.settle(i))) return rc;
-        host_copy(t_stage.buf[i], src + off, len);
-        if ((rc = t_stage.dma(i, dst + off, t_stage.buf[i], len, hipMemcpyHostToDevice, s)))
-            return rc;
+        if ((rc = st.settle(i))) return rc;
+        host_copy(st.buf[i], src + off, len);
+        if ((rc = st.dma(i, dst + off, st.buf[i], len, hipMemcpyHostToDevice, s))) return rc;
     }
     return MPD_OK;
 }
@@ -188,24 +227,26 @@ static int d2h(double* h, const double* d, size_t n)
         return MPD_OK;
     }
     int rc;
-    if ((rc = t_stage.init())) return rc;
+    StageLease lease;
+    if ((rc = lease.acquire())) return rc;
+    Staging& st = *lease.s;
     const char* src = (const char*)d;
     char* dst = (char*)h;
     const size_t nch = (bytes + kStageChunk - 1) / kStageChunk;
     auto issue = [&](size_t k) {
         const int i = (int)(k & 1);
         const size_t off = k * kStageChunk, len = std::min(kStageChunk, bytes - off);
-        int r = t_stage.settle(i);
-        return r ? r : t_stage.dma(i, t_stage.buf[i], src + off, len, hipMemcpyDeviceToHost, s);
+        int r = st.settle(i);
+        return r ? r : st.dma(i, st.buf[i], src + off, len, hipMemcpyDeviceToHost, s);
     };
     if ((rc = issue(0))) return rc;
     for (size_t k = 0; k < nch; k++) {
         // the next chunk's DMA runs while this chunk is copied out on the host
         if (k + 1 < nch && (rc = issue(k + 1))) return rc;
         const int i = (int)(k & 1);
-        if ((rc = t_stage.settle(i))) return rc;
+        if ((rc = st.settle(i))) return rc;
         const size_t off = k * kStageChunk;
-        host_copy(dst + off, t_stage.buf[i], std::min(kStageChunk, bytes - off));
+        host_copy(dst + off, st.buf[i], std::min(kStageChunk, bytes - off));
     }
     return MPD_OK;
 }
@@ -252,13 +293,56 @@ using namespace mpd;
 // ------------------------------------------------------------------ drop-in ABI ---
 
 namespace {
-// Per-thread device state of smolar(): reused while the grid shape and order repeat,
-// so repeated calls do not allocate (the reference mallocs per call, uu.c:115).
+// Device state of smolar(), leased per call from a pool per device: a call takes a free
+// state of its grid shape and order when there is one (no allocation: the reference
+// mallocs per call, uu.c:115), else any free state, reinitialised, else a new one.
+// Concurrent calls from several threads (smolar() is reentrant) hold states of their
+// own.  States are kept until the process ends, so their number is bounded by the calls
+// that ran at the same time, not by the threads that ever called.
 struct SmolarCache {
+    SmolarCache* next = nullptr;  // free list of the pool
     mpd_grid_s g;
     DevBuf sp[3], wd[6];
 };
-thread_local SmolarCache* t_cache = nullptr;
+constexpr int kCacheDevices = 64;
+std::mutex g_cache_mu;
+SmolarCache* g_cache_free[kCacheDevices] = {};
+
+struct CacheLease {
+    SmolarCache* c = nullptr;
+    int dev = 0;
+    int acquire(int nx, int ny, int ns, int order)
+    {
+        MPD_HIP(hipGetDevice(&dev));
+        if (dev < 0 || dev >= kCacheDevices)
+            return fail(MPD_ERR_UNSUPPORTED, "smolar: device %d beyond %d", dev, kCacheDevices);
+        {
+            std::lock_guard<std::mutex> lk(g_cache_mu);
+            SmolarCache** at = &g_cache_free[dev];
+            for (SmolarCache** q = at; *q; q = &(*q)->next) {
+                const mpd_grid_s& g = (*q)->g;
+                if (g.nx == nx && g.ny == ny && g.nz == ns && g.iord == order) {
+                    at = q;
+                    break;
+                }
+            }
+            c = *at;
+            if (c) *at = c->next;
+        }
+        if (!c) c = new SmolarCache;
+        return MPD_OK;
+    }
+    ~CacheLease()
+    {
+        if (!c) return;
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        c->next = g_cache_free[dev];
+        g_cache_free[dev] = c;
+    }
+    CacheLease() = default;
+    CacheLease(const CacheLease&) = delete;
+    CacheLease& operator=(const CacheLease&) = delete;
+};
 }  // namespace
 
 extern "C" int smolar(int nx, int ny, int ns, double* dx, double* dy, double* ds,
@@ -273,16 +357,18 @@ extern "C" int smolar(int nx, int ny, int ns, double* dx, double* dy, double* ds
         !*c1)
         return fail(MPD_ERR_ARG, "smolar: NULL array argument");
     if (order < 1) return fail(MPD_ERR_ARG, "smolar: order %d < 1", order);
-    if (!t_cache) t_cache = new SmolarCache;
-    mpd_grid_s& g = t_cache->g;
+    CacheLease lease;
     int rc;
+    if ((rc = lease.acquire(nx, ny, ns, order))) return rc;
+    SmolarCache* cache = lease.c;
+    mpd_grid_s& g = cache->g;
     if (g.nx != nx || g.ny != ny || g.nz != ns || g.iord != order || !g.c[0].p) {
-        t_cache->g.~mpd_grid_s();
-        new (&t_cache->g) mpd_grid_s();
+        cache->g.~mpd_grid_s();
+        new (&cache->g) mpd_grid_s();
         if ((rc = grid_init(&g, nx, ny, ns, MPD_FRAME, order, 0)) != MPD_OK) return rc;
     }
     const double* w[6] = {u0, u1, v0, v1, w0, w1};
-    if ((rc = winds_to_courant(&g, dx, dy, ds, dt, w, t_cache->sp, t_cache->wd))) return rc;
+    if ((rc = winds_to_courant(&g, dx, dy, ds, dt, w, cache->sp, cache->wd))) return rc;
     double* h0 = *c0;
     double* h1 = *c1;
     if ((rc = h2d(g.c[0].p, h0, g.ncell))) return rc;

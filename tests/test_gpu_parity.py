@@ -89,6 +89,68 @@ def test_smolar_large_arrays_through_the_staged_copies():
             o0, o1 = o1, o0
 
 
+def _device_free_bytes():
+    """Free device memory (hipMemGetInfo of the HIP runtime the library loaded)."""
+    import ctypes as C
+    with open("/proc/self/maps") as f:
+        path = next(line.split()[-1] for line in f if "libamdhip64.so" in line)
+    hip = C.CDLL(path)
+    free, total = C.c_size_t(), C.c_size_t()
+    assert hip.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
+    return free.value
+
+
+def test_smolar_from_concurrent_and_short_lived_threads():
+    """smolar() is reentrant: three threads calling it at once, each with host arrays
+    large enough for the staged copies, get the oracle's fields bit for bit; and threads
+    that call once and end keep no device memory each (the device state and the staging
+    buffers are leased from pools per device, which grow only with concurrent calls)."""
+    import threading
+    nx, ny, ns = 112, 110, 96
+    n = nx * ny * ns
+    rng = np.random.default_rng(5)
+    u, v, w = (rng.uniform(-0.25, 0.25, n) for _ in range(3))
+    sp = [rng.uniform(0.9, 1.1, m) for m in (nx, ny, ns)]
+    cs = [rng.uniform(0.0, 1.0, n) for _ in range(3)]
+    want = []
+    for c in cs:
+        _, o0, o1 = O.call_smolar(O.oracle().orc_smolar, nx, ny, ns, *sp, 0.8, u, u, v, v,
+                                  w, w, c.copy(), np.zeros(n), 2)
+        want.append((o0, o1))
+    got, errors = [None] * 3, []
+
+    def run(i):
+        try:
+            got[i] = M.smolar(nx, ny, ns, *sp, 0.8, u, u, v, v, w, w, cs[i].copy(),
+                              np.zeros(n), order=2)
+        except Exception as e:  # reported below, in the test's thread
+            errors.append(e)
+
+    print("case concurrent smolar: 3 threads", flush=True)
+    threads = [threading.Thread(target=run, args=(i,)) for i in range(3)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for (g0, g1), (o0, o1) in zip(got, want):
+        np.testing.assert_array_equal(g0, o0)
+        np.testing.assert_array_equal(g1, o1)
+    free0 = _device_free_bytes()
+    for k in range(6):
+        print(f"case short-lived smolar thread {k}", flush=True)
+        t = threading.Thread(target=run, args=(k % 3,))
+        t.start()
+        t.join()
+        assert not errors, errors
+    # one state of this grid holds ~15 arrays of 9.5 MB; six threads that each kept one
+    # would take ~850 MB
+    assert free0 - _device_free_bytes() < 64 << 20
+    for (g0, g1), (o0, o1) in zip(got, want):
+        np.testing.assert_array_equal(g0, o0)
+        np.testing.assert_array_equal(g1, o1)
+
+
 def test_ring2d_trajectory_matches_reference():
     d = dict(np.load(os.path.join(GOLD, "ring2d_32.npz")))
     n, _, m = (int(x) for x in d["dims"])
