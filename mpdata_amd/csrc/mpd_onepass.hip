@@ -644,15 +644,15 @@ int launch_f12(const FusedArgs& a, const Work& w, int grid, bool donor, bool emi
 // rounds for 1.1 rounds of work: 0.279 ms per step against 0.24 on two kernels per pass.)
 int choose_zc_f12(const mpd_grid_s* g)
 {
-    static int cus = 0;
-    if (!cus) {
+    // the CU count, queried once (a thread-safe static initialisation)
+    static const int cus = [] {
         int dev = 0, n = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
-            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
-            cus = n;
-        else
-            cus = 256;
-    }
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            n > 0)
+            return n;
+        return 256;
+    }();
     const long long cols =
         (long long)((g->nx + F12_TX - 1) / F12_TX) * ((g->ny + F12_TY - 1) / F12_TY);
     int best = g->nz;
