@@ -954,20 +954,26 @@ __device__ __forceinline__ FaceK face_k(double L, double a, double p, double m, 
                                         double s2, double u2a, double u2b, double d2a,
                                         double d2b)
 {
+    // With the sums U = up pair, Dn = down pair of a ratio term, n = U - Dn and
+    // D = U + Dn + MINV: D - n = 2 Dn + MINV and -D - n = -(2 U + MINV), formed directly
+    // (no cancellation), and the factors L K and L (-a/2) s shared by each pair of
+    // coefficients; every (x r) r keeps 1/den^2 from overflowing at den = MINV.
     FaceK k;
-    const double K = ab(a) * (1.0 - ab(a));
-    const double r0 = recip(p + m + MINV);  // (x r0) r0: no underflow of den^2
-    k.kp = L * (K * (((2.0 * m + MINV) * r0) * r0));
-    k.km = L * (-K * (((2.0 * p + MINV) * r0) * r0));
-    const double gb = -0.5 * a;
-    const double n1 = u1a + u1b - d1a - d1b, D1 = u1a + u1b + d1a + d1b + MINV;
-    const double r1 = recip(D1);
-    k.ku1 = L * (gb * s1 * (((D1 - n1) * r1) * r1));
-    k.kd1 = L * (gb * s1 * (((-D1 - n1) * r1) * r1));
-    const double n2 = u2a + u2b - d2a - d2b, D2 = u2a + u2b + d2a + d2b + MINV;
-    const double r2 = recip(D2);
-    k.ku2 = L * (gb * s2 * (((D2 - n2) * r2) * r2));
-    k.kd2 = L * (gb * s2 * (((-D2 - n2) * r2) * r2));
+    const double LK = L * (ab(a) * (1.0 - ab(a)));
+    const double r0 = recip(p + m + MINV);
+    k.kp = LK * (((2.0 * m + MINV) * r0) * r0);
+    k.km = -LK * (((2.0 * p + MINV) * r0) * r0);
+    const double Lg = L * (-0.5 * a);
+    const double U1 = u1a + u1b, Dn1 = d1a + d1b;
+    const double r1 = recip((U1 + Dn1) + MINV);
+    const double w1 = Lg * s1;
+    k.ku1 = w1 * (((2.0 * Dn1 + MINV) * r1) * r1);
+    k.kd1 = -w1 * (((2.0 * U1 + MINV) * r1) * r1);
+    const double U2 = u2a + u2b, Dn2 = d2a + d2b;
+    const double r2 = recip((U2 + Dn2) + MINV);
+    const double w2 = Lg * s2;
+    k.ku2 = w2 * (((2.0 * Dn2 + MINV) * r2) * r2);
+    k.kd2 = -w2 * (((2.0 * U2 + MINV) * r2) * r2);
     return k;
 }
 
